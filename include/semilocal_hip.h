@@ -1,0 +1,115 @@
+/*
+ * semilocal_hip.h -- C ABI of the MI355X semi-local LCS engine.
+ *
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ * Every entry point returns an int status (SL_OK = 0) and leaves a message for
+ * sl_last_error() on failure; nothing throws across the ABI.  Calls are
+ * thread-safe: each handle owns its HIP stream, and the one-shot entry points
+ * serialise on an internal lock.
+ *
+ * Result convention (identical to the reference's AbstractPermutation /
+ * common::Permutation filled by the combing solvers): for strings a (length
+ * m, rows) and b (length n, columns), `row_to_col[s]` is the exit position of
+ * strand s among the m+n strands: positions 0..n-1 are bottom-edge exits at
+ * column j, positions n..n+m-1 are right-edge exits at row m-1-(pos-n).
+ * `col_to_row` is its inverse.  LLCS(a, b) = m - #{s < m : row_to_col[s] >= n}.
+ */
+#ifndef SEMILOCAL_HIP_H
+#define SEMILOCAL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SL_OK 0
+#define SL_ERR_ARG 1
+#define SL_ERR_HIP 2
+#define SL_ERR_NOMEM 3
+#define SL_ERR_TIMEOUT 4
+#define SL_ERR_UNSUPPORTED 5
+
+/* Symbol element width accepted by the entry points that take `sym_bytes`:
+ * 1 = uint8_t symbols, 2 = uint16_t, 4 = int32_t (remapped internally to a
+ * dense alphabet; at most 65535 distinct symbols per call). */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int sl_version(void);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *sl_last_error(void);
+
+/* Selects the HIP device used by later calls on this thread and creates the
+ * library's streams.  Optional: every entry point initialises device 0 (or the
+ * current device) lazily. */
+int sl_init(int device);
+
+/* Writes "name;compute_units;clock_khz;hbm_bytes" into buf. */
+int sl_device_info(char *buf, int len);
+
+/* ------------------------------------------------------------------------ */
+/* Plain semi-local LCS of one pair (anti-diagonal sticky-braid combing).     */
+/* Replaces: semi_local::sticky_braid_mpi<Input,WithIf,WithWait>(matrix, a,   */
+/*   a_size, b, b_size, threads_num, is_reverse)  -- semi_local.h:212-214;    */
+/*   semi_local::sticky_braid_sequential -- semi_local.h:135-137;             */
+/*   semi_local::OpenMPIterativeCombing::compute -- SemiLocalLCS.h:238-244;   */
+/*   semi_local::SimpleIterativeCombing::compute -- SemiLocalLCS.h:68.         */
+/* Host buffers; col_to_row may be NULL.  m or n may be 0.                    */
+/* ------------------------------------------------------------------------ */
+int sl_semi_local_comb(const void *a, int m, const void *b, int n, int sym_bytes,
+                       int32_t *row_to_col, int32_t *col_to_row);
+
+/* LLCS read from a kernel: replaces lcs::LCSBySemiLocal::compute's count,
+ * include/LCS/LCS.h:223-233.  Pure host arithmetic over row_to_col. */
+int sl_llcs_from_kernel(const int32_t *row_to_col, int m, int n);
+
+/* ------------------------------------------------------------------------ */
+/* Ragged batch: thousands of independent pairs combed in ONE persistent      */
+/* launch fed by a device queue of (pair, row-pass) tiles.  No reference      */
+/* counterpart (the reference combs one pair per call); the per-pair results */
+/* are exactly those of sl_semi_local_comb on each pair.                      */
+/* Inputs are concatenated: pair i's a is a_cat[a_off[i] .. a_off[i]+a_len[i])*/
+/* in elements of sym_bytes.  Outputs are concatenated in caller order: pair  */
+/* i's row_to_col occupies (a_len[i]+b_len[i]) entries starting at the sum of */
+/* the previous pairs' sizes.                                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct sl_batch sl_batch;
+
+#define SL_BATCH_WITH_COL_TO_ROW 1   /* also produce col_to_row */
+
+/* Uploads the batch to HBM and prepares the queue.  `rows_per_lane` = 0
+ * lets the built-in length-bucket table choose tile heights; 1,2,4,8,16 forces
+ * one value for every pair. */
+int sl_batch_create(sl_batch **out, int npairs, const void *a_cat, const int64_t *a_off,
+                    const int32_t *a_len, const void *b_cat, const int64_t *b_off,
+                    const int32_t *b_len, int sym_bytes, int flags, int rows_per_lane);
+/* Combs every pair in one persistent launch (asynchronous on the batch's stream). */
+int sl_batch_run(sl_batch *bt);
+/* Combs the same pairs with one launch per pair, in caller order (the plain
+ * one-pair-per-launch form used as the comparison point). */
+int sl_batch_run_per_pair(sl_batch *bt);
+/* Waits for the batch's stream and reports a kernel-side timeout, if any. */
+int sl_batch_sync(sl_batch *bt);
+/* Copies results to host (after sync).  col_to_row_cat may be NULL. */
+int sl_batch_download(sl_batch *bt, int32_t *row_to_col_cat, int32_t *col_to_row_cat);
+/* Times `iters` runs after `warmup` runs with HIP events on the batch's
+ * stream.  mode 0 = persistent batched launch, 1 = one launch per pair.
+ * kernel_ms_out (may be NULL) receives the combing kernels' own time per run
+ * (events around the kernel launches only). */
+int sl_batch_time(sl_batch *bt, int mode, int warmup, int iters, double *ms_per_run,
+                  double *kernel_ms_out);
+/* Sum of m*n over the batch (cells), number of queue tiles, number of pairs. */
+int sl_batch_stats(const sl_batch *bt, int64_t *cells, int64_t *tiles, int64_t *pairs);
+void sl_batch_destroy(sl_batch *bt);
+
+/* One-shot convenience: create + run + sync + download + destroy. */
+int sl_comb_batch(int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                  const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes,
+                  int32_t *row_to_col_cat, int32_t *col_to_row_cat);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEMILOCAL_HIP_H */

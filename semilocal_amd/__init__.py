@@ -1,0 +1,18 @@
+"""semilocal_amd -- MI355X-native semi-local LCS engine.
+
+The product is libsemilocal_hip.so (hand-written HIP kernels for gfx950 behind
+the C ABI in include/semilocal_hip.h, plus the C++ drop-in strategies in
+include/semilocal_hip.hpp).  This package builds it and exposes a thin
+Python binding used by the tests and the benchmark.
+"""
+from ._lib import (Batch, SemiLocalError, comb_batch, device_info, lib, semi_local_comb)
+
+__all__ = ["Batch", "SemiLocalError", "comb_batch", "device_info", "lib", "semi_local_comb", "llcs_from_kernel"]
+
+
+def llcs_from_kernel(perm, m: int, n: int) -> int:
+    """LLCS = m - #{s < m : perm[s] >= n}  (reference include/LCS/LCS.h:223-233)."""
+    import numpy as np
+
+    perm = np.asarray(perm)
+    return int(m - np.count_nonzero(perm[:m] >= n))

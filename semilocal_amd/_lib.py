@@ -1,0 +1,208 @@
+"""ctypes binding of libsemilocal_hip.so (the C ABI in include/semilocal_hip.h).
+
+The library is required: importing this module on a machine where it has not
+been built raises immediately -- there is no CPU fallback anywhere in the
+product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsemilocal_hip.so")
+
+SL_OK = 0
+SL_BATCH_WITH_COL_TO_ROW = 1
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+class SemiLocalError(RuntimeError):
+    """A non-zero status from the C ABI; .code holds it."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"semilocal_hip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+# Exported entry points and their argtypes/restype (checked by the CPU tests).
+SIGNATURES = {
+    "sl_version": ([], C.c_int),
+    "sl_last_error": ([], C.c_char_p),
+    "sl_init": ([C.c_int], C.c_int),
+    "sl_device_info": ([C.c_char_p, C.c_int], C.c_int),
+    "sl_semi_local_comb": ([_vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "sl_llcs_from_kernel": ([_i32p, C.c_int, C.c_int], C.c_int),
+    "sl_batch_create": ([C.POINTER(_vp), C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, C.c_int, C.c_int],
+                        C.c_int),
+    "sl_batch_run": ([_vp], C.c_int),
+    "sl_batch_run_per_pair": ([_vp], C.c_int),
+    "sl_batch_sync": ([_vp], C.c_int),
+    "sl_batch_download": ([_vp, _vp, _vp], C.c_int),
+    "sl_batch_time": ([_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "sl_batch_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "sl_batch_destroy": ([_vp], None),
+    "sl_comb_batch": ([C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, _vp, _vp], C.c_int),
+}
+
+
+def lib():
+    """Load the engine library (once).  Raises if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with `python -m semilocal_amd.build` "
+                    "(there is no CPU fallback)")
+            L = C.CDLL(LIB_PATH)
+            for name, (argtypes, restype) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.argtypes = argtypes
+                fn.restype = restype
+            _lib = L
+        return _lib
+
+
+def check(status: int) -> None:
+    if status != SL_OK:
+        raise SemiLocalError(status, lib().sl_last_error().decode(errors="replace"))
+
+
+def _ptr(x: np.ndarray):
+    return x.ctypes.data_as(_vp)
+
+
+def as_symbols(x) -> tuple[np.ndarray, int]:
+    """Contiguous symbol array and its element width (1, 2 or 4 bytes)."""
+    arr = np.asarray(x)
+    if arr.dtype == np.uint8:
+        return np.ascontiguousarray(arr), 1
+    if arr.dtype == np.uint16:
+        return np.ascontiguousarray(arr), 2
+    return np.ascontiguousarray(arr, dtype=np.int32), 4
+
+
+def device_info() -> dict:
+    buf = C.create_string_buffer(512)
+    check(lib().sl_device_info(buf, 512))
+    name, cus, clk, hbm = buf.value.decode().split(";")
+    return {"name": name, "compute_units": int(cus), "clock_khz": int(clk), "hbm_bytes": int(hbm)}
+
+
+def semi_local_comb(a, b, with_col_to_row: bool = False):
+    """Kernel permutation (row->col, int32[m+n]) of strings a (rows), b (cols)."""
+    a_arr, wa = as_symbols(a)
+    b_arr, wb = as_symbols(b)
+    if wa != wb:
+        a_arr, wa = np.ascontiguousarray(a_arr, dtype=np.int32), 4
+        b_arr, wb = np.ascontiguousarray(b_arr, dtype=np.int32), 4
+    m, n = len(a_arr), len(b_arr)
+    rc = np.empty(max(m + n, 1), np.int32)
+    cr = np.empty(max(m + n, 1), np.int32) if with_col_to_row else None
+    check(lib().sl_semi_local_comb(_ptr(a_arr) if m else None, m, _ptr(b_arr) if n else None, n, wa,
+                                   _ptr(rc), _ptr(cr) if cr is not None else None))
+    rc = rc[:m + n]
+    return (rc, cr[:m + n]) if with_col_to_row else rc
+
+
+def _concat(pairs):
+    widths = set()
+    arrs = []
+    for a, b in pairs:
+        a_arr, wa = as_symbols(a)
+        b_arr, wb = as_symbols(b)
+        widths.update((wa, wb))
+        arrs.append((a_arr, b_arr))
+    width = 1 if widths <= {1} else (2 if widths <= {1, 2} else 4)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.int32}[width]
+    a_len = np.array([len(a) for a, _ in arrs], np.int32)
+    b_len = np.array([len(b) for _, b in arrs], np.int32)
+    a_cat = np.concatenate([a.astype(dt, copy=False) for a, _ in arrs]) if arrs else np.zeros(0, dt)
+    b_cat = np.concatenate([b.astype(dt, copy=False) for _, b in arrs]) if arrs else np.zeros(0, dt)
+    a_off = np.zeros(len(arrs), np.int64)
+    b_off = np.zeros(len(arrs), np.int64)
+    if len(arrs) > 1:
+        a_off[1:] = np.cumsum(a_len[:-1], dtype=np.int64)
+        b_off[1:] = np.cumsum(b_len[:-1], dtype=np.int64)
+    if a_cat.size == 0:
+        a_cat = np.zeros(1, dt)
+    if b_cat.size == 0:
+        b_cat = np.zeros(1, dt)
+    return a_cat, a_off, a_len, b_cat, b_off, b_len, width
+
+
+def _split(flat, a_len, b_len):
+    out, s = [], 0
+    for m, n in zip(a_len.tolist(), b_len.tolist()):
+        out.append(flat[s:s + m + n].copy())
+        s += m + n
+    return out
+
+
+class Batch:
+    """A ragged batch resident in HBM (sl_batch_* entry points)."""
+
+    def __init__(self, pairs, with_col_to_row: bool = False, rows_per_lane: int = 0):
+        self.a_cat, self.a_off, self.a_len, self.b_cat, self.b_off, self.b_len, w = _concat(pairs)
+        self.with_cr = with_col_to_row
+        h = _vp()
+        check(lib().sl_batch_create(C.byref(h), len(self.a_len), _ptr(self.a_cat), self.a_off, self.a_len,
+                                    _ptr(self.b_cat), self.b_off, self.b_len, w,
+                                    SL_BATCH_WITH_COL_TO_ROW if with_col_to_row else 0, rows_per_lane))
+        self.h = h
+        self.total = int((self.a_len.astype(np.int64) + self.b_len).sum())
+
+    def run(self, per_pair: bool = False):
+        check(lib().sl_batch_run_per_pair(self.h) if per_pair else lib().sl_batch_run(self.h))
+        check(lib().sl_batch_sync(self.h))
+        return self
+
+    def results(self):
+        rc = np.empty(max(self.total, 1), np.int32)
+        cr = np.empty(max(self.total, 1), np.int32) if self.with_cr else None
+        check(lib().sl_batch_download(self.h, _ptr(rc), _ptr(cr) if cr is not None else None))
+        rcs = _split(rc, self.a_len, self.b_len)
+        if cr is None:
+            return rcs
+        return rcs, _split(cr, self.a_len, self.b_len)
+
+    def time(self, per_pair: bool = False, warmup: int = 1, iters: int = 3):
+        ms = C.c_double()
+        kms = C.c_double()
+        check(lib().sl_batch_time(self.h, 1 if per_pair else 0, warmup, iters, C.byref(ms), C.byref(kms)))
+        return ms.value, kms.value
+
+    def stats(self):
+        c, t, p = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().sl_batch_stats(self.h, C.byref(c), C.byref(t), C.byref(p)))
+        return {"cells": c.value, "tiles": t.value, "pairs": p.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sl_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def comb_batch(pairs, with_col_to_row: bool = False):
+    """One-shot ragged batch: list of row->col arrays (and col->row)."""
+    b = Batch(pairs, with_col_to_row)
+    try:
+        return b.run().results()
+    finally:
+        b.close()

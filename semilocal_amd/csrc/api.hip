@@ -1,0 +1,566 @@
+// api.hip -- host side of the C ABI (include/semilocal_hip.h): device and
+// stream management, ragged-batch planning (orientation, length-bucket tile
+// table, LPT queue order), uploads, launches and timing.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/semilocal_hip.h"
+#include "common.h"
+
+namespace slhip {
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t stream);
+hipError_t launch_comb_batch(int sym_bytes, unsigned waves, hipStream_t stream, const PairDesc *pairs,
+                             const WorkItem *items, uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr,
+                             uint32_t *buf, uint32_t *progress, LaunchCtl *ctl, uint32_t spin_limit);
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+const char *last_error() { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Device state
+// ---------------------------------------------------------------------------
+struct DeviceState {
+    bool ready = false;
+    int device = 0;
+    int cus = 0;
+    int clock_khz = 0;
+    size_t hbm = 0;
+    char name[256] = {0};
+};
+static DeviceState g_dev;
+static std::mutex g_dev_mu;
+std::mutex g_oneshot_mu;
+
+int ensure_device(int device = -1) {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_dev.ready && (device < 0 || device == g_dev.device)) {
+        SL_HIP_TRY(hipSetDevice(g_dev.device));
+        return kOk;
+    }
+    int count = 0;
+    SL_HIP_TRY(hipGetDeviceCount(&count));
+    if (count <= 0) {
+        set_error("no HIP device visible");
+        return kErrHip;
+    }
+    if (device < 0) device = 0;
+    if (device >= count) {
+        set_error("device index out of range");
+        return kErrArg;
+    }
+    SL_HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    SL_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    g_dev.device = device;
+    g_dev.cus = prop.multiProcessorCount;
+    g_dev.clock_khz = prop.clockRate;
+    g_dev.hbm = prop.totalGlobalMem;
+    std::snprintf(g_dev.name, sizeof g_dev.name, "%s", prop.name);
+    g_dev.ready = true;
+    return kOk;
+}
+
+int device_cus() { return g_dev.cus > 0 ? g_dev.cus : 256; }
+
+// ---------------------------------------------------------------------------
+// Alphabet packing: symbols only matter through equality, so any injective
+// relabeling gives the same kernel.  uint8 input is used as is; wider input
+// is remapped to dense codes (uint8 if <= 256 distinct, else uint16).
+// ---------------------------------------------------------------------------
+struct PackedPool {
+    int sym_bytes = 1;
+    std::vector<uint8_t> bytes;   // pool in the chosen width
+};
+
+template <typename T>
+static int pack_pool_from(const T *a_cat, const int64_t *a_off, const int32_t *a_len, const T *b_cat,
+                          const int64_t *b_off, const int32_t *b_len, int npairs, bool identity_u8,
+                          std::vector<int64_t> &a_pos, std::vector<int64_t> &b_pos, PackedPool &out) {
+    int64_t total = 0;
+    for (int i = 0; i < npairs; ++i) total += (int64_t)a_len[i] + b_len[i];
+    a_pos.resize(npairs);
+    b_pos.resize(npairs);
+    if (identity_u8) {
+        out.sym_bytes = 1;
+        out.bytes.resize((size_t)std::max<int64_t>(total, 1));
+        int64_t p = 0;
+        for (int i = 0; i < npairs; ++i) {
+            a_pos[i] = p;
+            std::memcpy(out.bytes.data() + p, (const uint8_t *)a_cat + a_off[i], (size_t)a_len[i]);
+            p += a_len[i];
+            b_pos[i] = p;
+            std::memcpy(out.bytes.data() + p, (const uint8_t *)b_cat + b_off[i], (size_t)b_len[i]);
+            p += b_len[i];
+        }
+        return kOk;
+    }
+    std::unordered_map<int64_t, uint32_t> code;
+    code.reserve(1024);
+    std::vector<uint32_t> tmp((size_t)std::max<int64_t>(total, 1));
+    int64_t p = 0;
+    for (int i = 0; i < npairs; ++i) {
+        for (int side = 0; side < 2; ++side) {
+            const T *src = side == 0 ? a_cat + a_off[i] : b_cat + b_off[i];
+            const int32_t len = side == 0 ? a_len[i] : b_len[i];
+            (side == 0 ? a_pos : b_pos)[i] = p;
+            for (int32_t k = 0; k < len; ++k) {
+                auto it = code.find((int64_t)src[k]);
+                uint32_t c;
+                if (it == code.end()) {
+                    c = (uint32_t)code.size();
+                    code.emplace((int64_t)src[k], c);
+                } else {
+                    c = it->second;
+                }
+                tmp[(size_t)p++] = c;
+            }
+        }
+    }
+    if (code.size() > 65535) {
+        set_error("more than 65535 distinct symbols");
+        return kErrUnsupported;
+    }
+    out.sym_bytes = code.size() <= 256 ? 1 : 2;
+    out.bytes.resize((size_t)std::max<int64_t>(total, 1) * out.sym_bytes);
+    if (out.sym_bytes == 1)
+        for (int64_t k = 0; k < total; ++k) out.bytes[(size_t)k] = (uint8_t)tmp[(size_t)k];
+    else
+        for (int64_t k = 0; k < total; ++k) ((uint16_t *)out.bytes.data())[k] = (uint16_t)tmp[(size_t)k];
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Length-bucket tile table: rows held per lane (R) for a pair with m rows
+// (the shorter string) and n columns.  One pass = 64R rows; the passes of a
+// pair run concurrently, so R trades per-pass overhead (5 DPP moves per step
+// amortised over R cells) against pipeline depth.  Entries are produced by the
+// autotuner (semilocal_amd/autotune.py) and can be replaced at run time.
+// ---------------------------------------------------------------------------
+struct TileRule {
+    int32_t max_m;   // rule applies to pairs with m <= max_m
+    int32_t rows_per_lane;
+};
+static std::vector<TileRule> g_tile_rules = {
+    {64, 1}, {128, 2}, {256, 4}, {1024, 8}, {0x7fffffff, 16},
+};
+static std::mutex g_tile_mu;
+
+int pick_rows_per_lane(int32_t m) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    for (const auto &r : g_tile_rules)
+        if (m <= r.max_m) return r.rows_per_lane;
+    return 16;
+}
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+struct Batch {
+    int npairs = 0;
+    int sym_bytes = 1;
+    bool with_cr = false;
+    std::vector<PairDesc> pairs;       // caller order
+    std::vector<WorkItem> items;       // LPT order, passes of a pair consecutive
+    std::vector<int64_t> perm_off;     // caller order
+    int64_t perm_total = 0;
+    int64_t buf_total = 0;
+    int64_t cells = 0;
+    uint32_t spin_limit = 1u << 23;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    PairDesc *d_pairs = nullptr;
+    WorkItem *d_items = nullptr;
+    void *d_pool = nullptr;
+    int32_t *d_rc = nullptr;
+    int32_t *d_cr = nullptr;
+    uint32_t *d_buf = nullptr;
+    uint32_t *d_progress = nullptr;
+    LaunchCtl *d_ctl = nullptr;    // [0] batched launch, [1+i] per-pair launch of pair i
+    unsigned waves_cap = 0;
+};
+
+static void batch_free(Batch *b) {
+    if (!b) return;
+    (void)hipFree(b->d_pairs);
+    (void)hipFree(b->d_items);
+    (void)hipFree(b->d_pool);
+    (void)hipFree(b->d_rc);
+    (void)hipFree(b->d_cr);
+    (void)hipFree(b->d_buf);
+    (void)hipFree(b->d_progress);
+    (void)hipFree(b->d_ctl);
+    for (auto &e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+template <typename T>
+static hipError_t upload(T **dst, const T *src, size_t count) {
+    hipError_t e = hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(count, 1));
+    if (e != hipSuccess) return e;
+    if (count) e = hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice);
+    return e;
+}
+
+static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                        const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes, int flags,
+                        int rows_per_lane) {
+    if (!out || npairs < 0 || (npairs > 0 && (!a_cat || !a_off || !a_len || !b_cat || !b_off || !b_len))) {
+        set_error("sl_batch_create: null argument");
+        return kErrArg;
+    }
+    if (sym_bytes != 1 && sym_bytes != 2 && sym_bytes != 4) {
+        set_error("sym_bytes must be 1, 2 or 4");
+        return kErrArg;
+    }
+    if (rows_per_lane != 0 && rows_per_lane != 1 && rows_per_lane != 2 && rows_per_lane != 4 && rows_per_lane != 8 &&
+        rows_per_lane != 16) {
+        set_error("rows_per_lane must be 0, 1, 2, 4, 8 or 16");
+        return kErrArg;
+    }
+    for (int i = 0; i < npairs; ++i) {
+        if (a_len[i] < 0 || b_len[i] < 0 || a_off[i] < 0 || b_off[i] < 0) {
+            set_error("negative length or offset");
+            return kErrArg;
+        }
+        if ((int64_t)a_len[i] + b_len[i] > 0x7fffffffLL) {
+            set_error("pair too large: m + n must fit int32");
+            return kErrUnsupported;
+        }
+    }
+    int rc = ensure_device();
+    if (rc) return rc;
+
+    Batch *bt = new Batch();
+    bt->npairs = npairs;
+    bt->with_cr = (flags & SL_BATCH_WITH_COL_TO_ROW) != 0;
+
+    std::vector<int64_t> a_pos, b_pos;
+    PackedPool pool;
+    if (sym_bytes == 1)
+        rc = pack_pool_from<uint8_t>((const uint8_t *)a_cat, a_off, a_len, (const uint8_t *)b_cat, b_off, b_len,
+                                     npairs, true, a_pos, b_pos, pool);
+    else if (sym_bytes == 2)
+        rc = pack_pool_from<uint16_t>((const uint16_t *)a_cat, a_off, a_len, (const uint16_t *)b_cat, b_off, b_len,
+                                      npairs, false, a_pos, b_pos, pool);
+    else
+        rc = pack_pool_from<int32_t>((const int32_t *)a_cat, a_off, a_len, (const int32_t *)b_cat, b_off, b_len,
+                                     npairs, false, a_pos, b_pos, pool);
+    if (rc) {
+        delete bt;
+        return rc;
+    }
+    bt->sym_bytes = pool.sym_bytes;
+
+    // orientation, tile choice, output/buffer offsets (caller order)
+    bt->pairs.resize(npairs);
+    bt->perm_off.resize(npairs);
+    for (int i = 0; i < npairs; ++i) {
+        PairDesc &d = bt->pairs[i];
+        const bool rev = a_len[i] > b_len[i];
+        d.m = rev ? b_len[i] : a_len[i];
+        d.n = rev ? a_len[i] : b_len[i];
+        d.reverse = rev ? 1 : 0;
+        d.rows_off = rev ? b_pos[i] : a_pos[i];
+        d.cols_off = rev ? a_pos[i] : b_pos[i];
+        d.perm_off = bt->perm_total;
+        bt->perm_off[i] = bt->perm_total;
+        bt->perm_total += (int64_t)d.m + d.n;
+        bt->cells += (int64_t)d.m * d.n;
+        d.rows_per_lane = rows_per_lane ? rows_per_lane : pick_rows_per_lane(d.m);
+        const int64_t rpp = 64LL * d.rows_per_lane;
+        d.npasses = (d.m > 0 && d.n > 0) ? (int32_t)((d.m + rpp - 1) / rpp) : 0;
+        d.buf_off = 0;
+        if (d.npasses > 1) {
+            d.buf_off = bt->buf_total;
+            bt->buf_total += d.n;
+        }
+    }
+    // queue order: largest pairs first (longest-processing-time first), passes consecutive
+    std::vector<int> order(npairs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (int64_t)bt->pairs[x].m * bt->pairs[x].n > (int64_t)bt->pairs[y].m * bt->pairs[y].n;
+    });
+    for (int i : order) {
+        PairDesc &d = bt->pairs[i];
+        d.item0 = (int32_t)bt->items.size();
+        for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
+    }
+    bt->waves_cap = (unsigned)device_cus() * 16u;
+
+    hipError_t e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&bt->ev[k]);
+    if (e == hipSuccess) e = upload(&bt->d_pairs, bt->pairs.data(), bt->pairs.size());
+    if (e == hipSuccess) e = upload(&bt->d_items, bt->items.data(), bt->items.size());
+    if (e == hipSuccess) e = upload((uint8_t **)&bt->d_pool, pool.bytes.data(), pool.bytes.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_rc, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
+    if (e == hipSuccess && bt->with_cr)
+        e = hipMalloc((void **)&bt->d_cr, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_buf, sizeof(uint32_t) * std::max<int64_t>(bt->buf_total, 1));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&bt->d_progress, sizeof(uint32_t) * std::max<size_t>(bt->items.size(), 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_ctl, sizeof(LaunchCtl) * (size_t)(npairs + 1));
+    if (e != hipSuccess) {
+        set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
+        batch_free(bt);
+        return e == hipErrorOutOfMemory ? kErrNoMem : kErrHip;
+    }
+    // degenerate pairs (m == 0 or n == 0) have no tiles: their kernel is the identity
+    std::vector<int32_t> ident;
+    for (int i = 0; i < npairs; ++i) {
+        const PairDesc &d = bt->pairs[i];
+        if (d.npasses == 0 && d.m + d.n > 0) {
+            ident.resize((size_t)(d.m + d.n));
+            std::iota(ident.begin(), ident.end(), 0);
+            e = hipMemcpy(bt->d_rc + d.perm_off, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess && bt->with_cr)
+                e = hipMemcpy(bt->d_cr + d.perm_off, ident.data(), sizeof(int32_t) * ident.size(),
+                              hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
+                batch_free(bt);
+                return kErrHip;
+            }
+        }
+    }
+    *out = bt;
+    return kOk;
+}
+
+static int batch_reset(Batch *bt) {
+    SL_HIP_TRY(launch_fill_u32(bt->d_progress, 0u, bt->items.size(), bt->stream));
+    SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
+    return kOk;
+}
+
+static hipError_t launch_items(Batch *bt, const WorkItem *items, uint32_t nitems, LaunchCtl *ctl) {
+    const unsigned waves = std::min<unsigned>(nitems, bt->waves_cap);
+    const void *pool = bt->d_pool;
+    // per-pair pointers are resolved inside the kernel from PairDesc offsets
+    return launch_comb_batch(bt->sym_bytes, waves, bt->stream, bt->d_pairs, items, nitems, pool,
+                             bt->d_rc, bt->d_cr, bt->d_buf, bt->d_progress, ctl, bt->spin_limit);
+}
+
+static int batch_run(Batch *bt) {
+    int rc = batch_reset(bt);
+    if (rc) return rc;
+    SL_HIP_TRY(launch_items(bt, bt->d_items, (uint32_t)bt->items.size(), bt->d_ctl));
+    return kOk;
+}
+
+static int batch_run_per_pair(Batch *bt) {
+    int rc = batch_reset(bt);
+    if (rc) return rc;
+    for (int i = 0; i < bt->npairs; ++i) {
+        const PairDesc &d = bt->pairs[i];
+        if (d.npasses == 0) continue;
+        SL_HIP_TRY(launch_items(bt, bt->d_items + d.item0, (uint32_t)d.npasses, bt->d_ctl + 1 + i));
+    }
+    return kOk;
+}
+
+static int batch_sync(Batch *bt) {
+    SL_HIP_TRY(hipStreamSynchronize(bt->stream));
+    std::vector<LaunchCtl> ctl((size_t)bt->npairs + 1);
+    SL_HIP_TRY(hipMemcpy(ctl.data(), bt->d_ctl, sizeof(LaunchCtl) * ctl.size(), hipMemcpyDeviceToHost));
+    uint32_t err = 0;
+    for (const auto &c : ctl) err |= c.error;
+    if (err & 2u) {
+        set_error("combing kernel: strand id out of range (internal defect); results are void");
+        return kErrHip;
+    }
+    if (err & 1u) {
+        set_error("combing kernel: a bounded wait timed out; results are void");
+        return kErrTimeout;
+    }
+    return kOk;
+}
+
+}  // namespace slhip
+
+using namespace slhip;
+
+extern "C" {
+
+int sl_version(void) { return 100; }
+
+const char *sl_last_error(void) { return last_error(); }
+
+int sl_init(int device) { return ensure_device(device); }
+
+int sl_device_info(char *buf, int len) {
+    if (!buf || len <= 0) {
+        set_error("sl_device_info: null buffer");
+        return kErrArg;
+    }
+    int rc = ensure_device();
+    if (rc) return rc;
+    std::snprintf(buf, (size_t)len, "%s;%d;%d;%zu", g_dev.name, g_dev.cus, g_dev.clock_khz, g_dev.hbm);
+    return kOk;
+}
+
+int sl_llcs_from_kernel(const int32_t *row_to_col, int m, int n) {
+    if (!row_to_col || m < 0 || n < 0) return -1;
+    int acc = 0;
+    for (int s = 0; s < m; ++s)
+        if (row_to_col[s] >= n) acc++;
+    return m - acc;
+}
+
+int sl_batch_create(sl_batch **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                    const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes, int flags,
+                    int rows_per_lane) {
+    Batch *b = nullptr;
+    int rc = batch_create(&b, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes, flags, rows_per_lane);
+    if (rc) return rc;
+    // the opaque handle is the Batch itself
+    *out = reinterpret_cast<sl_batch *>(b);
+    return kOk;
+}
+
+int sl_batch_run(sl_batch *bt) {
+    if (!bt) {
+        set_error("null batch");
+        return kErrArg;
+    }
+    return batch_run(reinterpret_cast<Batch *>(bt));
+}
+
+int sl_batch_run_per_pair(sl_batch *bt) {
+    if (!bt) {
+        set_error("null batch");
+        return kErrArg;
+    }
+    return batch_run_per_pair(reinterpret_cast<Batch *>(bt));
+}
+
+int sl_batch_sync(sl_batch *bt) {
+    if (!bt) {
+        set_error("null batch");
+        return kErrArg;
+    }
+    return batch_sync(reinterpret_cast<Batch *>(bt));
+}
+
+int sl_batch_download(sl_batch *h, int32_t *rc_cat, int32_t *cr_cat) {
+    Batch *bt = reinterpret_cast<Batch *>(h);
+    if (!bt || !rc_cat) {
+        set_error("sl_batch_download: null argument");
+        return kErrArg;
+    }
+    if (cr_cat && !bt->with_cr) {
+        set_error("batch was created without SL_BATCH_WITH_COL_TO_ROW");
+        return kErrArg;
+    }
+    if (bt->perm_total) {
+        SL_HIP_TRY(hipMemcpy(rc_cat, bt->d_rc, sizeof(int32_t) * bt->perm_total, hipMemcpyDeviceToHost));
+        if (cr_cat) SL_HIP_TRY(hipMemcpy(cr_cat, bt->d_cr, sizeof(int32_t) * bt->perm_total, hipMemcpyDeviceToHost));
+    }
+    return kOk;
+}
+
+int sl_batch_time(sl_batch *h, int mode, int warmup, int iters, double *ms_per_run, double *kernel_ms_out) {
+    Batch *bt = reinterpret_cast<Batch *>(h);
+    if (!bt || !ms_per_run || iters <= 0 || warmup < 0) {
+        set_error("sl_batch_time: bad argument");
+        return kErrArg;
+    }
+    auto run = [&]() { return mode == 0 ? batch_run(bt) : batch_run_per_pair(bt); };
+    for (int w = 0; w < warmup; ++w) {
+        int rc = run();
+        if (rc) return rc;
+    }
+    int rc = batch_sync(bt);
+    if (rc) return rc;
+    SL_HIP_TRY(hipEventRecord(bt->ev[0], bt->stream));
+    for (int it = 0; it < iters; ++it) {
+        rc = run();
+        if (rc) return rc;
+    }
+    SL_HIP_TRY(hipEventRecord(bt->ev[1], bt->stream));
+    rc = batch_sync(bt);
+    if (rc) return rc;
+    float ms = 0.f;
+    SL_HIP_TRY(hipEventElapsedTime(&ms, bt->ev[0], bt->ev[1]));
+    *ms_per_run = ms / iters;
+    if (kernel_ms_out) {
+        // kernels only: reset outside the bracket, events around the launches
+        double tot = 0.0;
+        for (int it = 0; it < iters; ++it) {
+            rc = batch_reset(bt);
+            if (rc) return rc;
+            SL_HIP_TRY(hipEventRecord(bt->ev[2], bt->stream));
+            if (mode == 0) {
+                SL_HIP_TRY(launch_items(bt, bt->d_items, (uint32_t)bt->items.size(), bt->d_ctl));
+            } else {
+                for (int i = 0; i < bt->npairs; ++i) {
+                    const PairDesc &d = bt->pairs[i];
+                    if (d.npasses == 0) continue;
+                    SL_HIP_TRY(launch_items(bt, bt->d_items + d.item0, (uint32_t)d.npasses, bt->d_ctl + 1 + i));
+                }
+            }
+            SL_HIP_TRY(hipEventRecord(bt->ev[3], bt->stream));
+            rc = batch_sync(bt);
+            if (rc) return rc;
+            float kms = 0.f;
+            SL_HIP_TRY(hipEventElapsedTime(&kms, bt->ev[2], bt->ev[3]));
+            tot += kms;
+        }
+        *kernel_ms_out = tot / iters;
+    }
+    return kOk;
+}
+
+int sl_batch_stats(const sl_batch *h, int64_t *cells, int64_t *tiles, int64_t *pairs) {
+    const Batch *bt = reinterpret_cast<const Batch *>(h);
+    if (!bt) {
+        set_error("null batch");
+        return kErrArg;
+    }
+    if (cells) *cells = bt->cells;
+    if (tiles) *tiles = (int64_t)bt->items.size();
+    if (pairs) *pairs = bt->npairs;
+    return kOk;
+}
+
+void sl_batch_destroy(sl_batch *h) { batch_free(reinterpret_cast<Batch *>(h)); }
+
+int sl_comb_batch(int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len, const void *b_cat,
+                  const int64_t *b_off, const int32_t *b_len, int sym_bytes, int32_t *rc_cat, int32_t *cr_cat) {
+    if (!rc_cat && npairs > 0) {
+        set_error("sl_comb_batch: null output");
+        return kErrArg;
+    }
+    Batch *bt = nullptr;
+    int rc = batch_create(&bt, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes,
+                          cr_cat ? SL_BATCH_WITH_COL_TO_ROW : 0, 0);
+    if (rc) return rc;
+    rc = batch_run(bt);
+    if (!rc) rc = batch_sync(bt);
+    if (!rc) rc = sl_batch_download(reinterpret_cast<sl_batch *>(bt), rc_cat, cr_cat);
+    batch_free(bt);
+    return rc;
+}
+
+int sl_semi_local_comb(const void *a, int m, const void *b, int n, int sym_bytes, int32_t *row_to_col,
+                       int32_t *col_to_row) {
+    if (m < 0 || n < 0 || (m > 0 && !a) || (n > 0 && !b) || (!row_to_col && m + n > 0)) {
+        set_error("sl_semi_local_comb: bad argument");
+        return kErrArg;
+    }
+    static const uint8_t dummy = 0;
+    const int64_t zero = 0;
+    const int32_t ml = m, nl = n;
+    return sl_comb_batch(1, a ? a : &dummy, &zero, &ml, b ? b : &dummy, &zero, &nl, sym_bytes, row_to_col,
+                         col_to_row);
+}
+
+}  // extern "C"

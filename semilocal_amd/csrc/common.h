@@ -1,0 +1,60 @@
+// common.h -- shared host/device definitions of the semi-local LCS engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace slhip {
+
+// Status codes of the C ABI (mirrored in include/semilocal_hip.h).
+enum Status : int {
+    kOk = 0,
+    kErrArg = 1,        // bad argument (null pointer, negative size, ...)
+    kErrHip = 2,        // a HIP runtime call failed
+    kErrNoMem = 3,      // device allocation failed / admission refused
+    kErrTimeout = 4,    // a kernel's bounded wait ran out (reported by the kernel)
+    kErrUnsupported = 5 // input outside what the engine supports (e.g. > 65535 symbols)
+};
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+#define SL_HIP_TRY(expr)                                                              \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) {                                                       \
+            ::slhip::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));    \
+            return ::slhip::kErrHip;                                                  \
+        }                                                                             \
+    } while (0)
+
+// One pair of the ragged batch after orientation: the kernel always combs
+// rows = the shorter string (length m) against cols = the longer (length n),
+// and `reverse` records whether that swapped the caller's (a, b).
+struct PairDesc {
+    int64_t rows_off;   // offset of the row string in the symbol pool
+    int64_t cols_off;   // offset of the column string in the symbol pool
+    int64_t perm_off;   // offset of this pair's (m+n) outputs
+    int64_t buf_off;    // offset of this pair's inter-pass strand buffer (n entries)
+    int32_t m, n;       // rows, cols (m <= n)
+    int32_t reverse;    // 1 if the caller's a was the longer string
+    int32_t item0;      // queue index of pass 0 of this pair
+    int32_t npasses;    // ceil(m / rows_per_pass)
+    int32_t rows_per_lane;  // R: rows held by one lane (rows_per_pass = 64 R)
+};
+
+// Queue entry: (pair, pass).  Entries of one pair are consecutive and in pass
+// order, so a pass only ever waits on an entry handed out before it.
+struct WorkItem {
+    int32_t pair;
+    int32_t pass;
+};
+
+// Device-side control block of one launch.
+struct LaunchCtl {
+    uint32_t ticket;    // next queue entry to hand out
+    uint32_t error;     // bit 0: a bounded wait ran out; bit 1: a strand id failed the bounds check
+};
+
+}  // namespace slhip

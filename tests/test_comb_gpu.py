@@ -1,0 +1,100 @@
+"""GPU parity of kernel (a) (anti-diagonal sticky-braid combing) against the
+reference's golden vectors and the oracle, through the C ABI.  Bar: bit-exact
+row->col permutations (and col->row where requested)."""
+import numpy as np
+import pytest
+
+import golden_io as G
+from semilocal_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_golden_sweep_single_pair(gpu):
+    for a, b, perm, _ in G.comb_sweep():
+        got = gpu.semi_local_comb(a, b)
+        assert np.array_equal(got, perm), (len(a), len(b))
+
+
+def test_golden_sweep_as_one_batch(gpu):
+    cases = G.comb_sweep()
+    res = gpu.comb_batch([(a.astype(np.uint8), b.astype(np.uint8)) for a, b, _, _ in cases])
+    for (a, b, perm, _), got in zip(cases, res):
+        assert np.array_equal(got, perm), (len(a), len(b))
+
+
+@pytest.mark.parametrize("rows_per_lane", [1, 2, 4, 8, 16])
+def test_tile_height_does_not_change_result(gpu, oracle, rows_per_lane):
+    g = np.random.default_rng(rows_per_lane)
+    pairs = [(g.integers(0, 4, int(g.integers(1, 3000))).astype(np.uint8),
+              g.integers(0, 4, int(g.integers(1, 3000))).astype(np.uint8)) for _ in range(12)]
+    b = gpu.Batch(pairs, rows_per_lane=rows_per_lane)
+    res = b.run().results()
+    for (x, y), got in zip(pairs, res):
+        assert np.array_equal(got, oracle.comb(x, y))
+    res2 = b.run(per_pair=True).results()
+    for r1, r2 in zip(res, res2):
+        assert np.array_equal(r1, r2)
+    b.close()
+
+
+def test_edge_shapes(gpu, oracle):
+    g = np.random.default_rng(11)
+    shapes = [(0, 0), (0, 5), (5, 0), (1, 1), (1, 1000), (1000, 1), (63, 64), (64, 63), (1023, 1025),
+              (1025, 1023), (2049, 100), (100, 2049)]
+    for m, n in shapes:
+        a = g.integers(0, 3, m).astype(np.uint8)
+        b = g.integers(0, 3, n).astype(np.uint8)
+        got = gpu.semi_local_comb(a, b)
+        assert np.array_equal(got, oracle.comb(a, b)) if m and n else np.array_equal(got, np.arange(m + n))
+
+
+def test_symbol_widths_and_remapping(gpu, oracle):
+    g = np.random.default_rng(12)
+    a = g.integers(-1000, 1000, 700).astype(np.int32)      # > 256 distinct symbols -> uint16 pool
+    b = np.concatenate([a[100:500], g.integers(-1000, 1000, 300).astype(np.int32)])
+    assert np.array_equal(gpu.semi_local_comb(a, b), oracle.comb(a, b))
+    a8 = g.integers(0, 256, 900).astype(np.uint8)          # all byte values incl. 255
+    b8 = g.integers(0, 256, 1300).astype(np.uint8)
+    assert np.array_equal(gpu.semi_local_comb(a8, b8), oracle.comb(a8, b8))
+
+
+def test_col_to_row_is_inverse(gpu):
+    g = np.random.default_rng(13)
+    a, b = g.integers(0, 4, 1500).astype(np.uint8), g.integers(0, 4, 900).astype(np.uint8)
+    rc, cr = gpu.semi_local_comb(a, b, with_col_to_row=True)
+    assert np.array_equal(cr[rc], np.arange(len(rc)))
+
+
+def test_reference_sweep_naive_vs_openmp_shapes(gpu, oracle):
+    # the shapes of SemiLocalLCSTest.NaiveVsOpenMPCorrectness (4900..4999 step 11, sigma 11)
+    g = np.random.default_rng(14)
+    pairs = []
+    for ma in range(4900, 5000, 33):
+        for nb in range(4900, 5000, 33):
+            pairs.append((g.integers(0, 11, ma).astype(np.uint8), g.integers(0, 11, nb).astype(np.uint8)))
+    res = gpu.comb_batch(pairs)
+    for (x, y), got in zip(pairs, res):
+        assert np.array_equal(got, oracle.comb(x, y))
+
+
+def test_plain_pair_10k_matches_reference_fixture(gpu):
+    c = G.configs()["plain_pair_10k"]
+    a, b = W.plain_pair()
+    assert W.sha256(a, b) == c["input_sha256"]
+    p = gpu.semi_local_comb(a, b)
+    assert W.sha256(p) == c["perm_sha256"]
+    assert gpu.llcs_from_kernel(p, len(a), len(b)) == c["llcs"]
+
+
+def test_ragged_batch_4096_matches_reference_fixture(gpu):
+    c = G.configs()["ragged_batch_4096"]
+    pairs = W.ragged_batch()
+    b = gpu.Batch(pairs)
+    res = b.run().results()
+    bad = [i for i, p in enumerate(res) if W.sha256(p) != c["perm_sha256_each"][i]]
+    assert not bad, bad[:10]
+    # the same pairs, one launch per pair: identical
+    res2 = b.run(per_pair=True).results()
+    assert all(np.array_equal(x, y) for x, y in zip(res, res2))
+    b.close()
