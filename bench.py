@@ -100,6 +100,12 @@ def main():
     pairs = W.ragged_batch()
     log(f"workload generated in {time.perf_counter() - t0:.1f}s")
     batch = semilocal_amd.Batch(pairs, rows_per_lane=args.rows_per_lane)
+    if not args.rows_per_lane:
+        t0 = time.perf_counter()
+        tiles = batch.autotune(iters=1)
+        log(f"autotuner picked rows_per_lane={tiles[0]} waves_per_cu={tiles[1]} "
+            f"in {time.perf_counter() - t0:.1f}s")
+    tiles = batch.tiles()
     st = batch.stats()
     cells = st["cells"]
     log("batch:", st)
@@ -140,7 +146,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "ragged batch: 4096 pairs, lengths log-normal (median 2000, sigma_ln 1.0) "
                                "clipped to [100, 50000], alphabet 20, half mutated (10% subs, 2% indels), seed 4",
-                   "pairs": st["pairs"], "cells": cells, "tiles": st["tiles"]},
+                   "pairs": st["pairs"], "cells": cells, "tiles": st["tiles"],
+                   "rows_per_lane": tiles[0], "waves_per_cu": tiles[1]},
         "one_pair_per_launch": {"value": round(gcups_pp, 2), "unit": "GCUPS", "ms_per_step": round(pms, 3)},
         "batched_vs_per_pair": round(gcups / gcups_pp, 2),
         "parity": parity,

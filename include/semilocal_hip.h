@@ -79,8 +79,8 @@ typedef struct sl_batch sl_batch;
 #define SL_BATCH_WITH_COL_TO_ROW 1   /* also produce col_to_row */
 
 /* Uploads the batch to HBM and prepares the queue.  `rows_per_lane` = 0
- * lets the built-in length-bucket table choose tile heights; 1,2,4,8,16 forces
- * one value for every pair. */
+ * lets the built-in cost model choose the tile height from the batch's length
+ * profile; 4, 8 or 16 forces one (see sl_batch_set_tiles). */
 int sl_batch_create(sl_batch **out, int npairs, const void *a_cat, const int64_t *a_off,
                     const int32_t *a_len, const void *b_cat, const int64_t *b_off,
                     const int32_t *b_len, int sym_bytes, int flags, int rows_per_lane);
@@ -99,6 +99,14 @@ int sl_batch_download(sl_batch *bt, int32_t *row_to_col_cat, int32_t *col_to_row
  * (events around the kernel launches only). */
 int sl_batch_time(sl_batch *bt, int mode, int warmup, int iters, double *ms_per_run,
                   double *kernel_ms_out);
+/* Tile shape of the batch: rows per lane R (4, 8 or 16; one queue tile = one
+ * pass of 64R rows of one pair) and persistent waves per compute unit.
+ * Results never depend on it (the semi-local kernel is unique). */
+int sl_batch_set_tiles(sl_batch *bt, int rows_per_lane, int waves_per_cu);
+int sl_batch_get_tiles(const sl_batch *bt, int *rows_per_lane, int *waves_per_cu);
+/* Autotuner: times every tile shape on this batch (`iters` runs each, HIP
+ * events) and keeps the fastest; reports it through the out-pointers. */
+int sl_batch_autotune(sl_batch *bt, int iters, int *rows_per_lane, int *waves_per_cu);
 /* Sum of m*n over the batch (cells), number of queue tiles, number of pairs. */
 int sl_batch_stats(const sl_batch *bt, int64_t *cells, int64_t *tiles, int64_t *pairs);
 void sl_batch_destroy(sl_batch *bt);

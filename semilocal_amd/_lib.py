@@ -13,7 +13,7 @@ import threading
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libsemilocal_hip.so")
+LIB_PATH = os.environ.get("SEMILOCAL_LIB") or os.path.join(PKG, "libsemilocal_hip.so")
 
 SL_OK = 0
 SL_BATCH_WITH_COL_TO_ROW = 1
@@ -49,6 +49,9 @@ SIGNATURES = {
     "sl_batch_sync": ([_vp], C.c_int),
     "sl_batch_download": ([_vp, _vp, _vp], C.c_int),
     "sl_batch_time": ([_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "sl_batch_set_tiles": ([_vp, C.c_int, C.c_int], C.c_int),
+    "sl_batch_get_tiles": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "sl_batch_autotune": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "sl_batch_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "sl_batch_destroy": ([_vp], None),
     "sl_comb_batch": ([C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, _vp, _vp], C.c_int),
@@ -181,6 +184,19 @@ class Batch:
         kms = C.c_double()
         check(lib().sl_batch_time(self.h, 1 if per_pair else 0, warmup, iters, C.byref(ms), C.byref(kms)))
         return ms.value, kms.value
+
+    def set_tiles(self, rows_per_lane: int, waves_per_cu: int = 16):
+        check(lib().sl_batch_set_tiles(self.h, rows_per_lane, waves_per_cu))
+
+    def tiles(self):
+        r, w = C.c_int(), C.c_int()
+        check(lib().sl_batch_get_tiles(self.h, C.byref(r), C.byref(w)))
+        return r.value, w.value
+
+    def autotune(self, iters: int = 1):
+        r, w = C.c_int(), C.c_int()
+        check(lib().sl_batch_autotune(self.h, iters, C.byref(r), C.byref(w)))
+        return r.value, w.value
 
     def stats(self):
         c, t, p = C.c_int64(), C.c_int64(), C.c_int64()

@@ -2,6 +2,8 @@
 // stream management, ragged-batch planning (orientation, length-bucket tile
 // table, LPT queue order), uploads, launches and timing.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -14,9 +16,10 @@
 namespace slhip {
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t stream);
-hipError_t launch_comb_batch(int sym_bytes, unsigned waves, hipStream_t stream, const PairDesc *pairs,
-                             const WorkItem *items, uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr,
-                             uint32_t *buf, uint32_t *progress, LaunchCtl *ctl, uint32_t spin_limit);
+hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
+                             const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
+                             int32_t *rc, int32_t *cr, uint32_t *buf, uint32_t *progress, LaunchCtl *ctl,
+                             uint32_t spin_limit);
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -135,27 +138,67 @@ static int pack_pool_from(const T *a_cat, const int64_t *a_off, const int32_t *a
     return kOk;
 }
 
-// ---------------------------------------------------------------------------
-// Length-bucket tile table: rows held per lane (R) for a pair with m rows
-// (the shorter string) and n columns.  One pass = 64R rows; the passes of a
-// pair run concurrently, so R trades per-pass overhead (5 DPP moves per step
-// amortised over R cells) against pipeline depth.  Entries are produced by the
-// autotuner (semilocal_amd/autotune.py) and can be replaced at run time.
-// ---------------------------------------------------------------------------
-struct TileRule {
-    int32_t max_m;   // rule applies to pairs with m <= max_m
-    int32_t rows_per_lane;
-};
-static std::vector<TileRule> g_tile_rules = {
-    {64, 1}, {128, 2}, {256, 4}, {1024, 8}, {0x7fffffff, 16},
-};
-static std::mutex g_tile_mu;
+// Rewrites each pair's symbols (rows and columns share one map) to dense codes
+// 0..sigma_pair-1 in first-seen order and picks the cell mode.
+static void remap_pairs_dense(std::vector<PairDesc> &pairs, PackedPool &pool) {
+    std::vector<int32_t> table(pool.sym_bytes == 1 ? 256 : 65536, -1);
+    std::vector<uint32_t> touched;
+    for (auto &d : pairs) {
+        touched.clear();
+        int32_t sigma = 0;
+        auto remap = [&](int64_t off, int32_t len) {
+            for (int32_t k = 0; k < len; ++k) {
+                uint32_t v = pool.sym_bytes == 1 ? pool.bytes[(size_t)(off + k)]
+                                                 : ((uint16_t *)pool.bytes.data())[off + k];
+                if (table[v] < 0) {
+                    table[v] = sigma++;
+                    touched.push_back(v);
+                }
+                if (pool.sym_bytes == 1) pool.bytes[(size_t)(off + k)] = (uint8_t)table[v];
+                else ((uint16_t *)pool.bytes.data())[off + k] = (uint16_t)table[v];
+            }
+        };
+        remap(d.rows_off, d.m);
+        remap(d.cols_off, d.n);
+        for (uint32_t v : touched) table[v] = -1;
+        const uint32_t max_id = (uint32_t)(d.m + d.n > 1 ? d.m + d.n - 1 : 1);
+        int bits = 0;
+        while (bits < 32 && (max_id >> bits) != 0) ++bits;
+        d.tag_shift = bits;
+        d.onehot = (bits + sigma <= 32) ? 1 : 0;
+    }
+}
 
-int pick_rows_per_lane(int32_t m) {
-    std::lock_guard<std::mutex> lk(g_tile_mu);
-    for (const auto &r : g_tile_rules)
-        if (m <= r.max_m) return r.rows_per_lane;
-    return 16;
+// ---------------------------------------------------------------------------
+// Tile height (rows per lane, R) for a whole batch.  One pass = 64R rows; the
+// passes of a pair run concurrently, each ~2.5 chunks (160 steps) behind the
+// previous one.  Larger R costs less per cell (one DPP move and three LDS
+// accesses per step are shared by R cells) but lengthens the critical path of
+// the largest pair (fewer, longer passes).  The default minimises
+// max(throughput time, critical path) with per-R rates measured on MI355X;
+// sl_batch_autotune replaces it with a measurement on the batch itself.
+// ---------------------------------------------------------------------------
+static const int kTileHeights[3] = {4, 8, 16};
+
+int model_rows_per_lane(int64_t total_cells, int32_t m_max, int32_t n_max, int cus) {
+    const double rate[3] = {6.2e12, 7.2e12, 8.0e12};   // single-pass cells/s on 256 CUs
+    const double scale = cus > 0 ? cus / 256.0 : 1.0;
+    int best = 16;
+    double best_t = 1e30;
+    for (int k = 2; k >= 0; --k) {
+        const int R = kTileHeights[k];
+        const double passes = std::ceil(m_max / (64.0 * R));
+        const double steps = n_max + passes * 160.0;
+        const double clk_per_step = (R * 4.1 + 2.0) * 4.0 * 3.0;   // SIMD shared by ~3 resident waves
+        const double t_crit = steps * clk_per_step / 2.4e9;
+        const double t_thru = (double)total_cells / (rate[k] * scale);
+        const double t = std::max(t_crit, t_thru);
+        if (t < best_t * 0.97) {   // a smaller R must be clearly faster
+            best_t = t;
+            best = R;
+        }
+    }
+    return best;
 }
 
 // ---------------------------------------------------------------------------
@@ -172,6 +215,8 @@ struct Batch {
     int64_t buf_total = 0;
     int64_t cells = 0;
     uint32_t spin_limit = 1u << 23;
+    int rows_per_lane = 16;            // tile height of every pass of this batch
+    unsigned waves_per_cu = 16;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     PairDesc *d_pairs = nullptr;
@@ -183,6 +228,7 @@ struct Batch {
     uint32_t *d_progress = nullptr;
     LaunchCtl *d_ctl = nullptr;    // [0] batched launch, [1+i] per-pair launch of pair i
     unsigned waves_cap = 0;
+    size_t items_cap = 0;
 };
 
 static void batch_free(Batch *b) {
@@ -199,6 +245,46 @@ static void batch_free(Batch *b) {
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+// Splits every pair into passes of 64R rows and orders the queue: largest
+// pairs first (longest-processing-time first), passes of a pair consecutive.
+static void plan_items(Batch *bt) {
+    const int64_t rpp = 64LL * bt->rows_per_lane;
+    std::vector<int> order(bt->npairs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (int64_t)bt->pairs[x].m * bt->pairs[x].n > (int64_t)bt->pairs[y].m * bt->pairs[y].n;
+    });
+    bt->items.clear();
+    for (int i : order) {
+        PairDesc &d = bt->pairs[i];
+        d.rows_per_lane = bt->rows_per_lane;
+        d.npasses = (d.m > 0 && d.n > 0) ? (int32_t)((d.m + rpp - 1) / rpp) : 0;
+        d.item0 = (int32_t)bt->items.size();
+        for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
+    }
+    bt->waves_cap = (unsigned)device_cus() * bt->waves_per_cu;
+}
+
+// (Re)uploads the pair table and the queue; grows the device copies as needed.
+static hipError_t upload_items(Batch *bt) {
+    hipError_t e = hipSuccess;
+    if (!bt->d_pairs) e = hipMalloc((void **)&bt->d_pairs, sizeof(PairDesc) * std::max<size_t>(bt->pairs.size(), 1));
+    if (e == hipSuccess && !bt->pairs.empty())
+        e = hipMemcpy(bt->d_pairs, bt->pairs.data(), sizeof(PairDesc) * bt->pairs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && (!bt->d_items || bt->items.size() > bt->items_cap)) {
+        (void)hipFree(bt->d_items);
+        (void)hipFree(bt->d_progress);
+        bt->d_items = nullptr;
+        bt->d_progress = nullptr;
+        bt->items_cap = std::max<size_t>(bt->items.size(), 1);
+        e = hipMalloc((void **)&bt->d_items, sizeof(WorkItem) * bt->items_cap);
+        if (e == hipSuccess) e = hipMalloc((void **)&bt->d_progress, sizeof(uint32_t) * bt->items_cap);
+    }
+    if (e == hipSuccess && !bt->items.empty())
+        e = hipMemcpy(bt->d_items, bt->items.data(), sizeof(WorkItem) * bt->items.size(), hipMemcpyHostToDevice);
+    return e;
 }
 
 template <typename T>
@@ -220,9 +306,8 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
         set_error("sym_bytes must be 1, 2 or 4");
         return kErrArg;
     }
-    if (rows_per_lane != 0 && rows_per_lane != 1 && rows_per_lane != 2 && rows_per_lane != 4 && rows_per_lane != 8 &&
-        rows_per_lane != 16) {
-        set_error("rows_per_lane must be 0, 1, 2, 4, 8 or 16");
+    if (rows_per_lane != 0 && rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16) {
+        set_error("rows_per_lane must be 0 (model), 4, 8 or 16");
         return kErrArg;
     }
     for (int i = 0; i < npairs; ++i) {
@@ -259,9 +344,10 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     }
     bt->sym_bytes = pool.sym_bytes;
 
-    // orientation, tile choice, output/buffer offsets (caller order)
+    // orientation and output/buffer offsets (caller order)
     bt->pairs.resize(npairs);
     bt->perm_off.resize(npairs);
+    int32_t m_max = 0, n_max = 0;
     for (int i = 0; i < npairs; ++i) {
         PairDesc &d = bt->pairs[i];
         const bool rev = a_len[i] > b_len[i];
@@ -274,40 +360,30 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
         bt->perm_off[i] = bt->perm_total;
         bt->perm_total += (int64_t)d.m + d.n;
         bt->cells += (int64_t)d.m * d.n;
-        d.rows_per_lane = rows_per_lane ? rows_per_lane : pick_rows_per_lane(d.m);
-        const int64_t rpp = 64LL * d.rows_per_lane;
-        d.npasses = (d.m > 0 && d.n > 0) ? (int32_t)((d.m + rpp - 1) / rpp) : 0;
-        d.buf_off = 0;
-        if (d.npasses > 1) {
-            d.buf_off = bt->buf_total;
-            bt->buf_total += d.n;
-        }
+        d.buf_off = bt->buf_total;   // strand buffer for multi-pass tiling (sized for any R)
+        bt->buf_total += d.n;
+        m_max = std::max(m_max, d.m);
+        n_max = std::max(n_max, d.n);
     }
-    // queue order: largest pairs first (longest-processing-time first), passes consecutive
-    std::vector<int> order(npairs);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return (int64_t)bt->pairs[x].m * bt->pairs[x].n > (int64_t)bt->pairs[y].m * bt->pairs[y].n;
-    });
-    for (int i : order) {
-        PairDesc &d = bt->pairs[i];
-        d.item0 = (int32_t)bt->items.size();
-        for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
+    // per-pair dense symbol codes (only equality matters, so relabeling keeps the
+    // kernel) and the cell mode: one-hot tags above the strand-id bits when
+    // bits(m+n-1) + sigma_pair <= 32
+    remap_pairs_dense(bt->pairs, pool);
+    bt->rows_per_lane = rows_per_lane ? rows_per_lane : model_rows_per_lane(bt->cells, m_max, n_max, device_cus());
+    if (const char *env = std::getenv("SEMILOCAL_WAVES_PER_CU")) {   // tuning override
+        const long v = std::strtol(env, nullptr, 10);
+        if (v > 0 && v <= 64) bt->waves_per_cu = (unsigned)v;
     }
-    bt->waves_cap = (unsigned)device_cus() * 16u;
+    plan_items(bt);
 
     hipError_t e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking);
     for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&bt->ev[k]);
-    if (e == hipSuccess) e = upload(&bt->d_pairs, bt->pairs.data(), bt->pairs.size());
-    if (e == hipSuccess) e = upload(&bt->d_items, bt->items.data(), bt->items.size());
+    if (e == hipSuccess) e = upload_items(bt);
     if (e == hipSuccess) e = upload((uint8_t **)&bt->d_pool, pool.bytes.data(), pool.bytes.size());
     if (e == hipSuccess) e = hipMalloc((void **)&bt->d_rc, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
     if (e == hipSuccess && bt->with_cr)
         e = hipMalloc((void **)&bt->d_cr, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&bt->d_buf, sizeof(uint32_t) * std::max<int64_t>(bt->buf_total, 1));
-    if (e == hipSuccess)
-        e = hipMalloc((void **)&bt->d_progress, sizeof(uint32_t) * std::max<size_t>(bt->items.size(), 1));
-    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_ctl, sizeof(LaunchCtl) * (size_t)(npairs + 1));
     if (e != hipSuccess) {
         set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
         batch_free(bt);
@@ -345,7 +421,7 @@ static hipError_t launch_items(Batch *bt, const WorkItem *items, uint32_t nitems
     const unsigned waves = std::min<unsigned>(nitems, bt->waves_cap);
     const void *pool = bt->d_pool;
     // per-pair pointers are resolved inside the kernel from PairDesc offsets
-    return launch_comb_batch(bt->sym_bytes, waves, bt->stream, bt->d_pairs, items, nitems, pool,
+    return launch_comb_batch(bt->sym_bytes, bt->rows_per_lane, waves, bt->stream, bt->d_pairs, items, nitems, pool,
                              bt->d_rc, bt->d_cr, bt->d_buf, bt->d_progress, ctl, bt->spin_limit);
 }
 
@@ -516,6 +592,62 @@ int sl_batch_time(sl_batch *h, int mode, int warmup, int iters, double *ms_per_r
         }
         *kernel_ms_out = tot / iters;
     }
+    return kOk;
+}
+
+int sl_batch_set_tiles(sl_batch *h, int rows_per_lane, int waves_per_cu) {
+    Batch *bt = reinterpret_cast<Batch *>(h);
+    if (!bt || (rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16) || waves_per_cu < 1 ||
+        waves_per_cu > 64) {
+        set_error("sl_batch_set_tiles: rows_per_lane must be 4, 8 or 16 and waves_per_cu 1..64");
+        return kErrArg;
+    }
+    SL_HIP_TRY(hipStreamSynchronize(bt->stream));
+    bt->rows_per_lane = rows_per_lane;
+    bt->waves_per_cu = (unsigned)waves_per_cu;
+    plan_items(bt);
+    SL_HIP_TRY(upload_items(bt));
+    return kOk;
+}
+
+int sl_batch_get_tiles(const sl_batch *h, int *rows_per_lane, int *waves_per_cu) {
+    const Batch *bt = reinterpret_cast<const Batch *>(h);
+    if (!bt) {
+        set_error("null batch");
+        return kErrArg;
+    }
+    if (rows_per_lane) *rows_per_lane = bt->rows_per_lane;
+    if (waves_per_cu) *waves_per_cu = (int)bt->waves_per_cu;
+    return kOk;
+}
+
+int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per_cu) {
+    Batch *bt = reinterpret_cast<Batch *>(h);
+    if (!bt || iters <= 0) {
+        set_error("sl_batch_autotune: bad argument");
+        return kErrArg;
+    }
+    const int wpc_cand[3] = {8, 12, 16};
+    double best = 1e30;
+    int best_r = bt->rows_per_lane, best_w = (int)bt->waves_per_cu;
+    for (int r : kTileHeights) {
+        for (int w : wpc_cand) {
+            int rc = sl_batch_set_tiles(h, r, w);
+            if (rc) return rc;
+            double ms = 0.0;
+            rc = sl_batch_time(h, 0, 1, iters, &ms, nullptr);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                best_r = r;
+                best_w = w;
+            }
+        }
+    }
+    int rc = sl_batch_set_tiles(h, best_r, best_w);
+    if (rc) return rc;
+    if (rows_per_lane) *rows_per_lane = best_r;
+    if (waves_per_cu) *waves_per_cu = best_w;
     return kOk;
 }
 

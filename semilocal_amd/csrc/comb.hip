@@ -6,13 +6,16 @@
 //  * one work item = one "pass" of a pair = 64*R consecutive rows (grid rows
 //    of the shorter string).  Lane k of the wave owns rows
 //    [pass*64R + kR, pass*64R + kR + R) and keeps their left strands and row
-//    symbols in VGPRs for the whole pass.
+//    symbol values in VGPRs for the whole pass.
 //  * columns stream through the lanes: at step s lane k combs column s-k for
-//    its R rows (top to bottom), then hands the column's top strand and
-//    column symbol to lane k+1 with one DPP `wave_shr:1` move each.  Lane 0
-//    takes the next column from a 64-column chunk held in a VGPR that is
-//    rotated with `wave_shl:1`.  Lane 63's outgoing strands are collected
-//    into a VGPR the same way and written back coalesced every 64 steps.
+//    its R rows (top to bottom) and hands the column's top strand to lane k+1
+//    with one DPP `wave_shr:1` move; lane 0 takes the next column's strand.
+//    Column symbol values and the injected strands come from a wave-private
+//    LDS window filled 64 columns at a time (coalesced loads); every lane
+//    writes its outgoing strand into an LDS slot indexed by column, so the
+//    slot ends up holding lane 63's value, which is written back coalesced.
+//    Per step: R cells of 4 (one-hot symbols) or 5 VALU ops, one DPP move,
+//    two LDS reads and one LDS write.
 //  * passes of one pair run concurrently on different waves: pass p+1 reads
 //    the bottom strands pass p wrote to the pair's strand buffer, in place,
 //    after pass p's per-pass progress counter says the chunk is there
@@ -31,20 +34,10 @@ namespace slhip {
 namespace {
 
 constexpr uint32_t kNoStrand = 0xFFFFFFFFu;   // top strand of an absent column: l > t never holds
-constexpr uint32_t kPadRowSym = 0x10000u;     // symbol of a padding row: equals no column symbol
-constexpr uint32_t kPadColSym = 0x10001u;     // symbol of an absent column: equals no row symbol
 constexpr int kDppWaveShr1 = 0x138;           // lane k <- lane k-1 (lane 0 keeps `old`)
-constexpr int kDppWaveShl1 = 0x130;           // lane k <- lane k+1 (lane 63 keeps `old`)
 
 __device__ __forceinline__ uint32_t dpp_shr1(uint32_t old, uint32_t src) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, kDppWaveShr1, 0xf, 0xf, false);
-}
-__device__ __forceinline__ uint32_t dpp_shl1(uint32_t old, uint32_t src) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, kDppWaveShl1, 0xf, 0xf, false);
-}
-__device__ __forceinline__ uint32_t dpp_rot_down(uint32_t src) {
-    // lane k <- lane k+1; lane 63's value is never used
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)src, kDppWaveShl1, 0xf, 0xf, false);
 }
 
 // Write strand `s` exiting at position `c` of the oriented problem into the
@@ -64,12 +57,30 @@ __device__ __forceinline__ void emit(int32_t *rc, int32_t *cr, uint32_t size, bo
     if (cr) cr[c] = (int32_t)s;
 }
 
+// Inter-pass hand-off.  The strands one pass passes to the next and the
+// per-pass progress counters are only ever accessed with device-coherent
+// (agent-scope) relaxed atomics, which gfx950 performs with the sc1 bit, past
+// the per-XCD L2s.  The producer therefore needs no L2 write-back and the
+// consumer no L2 invalidate (an agent-scope release/acquire fence would issue
+// both on every chunk); ordering comes from waiting for the wave's stores
+// (vmcnt(0)) before the counter is bumped, and from the counter spin loop
+// before the strands are read.
+__device__ __forceinline__ uint32_t ld_coherent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0) (GFX9 encoding: vmcnt=0, expcnt=7, lgkmcnt=15): every
+// load and store this wave issued has been performed.
+__device__ __forceinline__ void wait_vm_idle() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Bounded wait until progress[idx] >= need.  Returns false on timeout (or when
 // another wave already reported one), after flagging the error.
 __device__ __forceinline__ bool wait_progress(const uint32_t *progress, uint32_t idx, uint32_t need,
                                               LaunchCtl *ctl, uint32_t spin_limit) {
     uint32_t spins = 0;
-    while (__hip_atomic_load(&progress[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+    while (ld_coherent(&progress[idx]) < need) {
         __builtin_amdgcn_s_sleep(2);
         if (++spins >= spin_limit ||
             __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
@@ -77,20 +88,63 @@ __device__ __forceinline__ bool wait_progress(const uint32_t *progress, uint32_t
             return false;
         }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");   // later strand loads stay after the spin
     return true;
 }
 
 __device__ __forceinline__ void publish(uint32_t *progress, uint32_t idx, uint32_t value, int lane) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) __hip_atomic_store(&progress[idx], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // strand stores stay before the counter
+    wait_vm_idle();
+    if (lane == 0) st_coherent(&progress[idx], value);
 }
 
-template <int R, typename Sym>
+// LDS words of one wave's private staging area: column values of the
+// 128-column window read by the lanes, the 64 top strands injected at lane 0,
+// and the 128-slot window the lanes' outgoing strands are written into.
+constexpr int kWinB = 0;      // [0, 128):   column value of column 64j-64+x
+constexpr int kInjT = 128;    // [128, 192): top strand entering column 64j+x
+constexpr int kOutT = 192;    // [192, 320): strand leaving lane k at step u -> slot 64+u-k
+constexpr int kWaveLds = 320;
+constexpr int kWavesPerBlock = 4;
+
+// Ordering point for lanes of one wave that exchange data through LDS.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One cell of the combing rule (semi_local.h:46-61): the strands swap iff the
+// symbols match or the left strand already crossed the top one (l > t).
+// A match lifts l above every strand id (a tag bit above the id bits), so a
+// single unsigned min gives the new left strand; since {l', t'} = {l, t} as a
+// multiset, t' = l ^ t ^ l'.
+//  ONEHOT: A and B are one-hot symbol masks shifted above the id bits, so the
+//          tag is (A & B): v_and_or_b32, v_min_u32, v_xor_b32 x2 (4 VALU).
+//  else:   A and B are symbol codes; ((A ^ B) - 1) has bit 31 set iff they are
+//          equal: v_xad_u32, v_and_or_b32, v_min_u32, v_xor_b32 x2 (5 VALU).
+template <bool ONEHOT>
+__device__ __forceinline__ void comb_cell(uint32_t A, uint32_t B, uint32_t &l, uint32_t &t) {
+    uint32_t x;
+    if (ONEHOT) x = (A & B) | l;
+    else x = (((A ^ B) - 1u) & 0x80000000u) | l;
+    const uint32_t nl = x < t ? x : t;
+    t = (l ^ t) ^ nl;
+    l = nl;
+}
+
+template <bool ONEHOT>
+__device__ __forceinline__ uint32_t sym_value(uint32_t code, uint32_t tag_shift) {
+    return ONEHOT ? (1u << (tag_shift + code)) : code;
+}
+
+template <int R, bool ONEHOT, typename Sym>
 __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ pool, int32_t *__restrict__ rc,
                           int32_t *__restrict__ cr, uint32_t *__restrict__ buf, uint32_t *progress,
-                          LaunchCtl *ctl, uint32_t spin_limit) {
+                          LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W) {
     const int lane = threadIdx.x & 63;
     const uint32_t m = (uint32_t)d.m, n = (uint32_t)d.n, size = m + n;
+    const uint32_t tag_shift = (uint32_t)d.tag_shift;
     const bool reverse = d.reverse != 0;
     const Sym *rows = pool + d.rows_off;
     const Sym *cols = pool + d.cols_off;
@@ -101,26 +155,29 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
     const bool last = pass == d.npasses - 1;
     const uint32_t item = (uint32_t)(d.item0 + pass);
     const int64_t row0 = (int64_t)pass * 64 * R + (int64_t)lane * R;
+    // values of padding rows / absent columns: never equal to anything
+    const uint32_t pad_row = ONEHOT ? 0u : 0x10000u;
+    const uint32_t pad_col = ONEHOT ? 0u : 0x20000u;
 
     // Left strand of grid row i is m-1-i (semi_local.h:70-82, 113-119).
-    uint32_t l[R], av[R];
+    uint32_t l[R], A[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
         const int64_t i = row0 + q;
         const bool ok = i < (int64_t)m;
         l[q] = ok ? (uint32_t)(m - 1 - i) : 0u;
-        av[q] = ok ? (uint32_t)rows[i] : kPadRowSym;
+        A[q] = ok ? sym_value<ONEHOT>((uint32_t)rows[i], tag_shift) : pad_row;
     }
 
     const uint32_t nchunks = (n + 63u + 63u) / 64u;   // steps 0 .. n+62 inclusive
     auto load_chunk = [&](uint32_t j, uint32_t &tin, uint32_t &bin) {
         const uint32_t col = 64u * j + (uint32_t)lane;
         if (col < n) {
-            tin = first ? (m + col) : sbuf[col];   // top strand ids m+j (semi_local.h:77-80)
-            bin = (uint32_t)cols[col];
+            tin = first ? (m + col) : ld_coherent(&sbuf[col]);   // top strand ids m+j (semi_local.h:77-80)
+            bin = sym_value<ONEHOT>((uint32_t)cols[col], tag_shift);
         } else {
             tin = kNoStrand;
-            bin = kPadColSym;
+            bin = pad_col;
         }
     };
     auto chunk_ready = [&](uint32_t j) -> bool {
@@ -129,41 +186,50 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
         return wait_progress(progress, item - 1u, need, ctl, spin_limit);
     };
 
-    uint32_t tin_nx = kNoStrand, bin_nx = kPadColSym;
+    uint32_t tin_nx = kNoStrand, bin_nx = pad_col;
     bool ok = chunk_ready(0);
     if (ok) load_chunk(0, tin_nx, bin_nx);
-    uint32_t t = kNoStrand, bs = kPadColSym, acc = 0;
+    uint32_t t = kNoStrand, b_prev = pad_col;
     for (uint32_t j = 0; j < nchunks && ok; ++j) {
-        uint32_t tin = tin_nx, bin = bin_nx;
+        const uint32_t tin = tin_nx, bin = bin_nx;
+        W[kWinB + lane] = b_prev;
+        W[kWinB + 64 + lane] = bin;
+        W[kInjT + lane] = tin;
+        b_prev = bin;
+        wave_lds_sync();
+        const uint32_t *wb = W + kWinB + 64 - lane;   // wb[u]: column 64j+u-lane
+        uint32_t *wo = W + kOutT + 64 - lane;         // wo[u]: slot of column 64j+u-lane
+        auto steps = [&](int u_begin, int u_end) {
+            for (int u0 = u_begin; u0 < u_end; u0 += 8) {
+#pragma unroll
+                for (int uu = 0; uu < 8; ++uu) {
+                    const int u = u0 + uu;
+                    const uint32_t bv = wb[u];
+                    t = dpp_shr1(W[kInjT + u], t);   // lane 0 takes column 64j+u, lane k>0 lane k-1's strand
+#pragma unroll
+                    for (int q = 0; q < R; ++q) comb_cell<ONEHOT>(A[q], bv, l[q], t);
+                    wo[u] = t;
+                }
+            }
+        };
+        steps(0, 32);
+        // fetch chunk j+1 half-way through chunk j: it is needed when chunk j+1
+        // starts, and asking for it later shortens the lag behind the previous pass
         if (j + 1u < nchunks) {
             ok = chunk_ready(j + 1u);
             if (ok) load_chunk(j + 1u, tin_nx, bin_nx);
         }
-#pragma unroll 4
-        for (int u = 0; u < 64; ++u) {
-            const uint32_t tin_rot = dpp_rot_down(tin);
-            const uint32_t bin_rot = dpp_rot_down(bin);
-            t = dpp_shr1(tin, t);      // lane 0 takes column s, lane k>0 takes lane k-1's
-            bs = dpp_shr1(bin, bs);
-            tin = tin_rot;
-            bin = bin_rot;
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                // combing rule: swap iff symbols match or the strands already crossed
-                const bool r = (av[q] == bs) | (l[q] > t);
-                const uint32_t nl = r ? t : l[q];
-                t = r ? l[q] : t;
-                l[q] = nl;
-            }
-            acc = dpp_shl1(t, acc);    // lane 63's outgoing strand enters the collector
-        }
-        // acc lane k holds the final top strand of column 64j + k - 63
+        steps(32, 64);
+        wave_lds_sync();
+        // slot 1+k was written last by lane 63 (step k): final top strand of column 64j+k-63
+        const uint32_t acc = W[kOutT + 1 + lane];
         const int64_t col = 64 * (int64_t)j + lane - 63;
         if (col >= 0 && col < (int64_t)n) {
             if (last) emit(prc, pcr, size, reverse, acc, (uint32_t)col, ctl);
-            else sbuf[col] = acc;
+            else st_coherent(&sbuf[col], acc);
         }
         if (!last) publish(progress, item, j + 1u, lane);
+        wave_lds_sync();
     }
     if (!ok) {
         // timed out: let any successor drain immediately (results are void)
@@ -178,15 +244,21 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
     }
 }
 
+
 }  // namespace
 
-template <typename Sym>
+// Persistent queue consumer: every wave takes (pair, pass) tickets until the
+// queue is empty.  One instantiation per tile height R (rows per lane); each
+// holds both cell modes, chosen per pair.
+template <typename Sym, int R>
 __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restrict__ pairs,
                                                          const WorkItem *__restrict__ items, uint32_t nitems,
                                                          const Sym *__restrict__ pool, int32_t *__restrict__ rc,
                                                          int32_t *__restrict__ cr, uint32_t *__restrict__ buf,
                                                          uint32_t *progress, LaunchCtl *ctl, uint32_t spin_limit) {
+    __shared__ uint32_t lds[kWavesPerBlock * kWaveLds];
     const int lane = threadIdx.x & 63;
+    uint32_t *W = lds + (threadIdx.x >> 6) * kWaveLds;   // this wave's private staging area
     for (;;) {
         uint32_t tk = 0;
         if (lane == 0) tk = atomicAdd(&ctl->ticket, 1u);
@@ -194,13 +266,10 @@ __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restr
         if (tk >= nitems) break;     // every wave leaves once the queue is empty
         const WorkItem it = items[tk];
         const PairDesc d = pairs[it.pair];
-        switch (d.rows_per_lane) {
-            case 1: comb_pass<1, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit); break;
-            case 2: comb_pass<2, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit); break;
-            case 4: comb_pass<4, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit); break;
-            case 8: comb_pass<8, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit); break;
-            default: comb_pass<16, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit); break;
-        }
+        if (d.onehot)
+            comb_pass<R, true, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit, W);
+        else
+            comb_pass<R, false, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit, W);
     }
 }
 
@@ -217,17 +286,31 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_comb_batch(int sym_bytes, unsigned waves, hipStream_t stream, const PairDesc *pairs,
-                             const WorkItem *items, uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr,
-                             uint32_t *buf, uint32_t *progress, LaunchCtl *ctl, uint32_t spin_limit) {
+template <typename Sym, int R>
+static void launch_one(unsigned blocks, hipStream_t stream, const PairDesc *pairs, const WorkItem *items,
+                       uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr, uint32_t *buf, uint32_t *progress,
+                       LaunchCtl *ctl, uint32_t spin_limit) {
+    hipLaunchKernelGGL((comb_batch_kernel<Sym, R>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, stream, pairs, items,
+                       nitems, (const Sym *)pool, rc, cr, buf, progress, ctl, spin_limit);
+}
+
+hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
+                             const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
+                             int32_t *rc, int32_t *cr, uint32_t *buf, uint32_t *progress, LaunchCtl *ctl,
+                             uint32_t spin_limit) {
     if (nitems == 0 || waves == 0) return hipSuccess;
-    const unsigned blocks = (waves + 3u) / 4u;
-    if (sym_bytes == 1)
-        hipLaunchKernelGGL(comb_batch_kernel<uint8_t>, dim3(blocks), dim3(256), 0, stream, pairs, items, nitems,
-                           (const uint8_t *)pool, rc, cr, buf, progress, ctl, spin_limit);
-    else
-        hipLaunchKernelGGL(comb_batch_kernel<uint16_t>, dim3(blocks), dim3(256), 0, stream, pairs, items, nitems,
-                           (const uint16_t *)pool, rc, cr, buf, progress, ctl, spin_limit);
+    const unsigned blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+#define SL_LAUNCH(SYM, RR) launch_one<SYM, RR>(blocks, stream, pairs, items, nitems, pool, rc, cr, buf, progress, ctl, spin_limit)
+    if (sym_bytes == 1) {
+        if (rows_per_lane == 4) SL_LAUNCH(uint8_t, 4);
+        else if (rows_per_lane == 8) SL_LAUNCH(uint8_t, 8);
+        else SL_LAUNCH(uint8_t, 16);
+    } else {
+        if (rows_per_lane == 4) SL_LAUNCH(uint16_t, 4);
+        else if (rows_per_lane == 8) SL_LAUNCH(uint16_t, 8);
+        else SL_LAUNCH(uint16_t, 16);
+    }
+#undef SL_LAUNCH
     return hipGetLastError();
 }
 

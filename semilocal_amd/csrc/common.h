@@ -42,6 +42,8 @@ struct PairDesc {
     int32_t item0;      // queue index of pass 0 of this pair
     int32_t npasses;    // ceil(m / rows_per_pass)
     int32_t rows_per_lane;  // R: rows held by one lane (rows_per_pass = 64 R)
+    int32_t onehot;     // 1: symbols are tested as one-hot tags above the strand-id bits
+    int32_t tag_shift;  // bit position of symbol code 0's tag (= bits of the largest strand id)
 };
 
 // Queue entry: (pair, pass).  Entries of one pair are consecutive and in pass
