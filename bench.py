@@ -154,7 +154,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 2),
                      "unit": "Tops/s (int32 VALU lane-ops)", "frac": round(achieved_tops / peak_tops, 4),
                      "traffic": None,
-                     "kernel": "comb_batch_kernel<uint8_t>", "kernel_ms": round(kms, 4),
+                     "kernel": f"comb_batch_kernel<uint8_t,{tiles[0]}>", "kernel_ms": round(kms, 4),
                      "ops_per_cell": VALU_OPS_PER_CELL},
     }
     batch.close()

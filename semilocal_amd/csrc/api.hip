@@ -384,6 +384,7 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     if (e == hipSuccess && bt->with_cr)
         e = hipMalloc((void **)&bt->d_cr, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&bt->d_buf, sizeof(uint32_t) * std::max<int64_t>(bt->buf_total, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_ctl, sizeof(LaunchCtl) * (size_t)(npairs + 1));
     if (e != hipSuccess) {
         set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
         batch_free(bt);
@@ -411,7 +412,19 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     return kOk;
 }
 
+// Host-side guard: every device buffer a launch touches must exist.
+static int batch_check_ready(const Batch *bt) {
+    if (!bt->d_pairs || !bt->d_items || !bt->d_pool || !bt->d_rc || !bt->d_buf || !bt->d_progress || !bt->d_ctl ||
+        (bt->with_cr && !bt->d_cr) || !bt->stream) {
+        set_error("batch is missing a device buffer (internal defect); nothing launched");
+        return kErrHip;
+    }
+    return kOk;
+}
+
 static int batch_reset(Batch *bt) {
+    const int rc = batch_check_ready(bt);
+    if (rc) return rc;
     SL_HIP_TRY(launch_fill_u32(bt->d_progress, 0u, bt->items.size(), bt->stream));
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
     return kOk;
