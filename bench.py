@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--rows-per-lane", type=int, default=0)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -124,8 +126,22 @@ def main():
 
     ms, kms = batch.time(per_pair=False, warmup=args.warmup, iters=args.steps)
     log(f"batched: {ms:.3f} ms/step (kernel {kms:.3f} ms)")
-    pms, pkms = batch.time(per_pair=True, warmup=1, iters=args.per_pair_steps)
-    log(f"one launch per pair: {pms:.3f} ms/step (kernels {pkms:.3f} ms)")
+    if args.per_pair_steps > 0:
+        pms, pkms = batch.time(per_pair=True, warmup=1, iters=args.per_pair_steps)
+        log(f"one launch per pair: {pms:.3f} ms/step (kernels {pkms:.3f} ms)")
+    else:
+        pms = float("nan")
+    if args.copy_probe_mb:
+        # a plain device copy of known size: the scale for the PMC byte counters
+        import torch
+
+        nbytes = args.copy_probe_mb << 20
+        src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        dst = torch.add(src, 1)   # elementwise kernel: reads nbytes, writes nbytes
+        torch.cuda.synchronize()
+        del dst
+        log(f"copy probe: {nbytes} bytes read, {nbytes} bytes written")
 
     gcups = cells / (ms * 1e-3) / 1e9
     gcups_pp = cells / (pms * 1e-3) / 1e9
@@ -148,8 +164,9 @@ def main():
                                "clipped to [100, 50000], alphabet 20, half mutated (10% subs, 2% indels), seed 4",
                    "pairs": st["pairs"], "cells": cells, "tiles": st["tiles"],
                    "rows_per_lane": tiles[0], "waves_per_cu": tiles[1]},
-        "one_pair_per_launch": {"value": round(gcups_pp, 2), "unit": "GCUPS", "ms_per_step": round(pms, 3)},
-        "batched_vs_per_pair": round(gcups / gcups_pp, 2),
+        "one_pair_per_launch": ({"value": round(gcups_pp, 2), "unit": "GCUPS", "ms_per_step": round(pms, 3)}
+                                if args.per_pair_steps > 0 else None),
+        "batched_vs_per_pair": round(gcups / gcups_pp, 2) if args.per_pair_steps > 0 else None,
         "parity": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 2),
                      "unit": "Tops/s (int32 VALU lane-ops)", "frac": round(achieved_tops / peak_tops, 4),
