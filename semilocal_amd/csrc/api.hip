@@ -18,7 +18,7 @@ namespace slhip {
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t stream);
 hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
                              const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
-                             int32_t *rc, int32_t *cr, uint32_t *buf, uint32_t *progress, LaunchCtl *ctl,
+                             int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch, LaunchCtl *ctl,
                              uint32_t spin_limit);
 
 static thread_local std::string g_last_error;
@@ -224,8 +224,8 @@ struct Batch {
     void *d_pool = nullptr;
     int32_t *d_rc = nullptr;
     int32_t *d_cr = nullptr;
-    uint32_t *d_buf = nullptr;
-    uint32_t *d_progress = nullptr;
+    uint64_t *d_buf = nullptr;         // tagged inter-pass strands, one entry per column of each pair
+    uint32_t epoch = 0;                // run counter; part of the strand tags
     LaunchCtl *d_ctl = nullptr;    // [0] batched launch, [1+i] per-pair launch of pair i
     unsigned waves_cap = 0;
     size_t items_cap = 0;
@@ -239,7 +239,6 @@ static void batch_free(Batch *b) {
     (void)hipFree(b->d_rc);
     (void)hipFree(b->d_cr);
     (void)hipFree(b->d_buf);
-    (void)hipFree(b->d_progress);
     (void)hipFree(b->d_ctl);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -275,12 +274,9 @@ static hipError_t upload_items(Batch *bt) {
         e = hipMemcpy(bt->d_pairs, bt->pairs.data(), sizeof(PairDesc) * bt->pairs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && (!bt->d_items || bt->items.size() > bt->items_cap)) {
         (void)hipFree(bt->d_items);
-        (void)hipFree(bt->d_progress);
         bt->d_items = nullptr;
-        bt->d_progress = nullptr;
         bt->items_cap = std::max<size_t>(bt->items.size(), 1);
         e = hipMalloc((void **)&bt->d_items, sizeof(WorkItem) * bt->items_cap);
-        if (e == hipSuccess) e = hipMalloc((void **)&bt->d_progress, sizeof(uint32_t) * bt->items_cap);
     }
     if (e == hipSuccess && !bt->items.empty())
         e = hipMemcpy(bt->d_items, bt->items.data(), sizeof(WorkItem) * bt->items.size(), hipMemcpyHostToDevice);
@@ -383,7 +379,8 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     if (e == hipSuccess) e = hipMalloc((void **)&bt->d_rc, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
     if (e == hipSuccess && bt->with_cr)
         e = hipMalloc((void **)&bt->d_cr, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
-    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_buf, sizeof(uint32_t) * std::max<int64_t>(bt->buf_total, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_buf, sizeof(uint64_t) * std::max<int64_t>(bt->buf_total, 1));
+    if (e == hipSuccess) e = hipMemset(bt->d_buf, 0, sizeof(uint64_t) * std::max<int64_t>(bt->buf_total, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&bt->d_ctl, sizeof(LaunchCtl) * (size_t)(npairs + 1));
     if (e != hipSuccess) {
         set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
@@ -414,7 +411,7 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
 
 // Host-side guard: every device buffer a launch touches must exist.
 static int batch_check_ready(const Batch *bt) {
-    if (!bt->d_pairs || !bt->d_items || !bt->d_pool || !bt->d_rc || !bt->d_buf || !bt->d_progress || !bt->d_ctl ||
+    if (!bt->d_pairs || !bt->d_items || !bt->d_pool || !bt->d_rc || !bt->d_buf || !bt->d_ctl ||
         (bt->with_cr && !bt->d_cr) || !bt->stream) {
         set_error("batch is missing a device buffer (internal defect); nothing launched");
         return kErrHip;
@@ -425,7 +422,8 @@ static int batch_check_ready(const Batch *bt) {
 static int batch_reset(Batch *bt) {
     const int rc = batch_check_ready(bt);
     if (rc) return rc;
-    SL_HIP_TRY(launch_fill_u32(bt->d_progress, 0u, bt->items.size(), bt->stream));
+    bt->epoch = (bt->epoch + 1u) & 0xFFFFu;   // new tags for this run's strand hand-offs
+    if (bt->epoch == 0) bt->epoch = 1;         // tag 0 is the buffer's initial content
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
     return kOk;
 }
@@ -435,7 +433,7 @@ static hipError_t launch_items(Batch *bt, const WorkItem *items, uint32_t nitems
     const void *pool = bt->d_pool;
     // per-pair pointers are resolved inside the kernel from PairDesc offsets
     return launch_comb_batch(bt->sym_bytes, bt->rows_per_lane, waves, bt->stream, bt->d_pairs, items, nitems, pool,
-                             bt->d_rc, bt->d_cr, bt->d_buf, bt->d_progress, ctl, bt->spin_limit);
+                             bt->d_rc, bt->d_cr, bt->d_buf, bt->epoch, ctl, bt->spin_limit);
 }
 
 static int batch_run(Batch *bt) {

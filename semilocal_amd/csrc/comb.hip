@@ -18,11 +18,10 @@
 //    two LDS reads and one LDS write.
 //  * passes of one pair run concurrently on different waves: pass p+1 reads
 //    the bottom strands pass p wrote to the pair's strand buffer, in place,
-//    after pass p's per-pass progress counter says the chunk is there
-//    (release/acquire at agent scope).  Queue tickets are handed out in
-//    (pair, pass) order, so a pass only waits on a pass that is already
-//    running; every wait is bounded (spin limit) and a timeout poisons the
-//    counter so that the grid still drains.
+//    as tagged 64-bit entries (strand + writer tag) with device-coherent
+//    accesses.  Queue tickets are handed out in (pair, pass) order, so a pass
+//    only waits on a pass that is already running; every wait is bounded
+//    (spin limit) and a timeout makes every wave of the launch leave.
 //
 // Output: row->col (and optionally col->row) of the (m+n)-permutation,
 // written straight into the caller's layout, including the reverse mapping of
@@ -57,45 +56,31 @@ __device__ __forceinline__ void emit(int32_t *rc, int32_t *cr, uint32_t size, bo
     if (cr) cr[c] = (int32_t)s;
 }
 
-// Inter-pass hand-off.  The strands one pass passes to the next and the
-// per-pass progress counters are only ever accessed with device-coherent
-// (agent-scope) relaxed atomics, which gfx950 performs with the sc1 bit, past
-// the per-XCD L2s.  The producer therefore needs no L2 write-back and the
-// consumer no L2 invalidate (an agent-scope release/acquire fence would issue
-// both on every chunk); ordering comes from waiting for the wave's stores
-// (vmcnt(0)) before the counter is bumped, and from the counter spin loop
-// before the strands are read.
-__device__ __forceinline__ uint32_t ld_coherent(const uint32_t *p) {
+// Inter-pass hand-off ("tagged strands").  Pass p+1 of a pair reads the
+// bottom strands of pass p from the pair's strand buffer, in place.  Every
+// entry is 64 bits: the strand in the low word and, in the high word, a tag =
+// (run epoch << 16) | (index of the writing pass + 1).  Entries are written and
+// read with device-coherent (agent-scope) relaxed atomics -- single-copy
+// atomic 64-bit accesses that gfx950 performs with the sc1 bit, past the
+// per-XCD L2s -- so a consumer that reads the tag it expects has the strand
+// that goes with it.  No progress counter, no fence and no wait for store
+// completion is needed; a consumer whose chunk is not there yet re-reads it
+// (bounded).  Stale entries carry an older tag: every run rewrites every entry
+// of a multi-pass pair, and a run's epoch differs from the previous run's.
+__device__ __forceinline__ uint64_t ld_coherent64(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_coherent(uint32_t *p, uint32_t v) {
+__device__ __forceinline__ void st_coherent64(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// s_waitcnt vmcnt(0) (GFX9 encoding: vmcnt=0, expcnt=7, lgkmcnt=15): every
-// load and store this wave issued has been performed.
-__device__ __forceinline__ void wait_vm_idle() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
-// Bounded wait until progress[idx] >= need.  Returns false on timeout (or when
-// another wave already reported one), after flagging the error.
-__device__ __forceinline__ bool wait_progress(const uint32_t *progress, uint32_t idx, uint32_t need,
-                                              LaunchCtl *ctl, uint32_t spin_limit) {
-    uint32_t spins = 0;
-    while (ld_coherent(&progress[idx]) < need) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins >= spin_limit ||
-            __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            __hip_atomic_fetch_or(&ctl->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");   // later strand loads stay after the spin
-    return true;
+__device__ __forceinline__ uint32_t strand_tag(uint32_t epoch, uint32_t writer_pass) {
+    return (epoch << 16) | ((writer_pass + 1u) & 0xFFFFu);
 }
-
-__device__ __forceinline__ void publish(uint32_t *progress, uint32_t idx, uint32_t value, int lane) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // strand stores stay before the counter
-    wait_vm_idle();
-    if (lane == 0) st_coherent(&progress[idx], value);
+__device__ __forceinline__ bool launch_failed(LaunchCtl *ctl) {
+    return __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+__device__ __forceinline__ void report_timeout(LaunchCtl *ctl) {
+    __hip_atomic_fetch_or(&ctl->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // LDS words of one wave's private staging area: column values of the
@@ -140,7 +125,7 @@ __device__ __forceinline__ uint32_t sym_value(uint32_t code, uint32_t tag_shift)
 
 template <int R, bool ONEHOT, typename Sym>
 __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ pool, int32_t *__restrict__ rc,
-                          int32_t *__restrict__ cr, uint32_t *__restrict__ buf, uint32_t *progress,
+                          int32_t *__restrict__ cr, uint64_t *__restrict__ buf, uint32_t epoch,
                           LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W) {
     const int lane = threadIdx.x & 63;
     const uint32_t m = (uint32_t)d.m, n = (uint32_t)d.n, size = m + n;
@@ -148,12 +133,13 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
     const bool reverse = d.reverse != 0;
     const Sym *rows = pool + d.rows_off;
     const Sym *cols = pool + d.cols_off;
-    uint32_t *sbuf = buf + d.buf_off;
+    uint64_t *sbuf = buf + d.buf_off;
     int32_t *prc = rc + d.perm_off;
     int32_t *pcr = cr ? cr + d.perm_off : nullptr;
     const bool first = pass == 0;
     const bool last = pass == d.npasses - 1;
-    const uint32_t item = (uint32_t)(d.item0 + pass);
+    const uint32_t want_tag = strand_tag(epoch, (uint32_t)pass - 1u);   // entries written by pass-1
+    const uint64_t my_tag = (uint64_t)strand_tag(epoch, (uint32_t)pass) << 32;
     const int64_t row0 = (int64_t)pass * 64 * R + (int64_t)lane * R;
     // values of padding rows / absent columns: never equal to anything
     const uint32_t pad_row = ONEHOT ? 0u : 0x10000u;
@@ -170,28 +156,46 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
     }
 
     const uint32_t nchunks = (n + 63u + 63u) / 64u;   // steps 0 .. n+62 inclusive
-    auto load_chunk = [&](uint32_t j, uint32_t &tin, uint32_t &bin) {
+    // Chunk j = columns [64j, 64j+64): lane k holds column 64j+k.  For pass > 0
+    // the top strands come from the strand buffer as tagged entries.
+    auto issue_chunk = [&](uint32_t j, uint64_t &tv, uint32_t &bin) {
         const uint32_t col = 64u * j + (uint32_t)lane;
         if (col < n) {
-            tin = first ? (m + col) : ld_coherent(&sbuf[col]);   // top strand ids m+j (semi_local.h:77-80)
+            tv = first ? (uint64_t)(m + col) : ld_coherent64(&sbuf[col]);   // top strand ids m+j (semi_local.h:77-80)
             bin = sym_value<ONEHOT>((uint32_t)cols[col], tag_shift);
         } else {
-            tin = kNoStrand;
+            tv = kNoStrand;
             bin = pad_col;
         }
     };
-    auto chunk_ready = [&](uint32_t j) -> bool {
+    // Bounded wait until every lane's entry of chunk j carries want_tag.
+    auto settle_chunk = [&](uint32_t j, uint64_t &tv) -> bool {
         if (first) return true;
-        const uint32_t need = (j + 2u < nchunks) ? j + 2u : nchunks;
-        return wait_progress(progress, item - 1u, need, ctl, spin_limit);
+        const uint32_t col = 64u * j + (uint32_t)lane;
+        uint32_t spins = 0;
+        for (;;) {
+            const bool ready = col >= n || (uint32_t)(tv >> 32) == want_tag;
+            if (__builtin_amdgcn_ballot_w64(!ready) == 0) return true;
+            if (++spins >= spin_limit || launch_failed(ctl)) {
+                report_timeout(ctl);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            if (!ready) tv = ld_coherent64(&sbuf[col]);
+        }
     };
 
-    uint32_t tin_nx = kNoStrand, bin_nx = pad_col;
-    bool ok = chunk_ready(0);
-    if (ok) load_chunk(0, tin_nx, bin_nx);
+    uint64_t tv_nx = kNoStrand;
+    uint32_t bin_nx = pad_col;
+    issue_chunk(0, tv_nx, bin_nx);
     uint32_t t = kNoStrand, b_prev = pad_col;
-    for (uint32_t j = 0; j < nchunks && ok; ++j) {
-        const uint32_t tin = tin_nx, bin = bin_nx;
+    bool ok = true;
+    for (uint32_t j = 0; j < nchunks; ++j) {
+        if (!settle_chunk(j, tv_nx)) {
+            ok = false;
+            break;
+        }
+        const uint32_t tin = (uint32_t)tv_nx, bin = bin_nx;
         W[kWinB + lane] = b_prev;
         W[kWinB + 64 + lane] = bin;
         W[kInjT + lane] = tin;
@@ -213,12 +217,9 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
             }
         };
         steps(0, 32);
-        // fetch chunk j+1 half-way through chunk j: it is needed when chunk j+1
-        // starts, and asking for it later shortens the lag behind the previous pass
-        if (j + 1u < nchunks) {
-            ok = chunk_ready(j + 1u);
-            if (ok) load_chunk(j + 1u, tin_nx, bin_nx);
-        }
+        // read chunk j+1 half-way through chunk j: it is needed when chunk j+1
+        // starts, and reading it late shortens the lag behind the previous pass
+        if (j + 1u < nchunks) issue_chunk(j + 1u, tv_nx, bin_nx);
         steps(32, 64);
         wave_lds_sync();
         // slot 1+k was written last by lane 63 (step k): final top strand of column 64j+k-63
@@ -226,16 +227,11 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
         const int64_t col = 64 * (int64_t)j + lane - 63;
         if (col >= 0 && col < (int64_t)n) {
             if (last) emit(prc, pcr, size, reverse, acc, (uint32_t)col, ctl);
-            else st_coherent(&sbuf[col], acc);
+            else st_coherent64(&sbuf[col], my_tag | acc);
         }
-        if (!last) publish(progress, item, j + 1u, lane);
         wave_lds_sync();
     }
-    if (!ok) {
-        // timed out: let any successor drain immediately (results are void)
-        if (!last) publish(progress, item, 0xFFFFFFFFu, lane);
-        return;
-    }
+    if (!ok) return;   // timed out (reported); successors see launch_failed and leave
     // right-edge exits: row i leaves at column n + (m-1-i)  (semi_local.h:93-95)
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -254,8 +250,8 @@ template <typename Sym, int R>
 __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restrict__ pairs,
                                                          const WorkItem *__restrict__ items, uint32_t nitems,
                                                          const Sym *__restrict__ pool, int32_t *__restrict__ rc,
-                                                         int32_t *__restrict__ cr, uint32_t *__restrict__ buf,
-                                                         uint32_t *progress, LaunchCtl *ctl, uint32_t spin_limit) {
+                                                         int32_t *__restrict__ cr, uint64_t *__restrict__ buf,
+                                                         uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit) {
     __shared__ uint32_t lds[kWavesPerBlock * kWaveLds];
     const int lane = threadIdx.x & 63;
     uint32_t *W = lds + (threadIdx.x >> 6) * kWaveLds;   // this wave's private staging area
@@ -267,9 +263,9 @@ __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restr
         const WorkItem it = items[tk];
         const PairDesc d = pairs[it.pair];
         if (d.onehot)
-            comb_pass<R, true, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit, W);
+            comb_pass<R, true, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W);
         else
-            comb_pass<R, false, Sym>(d, it.pass, pool, rc, cr, buf, progress, ctl, spin_limit, W);
+            comb_pass<R, false, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W);
     }
 }
 
@@ -288,19 +284,19 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStrea
 
 template <typename Sym, int R>
 static void launch_one(unsigned blocks, hipStream_t stream, const PairDesc *pairs, const WorkItem *items,
-                       uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr, uint32_t *buf, uint32_t *progress,
+                       uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch,
                        LaunchCtl *ctl, uint32_t spin_limit) {
     hipLaunchKernelGGL((comb_batch_kernel<Sym, R>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, stream, pairs, items,
-                       nitems, (const Sym *)pool, rc, cr, buf, progress, ctl, spin_limit);
+                       nitems, (const Sym *)pool, rc, cr, buf, epoch, ctl, spin_limit);
 }
 
 hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
                              const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
-                             int32_t *rc, int32_t *cr, uint32_t *buf, uint32_t *progress, LaunchCtl *ctl,
+                             int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch, LaunchCtl *ctl,
                              uint32_t spin_limit) {
     if (nitems == 0 || waves == 0) return hipSuccess;
     const unsigned blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
-#define SL_LAUNCH(SYM, RR) launch_one<SYM, RR>(blocks, stream, pairs, items, nitems, pool, rc, cr, buf, progress, ctl, spin_limit)
+#define SL_LAUNCH(SYM, RR) launch_one<SYM, RR>(blocks, stream, pairs, items, nitems, pool, rc, cr, buf, epoch, ctl, spin_limit)
     if (sym_bytes == 1) {
         if (rows_per_lane == 4) SL_LAUNCH(uint8_t, 4);
         else if (rows_per_lane == 8) SL_LAUNCH(uint8_t, 8);
