@@ -116,6 +116,29 @@ int sl_comb_batch(int npairs, const void *a_cat, const int64_t *a_off, const int
                   const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes,
                   int32_t *row_to_col_cat, int32_t *col_to_row_cat);
 
+/* ------------------------------------------------------------------------ */
+/* Bit-parallel LLCS of two binary strings (1-bit strands, formula 2).        */
+/* Replaces: prefix_lcs_via_semi_local::binary::llcs_2symbol_smart_combing<W>  */
+/*   (W *a_reverse, int a_size, W *b, int b_size, int a_total_symbols,        */
+/*    int threads_num, bool formula_one) -> int                                */
+/*   -- prefix_lcs/bitwise/transposition_network_binary_alphabet.h:598-599.    */
+/* Same packing: a reversed, MSB-first uint64 words (encode_reverse,          */
+/* encoders_and_decoders.h:100-139); b LSB-first (encode, :50-87).  Like the  */
+/* reference, lengths are expected to be multiples of 64.  threads_num and    */
+/* formula_one are accepted and ignored (the score is the same).              */
+/* ------------------------------------------------------------------------ */
+int sl_llcs_bitparallel(const uint64_t *a_rev, int a_words, const uint64_t *b, int b_words,
+                        int a_total_symbols, int threads_num, int formula_one, int *llcs);
+
+/* Resident form (inputs uploaded once; run/time repeatedly). */
+typedef struct sl_bitpar sl_bitpar;
+int sl_bitpar_create(sl_bitpar **out, const uint64_t *a_rev, int a_words, const uint64_t *b,
+                     int b_words, int a_total_symbols);
+int sl_bitpar_run(sl_bitpar *plan);                     /* asynchronous */
+int sl_bitpar_result(sl_bitpar *plan, int *llcs);      /* waits, then reads the score */
+int sl_bitpar_time(sl_bitpar *plan, int warmup, int iters, double *ms_per_run);
+void sl_bitpar_destroy(sl_bitpar *plan);
+
 #ifdef __cplusplus
 }
 #endif

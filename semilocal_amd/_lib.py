@@ -54,6 +54,12 @@ SIGNATURES = {
     "sl_batch_autotune": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "sl_batch_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "sl_batch_destroy": ([_vp], None),
+    "sl_llcs_bitparallel": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "sl_bitpar_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int], C.c_int),
+    "sl_bitpar_run": ([_vp], C.c_int),
+    "sl_bitpar_result": ([_vp, C.POINTER(C.c_int)], C.c_int),
+    "sl_bitpar_time": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "sl_bitpar_destroy": ([_vp], None),
     "sl_comb_batch": ([C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, _vp, _vp], C.c_int),
 }
 
@@ -222,3 +228,65 @@ def comb_batch(pairs, with_col_to_row: bool = False):
         return b.run().results()
     finally:
         b.close()
+
+
+def pack_bits_reverse(bits) -> np.ndarray:
+    """a reversed, MSB-first uint64 words (the reference's encode_reverse layout)."""
+    bits = np.asarray(bits, dtype=np.uint8) & 1
+    words = (len(bits) + 63) // 64
+    padded = np.zeros(words * 64, np.uint8)
+    padded[:len(bits)] = bits
+    # word w holds bits[64w .. 64w+63] with bits[64w] at bit 63; stored at index words-1-w
+    packed = np.packbits(padded.reshape(words, 64), axis=1, bitorder="big").view(">u8").astype(np.uint64).ravel()
+    return np.ascontiguousarray(packed[::-1])
+
+
+def pack_bits(bits) -> np.ndarray:
+    """LSB-first uint64 words (the reference's encode layout)."""
+    bits = np.asarray(bits, dtype=np.uint8) & 1
+    words = (len(bits) + 63) // 64
+    padded = np.zeros(words * 64, np.uint8)
+    padded[:len(bits)] = bits
+    return np.ascontiguousarray(np.packbits(padded.reshape(words, 64), axis=1, bitorder="little").view("<u8").astype(np.uint64).ravel())
+
+
+class BitParallel:
+    """Resident bit-parallel LLCS problem (sl_bitpar_* entry points)."""
+
+    def __init__(self, a_bits, b_bits):
+        self.ar = pack_bits_reverse(a_bits)
+        self.bw = pack_bits(b_bits)
+        h = _vp()
+        check(lib().sl_bitpar_create(C.byref(h), _ptr(self.ar), len(self.ar), _ptr(self.bw), len(self.bw),
+                                     len(a_bits)))
+        self.h = h
+
+    def llcs(self) -> int:
+        out = C.c_int()
+        check(lib().sl_bitpar_run(self.h))
+        check(lib().sl_bitpar_result(self.h, C.byref(out)))
+        return out.value
+
+    def time(self, warmup: int = 1, iters: int = 3) -> float:
+        ms = C.c_double()
+        check(lib().sl_bitpar_time(self.h, warmup, iters, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sl_bitpar_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def llcs_bitparallel(a_bits, b_bits) -> int:
+    """LLCS of two binary strings by bit-parallel combing (one-shot)."""
+    ar, bw = pack_bits_reverse(a_bits), pack_bits(b_bits)
+    out = C.c_int()
+    check(lib().sl_llcs_bitparallel(_ptr(ar), len(ar), _ptr(bw), len(bw), len(a_bits), 0, 0, C.byref(out)))
+    return out.value

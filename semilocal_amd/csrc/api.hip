@@ -40,7 +40,7 @@ static DeviceState g_dev;
 static std::mutex g_dev_mu;
 std::mutex g_oneshot_mu;
 
-int ensure_device(int device = -1) {
+int ensure_device(int device) {
     std::lock_guard<std::mutex> lk(g_dev_mu);
     if (g_dev.ready && (device < 0 || device == g_dev.device)) {
         SL_HIP_TRY(hipSetDevice(g_dev.device));

@@ -1,0 +1,166 @@
+// bitpar_api.hip -- host side of kernel (b): resident bit-parallel problems.
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/semilocal_hip.h"
+#include "common.h"
+
+namespace slhip {
+
+struct BitDesc {
+    const uint32_t *a_rev;
+    const uint32_t *b;
+    uint64_t *buf;
+    uint32_t *dis;
+    int32_t M, N;
+    int32_t npasses;
+    uint32_t epoch;
+};
+hipError_t launch_bitpar(const BitDesc &d, unsigned waves, LaunchCtl *ctl, uint32_t spin_limit, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t stream);
+
+struct BitPlan {
+    BitDesc d{};
+    int64_t a_total = 0;
+    uint32_t *d_words = nullptr;   // a_rev (M words) then b (N words), 32-bit view
+    uint64_t *d_buf = nullptr;
+    uint32_t *d_scratch = nullptr; // [0,2): LaunchCtl, [2]: dis-braid counter
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    uint32_t spin_limit = 1u << 23;
+    unsigned waves = 0;
+};
+
+static void bitplan_free(BitPlan *p) {
+    if (!p) return;
+    (void)hipFree(p->d_words);
+    (void)hipFree(p->d_buf);
+    (void)hipFree(p->d_scratch);
+    for (auto &e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+static int bitplan_run(BitPlan *p) {
+    if (!p->d_words || !p->d_buf || !p->d_scratch || !p->stream) {
+        set_error("bit-parallel plan is missing a device buffer (internal defect); nothing launched");
+        return kErrHip;
+    }
+    p->d.epoch = (p->d.epoch + 1u) & 0xFFFFu;
+    if (p->d.epoch == 0) p->d.epoch = 1;   // tag 0 is the buffer's initial content
+    SL_HIP_TRY(launch_fill_u32(p->d_scratch, 0u, 4, p->stream));
+    SL_HIP_TRY(launch_bitpar(p->d, p->waves, reinterpret_cast<LaunchCtl *>(p->d_scratch), p->spin_limit, p->stream));
+    return kOk;
+}
+
+static int bitplan_result(BitPlan *p, int *llcs) {
+    SL_HIP_TRY(hipStreamSynchronize(p->stream));
+    uint32_t h[4];
+    SL_HIP_TRY(hipMemcpy(h, p->d_scratch, sizeof h, hipMemcpyDeviceToHost));
+    if (h[1]) {
+        set_error("bit-parallel kernel: a bounded wait timed out; result is void");
+        return kErrTimeout;
+    }
+    if (llcs) *llcs = (int)(p->a_total - (int64_t)h[2]);
+    return kOk;
+}
+
+}  // namespace slhip
+
+using namespace slhip;
+
+extern "C" {
+
+int sl_bitpar_create(sl_bitpar **out, const uint64_t *a_rev, int a_words, const uint64_t *b, int b_words,
+                     int a_total_symbols) {
+    if (!out || !a_rev || !b || a_words <= 0 || b_words <= 0 || a_total_symbols < 0 ||
+        (int64_t)a_words * 64 > 0x7fffffffLL || (int64_t)b_words * 64 > 0x7fffffffLL) {
+        set_error("sl_bitpar_create: bad argument");
+        return kErrArg;
+    }
+    int rc = ensure_device();
+    if (rc) return rc;
+    BitPlan *p = new BitPlan();
+    p->a_total = a_total_symbols;
+    const int64_t M = 2LL * a_words, N = 2LL * b_words;   // 32-bit words
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&p->ev[k]);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_words, sizeof(uint32_t) * (size_t)(M + N));
+    if (e == hipSuccess) e = hipMemcpy(p->d_words, a_rev, sizeof(uint64_t) * a_words, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_words + M, b, sizeof(uint64_t) * b_words, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_buf, sizeof(uint64_t) * (size_t)N);
+    if (e == hipSuccess) e = hipMemset(p->d_buf, 0, sizeof(uint64_t) * (size_t)N);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_scratch, sizeof(uint32_t) * 4);
+    if (e != hipSuccess) {
+        set_error(std::string("sl_bitpar_create: ") + hipGetErrorString(e));
+        bitplan_free(p);
+        return e == hipErrorOutOfMemory ? kErrNoMem : kErrHip;
+    }
+    p->d.a_rev = p->d_words;
+    p->d.b = p->d_words + M;
+    p->d.buf = p->d_buf;
+    p->d.dis = p->d_scratch + 2;
+    p->d.M = (int32_t)M;
+    p->d.N = (int32_t)N;
+    p->d.npasses = (int32_t)((M + 63) / 64);
+    p->d.epoch = 0;
+    p->waves = (unsigned)std::min<int64_t>(p->d.npasses, (int64_t)device_cus() * 16);
+    *out = reinterpret_cast<sl_bitpar *>(p);
+    return kOk;
+}
+
+int sl_bitpar_run(sl_bitpar *h) {
+    if (!h) {
+        set_error("null plan");
+        return kErrArg;
+    }
+    return bitplan_run(reinterpret_cast<BitPlan *>(h));
+}
+
+int sl_bitpar_result(sl_bitpar *h, int *llcs) {
+    if (!h || !llcs) {
+        set_error("sl_bitpar_result: null argument");
+        return kErrArg;
+    }
+    return bitplan_result(reinterpret_cast<BitPlan *>(h), llcs);
+}
+
+int sl_bitpar_time(sl_bitpar *h, int warmup, int iters, double *ms_per_run) {
+    BitPlan *p = reinterpret_cast<BitPlan *>(h);
+    if (!p || iters <= 0 || warmup < 0 || !ms_per_run) {
+        set_error("sl_bitpar_time: bad argument");
+        return kErrArg;
+    }
+    int rc = kOk;
+    for (int w = 0; w < warmup && !rc; ++w) rc = bitplan_run(p);
+    if (!rc) rc = bitplan_result(p, nullptr);
+    if (rc) return rc;
+    SL_HIP_TRY(hipEventRecord(p->ev[0], p->stream));
+    for (int it = 0; it < iters && !rc; ++it) rc = bitplan_run(p);
+    if (rc) return rc;
+    SL_HIP_TRY(hipEventRecord(p->ev[1], p->stream));
+    rc = bitplan_result(p, nullptr);
+    if (rc) return rc;
+    float ms = 0.f;
+    SL_HIP_TRY(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+    *ms_per_run = ms / iters;
+    return kOk;
+}
+
+void sl_bitpar_destroy(sl_bitpar *h) { bitplan_free(reinterpret_cast<BitPlan *>(h)); }
+
+int sl_llcs_bitparallel(const uint64_t *a_rev, int a_words, const uint64_t *b, int b_words, int a_total_symbols,
+                        int threads, int formula_one, int *llcs) {
+    (void)threads;      // the GPU decides its own parallelism
+    (void)formula_one;  // both cell formulas give the same score; formula 2 is used
+    sl_bitpar *h = nullptr;
+    int rc = sl_bitpar_create(&h, a_rev, a_words, b, b_words, a_total_symbols);
+    if (rc) return rc;
+    rc = sl_bitpar_run(h);
+    if (!rc) rc = sl_bitpar_result(h, llcs);
+    sl_bitpar_destroy(h);
+    return rc;
+}
+
+}  // extern "C"

@@ -19,6 +19,8 @@ enum Status : int {
 
 void set_error(const std::string &msg);
 const char *last_error();
+int ensure_device(int device = -1);   // selects (and on first use initialises) the HIP device
+int device_cus();                     // compute units of the selected device
 
 #define SL_HIP_TRY(expr)                                                              \
     do {                                                                              \

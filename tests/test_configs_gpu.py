@@ -1,0 +1,39 @@
+"""Full-size workloads on the GPU against the reference's own outputs
+(SHA-256 of the row->col permutation, LLCS, sampled entries) and
+size-independent properties."""
+import numpy as np
+import pytest
+
+import golden_io as G
+from semilocal_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(perm, rec):
+    assert W.sha256(perm) == rec["perm_sha256"]
+    idx = np.array(rec["sample_idx"])
+    assert np.array_equal(perm[idx], np.array(rec["sample_val"]))
+
+
+def test_binary_pair_2p20_permutation_and_score(gpu):
+    rec = G.configs()["binary_pair_2p20"]
+    a, b = W.binary_pair()
+    assert W.sha256(a, b) == rec["input_sha256"]
+    perm = gpu.semi_local_comb(a, b)
+    assert np.array_equal(np.sort(perm), np.arange(len(perm)))      # a permutation
+    _check(perm, rec)
+    llcs = gpu.llcs_from_kernel(perm, len(a), len(b))
+    assert llcs == rec["llcs"] == rec["llcs_bitparallel"]
+    assert gpu.llcs_bitparallel(a, b) == rec["llcs_bitparallel"]
+
+
+def test_similar_pair_1e6_direct_combing_equals_reference_hybrid(gpu):
+    # the kernel is unique: combing the whole pair equals the reference's
+    # 8x8-block hybrid with steady-ant merges
+    rec = G.configs()["similar_pair_1e6"]
+    a, b = W.similar_pair()
+    assert W.sha256(a, b) == rec["input_sha256"]
+    perm = gpu.semi_local_comb(a, b)
+    _check(perm, rec)
+    assert gpu.llcs_from_kernel(perm, len(a), len(b)) == rec["llcs"]
