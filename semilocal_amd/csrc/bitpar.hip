@@ -96,7 +96,8 @@ struct BitDesc {
 // One pass = 64 consecutive word-rows (grid order: row g uses a_rev word
 // M-1-g, its left strands start as all ones, :619-620).  Top strands of pass
 // 0 start as zeros (:621-622).
-__device__ void bit_pass(const BitDesc &d, int pass, LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W) {
+// Returns this wave's dis-braid count for the pass (uniform across the wave).
+__device__ uint32_t bit_pass(const BitDesc &d, int pass, LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W) {
     const int lane = threadIdx.x & 63;
     const uint32_t M = (uint32_t)d.M, N = (uint32_t)d.N;
     const int64_t g = (int64_t)pass * 64 + lane;
@@ -141,8 +142,12 @@ __device__ void bit_pass(const BitDesc &d, int pass, LaunchCtl *ctl, uint32_t sp
     uint32_t bw = 0;
     issue(0, tv, bw);
     uint32_t t = 0;
+    bool ok = true;
     for (uint32_t grp = 0; grp < ngroups; ++grp) {
-        if (!settle(grp, tv)) return;
+        if (!settle(grp, tv)) {   // timed out (reported): leave the loop, no early return
+            ok = false;
+            break;
+        }
         if (lane < kGroup) {
             const uint32_t slot = (grp * kGroup + (uint32_t)lane) & (kRing - 1);
             ringB[slot] = bw;
@@ -170,22 +175,27 @@ __device__ void bit_pass(const BitDesc &d, int pass, LaunchCtl *ctl, uint32_t sp
         lds_sync();
     }
     // dis-braid: strands that entered at the left and exit at the right (:648-659)
-    uint32_t cnt = row_ok ? (uint32_t)__builtin_popcount(l) : 0u;
+    uint32_t cnt = (ok && row_ok) ? (uint32_t)__builtin_popcount(l) : 0u;
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-    if (lane == 0) atomicAdd(d.dis, cnt);
+    return __builtin_amdgcn_readfirstlane(cnt);
 }
 
 __global__ __launch_bounds__(256) void bitpar_kernel(BitDesc d, LaunchCtl *ctl, uint32_t spin_limit) {
     __shared__ uint32_t lds[kBitWavesPerBlock * kBitWaveLds];
     const int lane = threadIdx.x & 63;
     uint32_t *W = lds + (threadIdx.x >> 6) * kBitWaveLds;
+    // The wave's passes add their dis-braid counts here; one atomic after the
+    // queue is drained keeps lane-0-only work out of the ticket loop, whose
+    // readfirstlane needs the whole wave converged at the top of every trip.
+    uint32_t dis = 0;
     for (;;) {
         uint32_t tk = 0;
         if (lane == 0) tk = atomicAdd(&ctl->ticket, 1u);
         tk = __builtin_amdgcn_readfirstlane(tk);
         if (tk >= (uint32_t)d.npasses) break;   // passes are handed out in order
-        bit_pass(d, (int)tk, ctl, spin_limit, W);
+        dis += bit_pass(d, (int)tk, ctl, spin_limit, W);
     }
+    if (lane == 0 && dis != 0) atomicAdd(d.dis, dis);
 }
 
 hipError_t launch_bitpar(const BitDesc &d, unsigned waves, LaunchCtl *ctl, uint32_t spin_limit, hipStream_t stream) {
