@@ -231,12 +231,15 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
         }
         wave_lds_sync();
     }
-    if (!ok) return;   // timed out (reported); successors see launch_failed and leave
-    // right-edge exits: row i leaves at column n + (m-1-i)  (semi_local.h:93-95)
+    // right-edge exits: row i leaves at column n + (m-1-i)  (semi_local.h:93-95).
+    // Skipped after a reported timeout; no early return, so the wave reaches
+    // the ticket loop's readfirstlane converged.
+    if (ok) {
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int64_t i = row0 + q;
-        if (i < (int64_t)m) emit(prc, pcr, size, reverse, l[q], n + (uint32_t)(m - 1 - i), ctl);
+        for (int q = 0; q < R; ++q) {
+            const int64_t i = row0 + q;
+            if (i < (int64_t)m) emit(prc, pcr, size, reverse, l[q], n + (uint32_t)(m - 1 - i), ctl);
+        }
     }
 }
 
