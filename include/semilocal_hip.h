@@ -3,9 +3,11 @@
  *
  * Plain pointers and sizes only; no C++ or torch types cross this boundary.
  * Every entry point returns an int status (SL_OK = 0) and leaves a message for
- * sl_last_error() on failure; nothing throws across the ABI.  Calls are
- * thread-safe: each handle owns its HIP stream, and the one-shot entry points
- * serialise on an internal lock.
+ * sl_last_error() on failure; nothing throws across the ABI.  Calls on
+ * different handles may run concurrently from different threads: each handle
+ * owns its HIP stream and device buffers, and every one-shot entry point
+ * builds, uses and frees a private handle.  Calls on one handle must not
+ * overlap.
  *
  * Result convention (identical to the reference's AbstractPermutation /
  * common::Permutation filled by the combing solvers): for strings a (length
@@ -138,6 +140,71 @@ int sl_bitpar_run(sl_bitpar *plan);                     /* asynchronous */
 int sl_bitpar_result(sl_bitpar *plan, int *llcs);      /* waits, then reads the score */
 int sl_bitpar_time(sl_bitpar *plan, int warmup, int iters, double *ms_per_run);
 void sl_bitpar_destroy(sl_bitpar *plan);
+
+/* ------------------------------------------------------------------------ */
+/* Unit-Monge (sticky braid) multiplication R = P (.) Q of n x n permutations  */
+/* (row -> col arrays), level-by-level steady ant on the GPU.                 */
+/* Replaces: steady_ant::steady_ant_parallel_wrapper(p, q, product, map,      */
+/*   nested_lvls) -- unit_monge_mult/steady_ant.h:811-812 (and the sequential */
+/*   steady_ant_optimized_seq, :378-453);                                     */
+/*   common::BraidMultiplicationStrategy::multiply(p, q, product) --          */
+/*   common/include/Permutations/MongeMultuplicationStrategy.h:18-21.         */
+/* Inputs must be full permutations of [0, n) (no NOPOINT entries); anything  */
+/* else is rejected with SL_ERR_ARG.  The precalc map of the reference is not */
+/* needed (base cases are solved on the device).                              */
+/* ------------------------------------------------------------------------ */
+int sl_braid_multiply(const int32_t *p, const int32_t *q, int n, int32_t *r);
+/* A batch of independent products of sizes `sizes[i]`, concatenated densely:
+ * product i reads p_cat/q_cat and writes r_cat at the sum of the previous
+ * sizes.  All products of the batch run in the same level launches. */
+int sl_braid_multiply_batch(int nprob, const int32_t *p_cat, const int32_t *q_cat,
+                            const int32_t *sizes, int32_t *r_cat);
+/* Times `iters` device runs of the batch (inputs resident in HBM) after
+ * `warmup` runs, with HIP events.  elements_levels (may be NULL) receives the
+ * sum over recursion levels of the elements processed (the work unit). */
+int sl_braid_multiply_time(int nprob, const int32_t *p_cat, const int32_t *q_cat,
+                           const int32_t *sizes, int warmup, int iters, double *ms_per_run,
+                           int64_t *elements_levels);
+
+/* Staggered multiplication: glues two adjacent block braids sharing k strands.
+ * Replaces: steady_ant::staggered_sticky_multiplication<RowGlue, UseParallel>
+ *   (p, q, k, map, product, nested_parall_regions) -- steady_ant.h:609-612;
+ *   StaggeredStickyBraidMultiplication::staggeredStickyMultiplication<RowGlue>
+ *   -- MongeMultuplicationStrategy.h:679-758.
+ * p is pr x pr, q is qr x qr (row -> col); the product is (pr + qr - k)
+ * square.  row_glue != 0: vertical neighbours, p combs the earlier rows of a
+ * (upper block), q the later ones, k = the shared columns; row_glue == 0:
+ * horizontal neighbours, p combs the earlier columns of b, k = the shared
+ * rows.  k == pr == qr is rejected with SL_ERR_ARG: the reference prints a
+ * message and leaves the product unset there (steady_ant.h:616-619). */
+int sl_staggered_multiply(const int32_t *p, int pr, const int32_t *q, int qr, int k, int row_glue,
+                          int32_t *product);
+
+/* ------------------------------------------------------------------------ */
+/* Hybrid block decomposition: small_m x small_n blocks combed in one batched */
+/* launch, then merged level by level with staggered multiplications (all     */
+/* merges of a level in one batched multiply).  The permutation equals that   */
+/* of sl_semi_local_comb on (a, b).                                            */
+/* Replaces: semi_local::hybrid_iterative<Input,WithIf>(perm, a, a_size, b,    */
+/*   b_size, map, small_m, small_n, threads_num) -- semi_local.h:577-580;      */
+/*   semi_local::OpenMPHybridIterative::compute -- SemiLocalLCS.h:594-794.     */
+/* ------------------------------------------------------------------------ */
+int sl_hybrid_iterative(const void *a, int m, const void *b, int n, int sym_bytes, int small_m,
+                        int small_n, int32_t *row_to_col);
+
+/* Resident form: strings uploaded and every level planned once. */
+typedef struct sl_hybrid sl_hybrid;
+int sl_hybrid_create(sl_hybrid **out, const void *a, int m, const void *b, int n, int sym_bytes,
+                     int small_m, int small_n);
+int sl_hybrid_run(sl_hybrid *h);                        /* combing + merges, waits */
+int sl_hybrid_download(sl_hybrid *h, int32_t *row_to_col);
+/* Times `iters` runs after `warmup` runs (HIP events): total, combing part
+ * and merge part per run (comb_ms / merge_ms may be NULL). */
+int sl_hybrid_time(sl_hybrid *h, int warmup, int iters, double *ms_per_run, double *comb_ms,
+                   double *merge_ms);
+/* Cells combed per run (sum of block areas) and number of merge levels. */
+int sl_hybrid_stats(const sl_hybrid *h, int64_t *cells, int *merge_levels);
+void sl_hybrid_destroy(sl_hybrid *h);
 
 #ifdef __cplusplus
 }

@@ -416,7 +416,8 @@ void orc_steady_ant(const int32_t *p, const int32_t *q, int n, int32_t *r)
 
 /* staggered_sticky_multiplication<RowGlue,false>, steady_ant.h:609-737.
  * p is (pr x pr), q is (qr x qr), k shared strands; product is
- * (pr+qr-k) square.  row_glue != 0 glues vertically (q above p).            */
+ * (pr+qr-k) square.  row_glue != 0 glues vertical neighbours (p = the block *
+ * of the earlier rows of a, as semi_local.h:711-715 calls it).               */
 void orc_staggered(const int32_t *p, int pr, const int32_t *q, int qr, int k, int row_glue, int32_t *prod)
 {
     int N = pr + qr - k;

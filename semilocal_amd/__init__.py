@@ -5,11 +5,13 @@ the C ABI in include/semilocal_hip.h, plus the C++ drop-in strategies in
 include/semilocal_hip.hpp).  This package builds it and exposes a thin
 Python binding used by the tests and the benchmark.
 """
-from ._lib import (Batch, BitParallel, SemiLocalError, comb_batch, device_info, lib, llcs_bitparallel,
-                   semi_local_comb)
+from ._lib import (Batch, BitParallel, Hybrid, SemiLocalError, braid_multiply, braid_multiply_batch,
+                   braid_multiply_time, comb_batch, device_info, hybrid_iterative, lib, llcs_bitparallel,
+                   semi_local_comb, staggered_multiply)
 
-__all__ = ["Batch", "BitParallel", "SemiLocalError", "comb_batch", "device_info", "lib", "llcs_bitparallel",
-           "semi_local_comb", "llcs_from_kernel"]
+__all__ = ["Batch", "BitParallel", "Hybrid", "SemiLocalError", "braid_multiply", "braid_multiply_batch",
+           "braid_multiply_time", "comb_batch", "device_info", "hybrid_iterative", "lib", "llcs_bitparallel",
+           "semi_local_comb", "staggered_multiply", "llcs_from_kernel"]
 
 
 def llcs_from_kernel(perm, m: int, n: int) -> int:

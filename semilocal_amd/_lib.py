@@ -61,6 +61,19 @@ SIGNATURES = {
     "sl_bitpar_time": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "sl_bitpar_destroy": ([_vp], None),
     "sl_comb_batch": ([C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, _vp, _vp], C.c_int),
+    "sl_braid_multiply": ([_vp, _vp, C.c_int, _vp], C.c_int),
+    "sl_braid_multiply_batch": ([C.c_int, _vp, _vp, _vp, _vp], C.c_int),
+    "sl_braid_multiply_time": ([C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                C.POINTER(C.c_int64)], C.c_int),
+    "sl_staggered_multiply": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp], C.c_int),
+    "sl_hybrid_iterative": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp], C.c_int),
+    "sl_hybrid_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "sl_hybrid_run": ([_vp], C.c_int),
+    "sl_hybrid_download": ([_vp, _vp], C.c_int),
+    "sl_hybrid_time": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                        C.POINTER(C.c_double)], C.c_int),
+    "sl_hybrid_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)], C.c_int),
+    "sl_hybrid_destroy": ([_vp], None),
 }
 
 
@@ -290,3 +303,124 @@ def llcs_bitparallel(a_bits, b_bits) -> int:
     out = C.c_int()
     check(lib().sl_llcs_bitparallel(_ptr(ar), len(ar), _ptr(bw), len(bw), len(a_bits), 0, 0, C.byref(out)))
     return out.value
+
+
+# ---------------------------------------------------------------------------
+# Braid multiplication (kernel (c)), staggered glue and the hybrid scheduler.
+# ---------------------------------------------------------------------------
+def _perm(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x), dtype=np.int32)
+
+
+def braid_multiply(p, q) -> np.ndarray:
+    """R = P (.) Q for full permutations p, q (row -> col) of equal size."""
+    p, q = _perm(p), _perm(q)
+    if p.shape != q.shape or p.ndim != 1:
+        raise ValueError("p and q must be 1-D permutations of the same size")
+    n = len(p)
+    r = np.empty(max(n, 1), np.int32)
+    check(lib().sl_braid_multiply(_ptr(p), _ptr(q), n, _ptr(r)))
+    return r[:n]
+
+
+def _mult_batch_args(pairs):
+    sizes = np.array([len(p) for p, _ in pairs], np.int32)
+    for p, q in pairs:
+        if len(p) != len(q):
+            raise ValueError("p and q must have the same size")
+    pc = _perm(np.concatenate([_perm(p) for p, _ in pairs])) if len(pairs) else np.zeros(1, np.int32)
+    qc = _perm(np.concatenate([_perm(q) for _, q in pairs])) if len(pairs) else np.zeros(1, np.int32)
+    if pc.size == 0:
+        pc = np.zeros(1, np.int32)
+        qc = np.zeros(1, np.int32)
+    return sizes, pc, qc
+
+
+def braid_multiply_batch(pairs) -> list:
+    """Products of a list of (p, q) pairs, all in the same level launches."""
+    sizes, pc, qc = _mult_batch_args(pairs)
+    out = np.empty(max(int(sizes.sum()), 1), np.int32)
+    check(lib().sl_braid_multiply_batch(len(pairs), _ptr(pc), _ptr(qc), _ptr(sizes) if len(pairs) else None,
+                                        _ptr(out)))
+    res, o = [], 0
+    for n in sizes:
+        res.append(out[o:o + n].copy())
+        o += int(n)
+    return res
+
+
+def braid_multiply_time(pairs, warmup: int = 1, iters: int = 3):
+    """(ms per run with inputs resident, sum over levels of elements processed)."""
+    sizes, pc, qc = _mult_batch_args(pairs)
+    ms, el = C.c_double(), C.c_int64()
+    check(lib().sl_braid_multiply_time(len(pairs), _ptr(pc), _ptr(qc), _ptr(sizes), warmup, iters, C.byref(ms),
+                                       C.byref(el)))
+    return ms.value, el.value
+
+
+def staggered_multiply(p, q, k: int, row_glue: bool) -> np.ndarray:
+    """Glue of two adjacent block braids sharing k strands (steady_ant.h:609-737)."""
+    p, q = _perm(p), _perm(q)
+    pr, qr = len(p), len(q)
+    out = np.empty(max(pr + qr - k, 1), np.int32)
+    check(lib().sl_staggered_multiply(_ptr(p) if pr else None, pr, _ptr(q) if qr else None, qr, int(k),
+                                      1 if row_glue else 0, _ptr(out)))
+    return out[:pr + qr - k]
+
+
+def _pair_symbols(a, b):
+    a_arr, wa = as_symbols(a)
+    b_arr, wb = as_symbols(b)
+    if wa != wb:
+        a_arr, wa = np.ascontiguousarray(a_arr, dtype=np.int32), 4
+        b_arr, wb = np.ascontiguousarray(b_arr, dtype=np.int32), 4
+    return a_arr, b_arr, wa
+
+
+class Hybrid:
+    """Resident hybrid decomposition (sl_hybrid_* entry points)."""
+
+    def __init__(self, a, b, small_m: int, small_n: int):
+        self.a, self.b, w = _pair_symbols(a, b)
+        self.m, self.n = len(self.a), len(self.b)
+        h = _vp()
+        check(lib().sl_hybrid_create(C.byref(h), _ptr(self.a) if self.m else None, self.m,
+                                     _ptr(self.b) if self.n else None, self.n, w, small_m, small_n))
+        self.h = h
+
+    def run(self) -> np.ndarray:
+        check(lib().sl_hybrid_run(self.h))
+        out = np.empty(max(self.m + self.n, 1), np.int32)
+        check(lib().sl_hybrid_download(self.h, _ptr(out)))
+        return out[:self.m + self.n]
+
+    def time(self, warmup: int = 1, iters: int = 3):
+        tot, comb, merge = C.c_double(), C.c_double(), C.c_double()
+        check(lib().sl_hybrid_time(self.h, warmup, iters, C.byref(tot), C.byref(comb), C.byref(merge)))
+        return tot.value, comb.value, merge.value
+
+    def stats(self):
+        cells, levels = C.c_int64(), C.c_int()
+        check(lib().sl_hybrid_stats(self.h, C.byref(cells), C.byref(levels)))
+        return cells.value, levels.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sl_hybrid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hybrid_iterative(a, b, small_m: int, small_n: int) -> np.ndarray:
+    """Kernel permutation of (a, b) by the hybrid block decomposition (one-shot)."""
+    a_arr, b_arr, w = _pair_symbols(a, b)
+    m, n = len(a_arr), len(b_arr)
+    out = np.empty(max(m + n, 1), np.int32)
+    check(lib().sl_hybrid_iterative(_ptr(a_arr) if m else None, m, _ptr(b_arr) if n else None, n, w, small_m,
+                                    small_n, _ptr(out)))
+    return out[:m + n]

@@ -471,6 +471,19 @@ static int batch_sync(Batch *bt) {
     return kOk;
 }
 
+// Internal interface used by the hybrid scheduler (hybrid.hip).
+int batch_create_internal(Batch **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                          const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes) {
+    return batch_create(out, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes, 0, 0);
+}
+int batch_launch_internal(Batch *bt) { return batch_run(bt); }
+int batch_wait_internal(Batch *bt) { return batch_sync(bt); }
+void batch_destroy_internal(Batch *bt) { batch_free(bt); }
+hipStream_t batch_stream_internal(const Batch *bt) { return bt->stream; }
+const int32_t *batch_rc_internal(const Batch *bt) { return bt->d_rc; }
+int64_t batch_perm_off_internal(const Batch *bt, int pair) { return bt->perm_off[(size_t)pair]; }
+int64_t batch_cells_internal(const Batch *bt) { return bt->cells; }
+
 }  // namespace slhip
 
 using namespace slhip;
