@@ -42,10 +42,7 @@ __global__ void split_scatter(const int32_t *, const int32_t *, const int32_t *,
 __global__ void solve_small(const int32_t *, const int32_t *, const int32_t *, const SaProb *, int, int32_t *);
 __global__ void merge_map(const int32_t *, const int32_t *, const int32_t *, const int32_t *, const SaProb *,
                           const int32_t *, int, int64_t, uint32_t *, uint32_t *, int32_t *, int32_t *);
-__global__ void table_hist(const uint32_t *, const int32_t *, const SaProb *, const int32_t *, const SaSeg *, int,
-                           int64_t, int32_t *, int32_t *);
-__global__ void table_rows(const int32_t *, const int32_t *, const SaSeg *, const int2 *, int, int32_t *,
-                           int32_t *);
+__global__ void table_rows(const uint32_t *, const SaSeg *, const int2 *, int32_t *, int32_t *);
 __global__ void table_cols(const SaSeg *, const int2 *, int, int32_t *, int32_t *);
 __global__ void seg_search(const uint32_t *, const uint32_t *, const int32_t *, const int32_t *, const SaSeg *,
                            const int2 *, int, const int32_t *, int4 *);
@@ -95,7 +92,7 @@ hipError_t fill(int32_t *p, int32_t v, int64_t count, hipStream_t s) {
 struct Level {
     int S = 0;
     int nseg = 0, nwalk = 0, nrow = 0, ncol = 0;
-    int64_t hist_total = 0, tab_total = 0;
+    int64_t tab_total = 0;
     size_t seg0 = 0, walk0 = 0, row0 = 0, col0 = 0, sf0 = 0;   // offsets into the packed arrays
 };
 
@@ -108,7 +105,7 @@ struct MulPlan {
     int S_top = 1, S0 = 1;
     std::vector<Level> lv;   // large levels, lv[i].S = S_top >> i
     int max_seg = 0, max_walk = 0;
-    int64_t max_hist = 0, max_tab = 0;
+    int64_t max_tab = 0;
     // device
     SaProb *d_probs = nullptr;
     int32_t *d_tile_prob = nullptr;
@@ -120,7 +117,7 @@ struct MulPlan {
     int32_t *d_R[2] = {nullptr, nullptr};
     uint32_t *d_U = nullptr, *d_Ui = nullptr;
     int32_t *d_cntP = nullptr, *d_cntQ = nullptr, *d_cend = nullptr;
-    int32_t *d_histHi = nullptr, *d_histLo = nullptr, *d_cntHi = nullptr, *d_cntLo = nullptr;
+    int32_t *d_cntHi = nullptr, *d_cntLo = nullptr;
     int4 *d_start = nullptr;
     int32_t *d_map_row = nullptr, *d_map_col = nullptr;   // staggered: core -> block rows / cols
     int result = 0;                                        // index of the R buffer holding the product
@@ -131,8 +128,7 @@ static void plan_free(MulPlan *p) {
     if (!p) return;
     void *bufs[] = {p->d_probs, p->d_tile_prob, p->d_segs, p->d_walks, p->d_rows, p->d_cols, p->d_seg_first,
                     p->d_P[0], p->d_P[1], p->d_Qi[0], p->d_Qi[1], p->d_PR, p->d_QC, p->d_R[0], p->d_R[1],
-                    p->d_U, p->d_Ui, p->d_cntP, p->d_cntQ, p->d_cend, p->d_histHi, p->d_histLo, p->d_cntHi,
-                    p->d_cntLo, p->d_start, p->d_map_row, p->d_map_col};
+                    p->d_U, p->d_Ui, p->d_cntP, p->d_cntQ, p->d_cend, p->d_cntHi, p->d_cntLo, p->d_start, p->d_map_row, p->d_map_col};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -150,6 +146,12 @@ static void plan_free(MulPlan *p) {
 
 static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool staggered) {
     *out = nullptr;
+    for (int32_t ni : sizes)
+        if ((int64_t)ni > (int64_t)kMaxNb * kBlk) {
+            set_error("braid multiply: size " + std::to_string(ni) + " exceeds the supported maximum " +
+                      std::to_string((int64_t)kMaxNb * kBlk));
+            return kErrUnsupported;
+        }
     MulPlan *p = new MulPlan();
     p->nprob = (int)sizes.size();
     p->n = sizes;
@@ -192,10 +194,8 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
                 sg.ntiles = (int32_t)ceil_div(sg.s, kTile);
                 const bool split = sg.s > S / 2;
                 sg.nb = split ? (int32_t)ceil_div(sg.s, kBlk) : 0;
-                sg.tab_off = L.hist_total;
                 sg.tab_off2 = L.tab_total;
                 if (split) {
-                    L.hist_total += (int64_t)sg.nb * sg.nb;
                     L.tab_total += (int64_t)(sg.nb + 1) * (sg.nb + 1);
                     const int nw = (int)ceil_div(2 * (int64_t)sg.s, kSegSteps);
                     for (int k = 0; k < nw; ++k) walks.push_back(make_int2(L.nseg, k));
@@ -211,7 +211,6 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
         }
         p->max_seg = std::max(p->max_seg, L.nseg);
         p->max_walk = std::max(p->max_walk, L.nwalk);
-        p->max_hist = std::max(p->max_hist, L.hist_total);
         p->max_tab = std::max(p->max_tab, L.tab_total);
         p->lv.push_back(L);
     }
@@ -238,8 +237,6 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
     PLAN_TRY(dalloc(&p->d_cntP, (size_t)p->ntiles));
     PLAN_TRY(dalloc(&p->d_cntQ, (size_t)p->ntiles));
     PLAN_TRY(dalloc(&p->d_cend, (size_t)p->max_seg));
-    PLAN_TRY(dalloc(&p->d_histHi, (size_t)p->max_hist));
-    PLAN_TRY(dalloc(&p->d_histLo, (size_t)p->max_hist));
     PLAN_TRY(dalloc(&p->d_cntHi, (size_t)p->max_tab));
     PLAN_TRY(dalloc(&p->d_cntLo, (size_t)p->max_tab));
     PLAN_TRY(dalloc(&p->d_start, (size_t)p->max_walk));
@@ -278,17 +275,13 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
         const SaSeg *segs = p->d_segs + L.seg0;
         const int2 *walks = p->d_walks + L.walk0;
         SL_HIP_TRY(fill(p->d_cend, 0, L.nseg, s));
-        SL_HIP_TRY(fill(p->d_histHi, 0, L.hist_total, s));
-        SL_HIP_TRY(fill(p->d_histLo, 0, L.hist_total, s));
         hipLaunchKernelGGL(merge_map, dim3(eb), dim3(256), 0, s, p->d_R[rc], p->d_PR + k * ntot,
                            p->d_QC + k * ntot, p->d_tile_prob, p->d_probs, p->d_seg_first + L.sf0, L.S, ntot,
                            p->d_U, p->d_Ui, p->d_R[rc ^ 1], p->d_cend);
         rc ^= 1;
         if (L.nwalk == 0) continue;
-        hipLaunchKernelGGL(table_hist, dim3(eb), dim3(256), 0, s, p->d_U, p->d_tile_prob, p->d_probs,
-                           p->d_seg_first + L.sf0, segs, L.S, ntot, p->d_histHi, p->d_histLo);
-        hipLaunchKernelGGL(table_rows, dim3((unsigned)ceil_div(L.nrow, 4)), dim3(256), 0, s, p->d_histHi,
-                           p->d_histLo, segs, p->d_rows + L.row0, L.nrow, p->d_cntHi, p->d_cntLo);
+        hipLaunchKernelGGL(table_rows, dim3((unsigned)L.nrow), dim3(256), 0, s, p->d_U, segs, p->d_rows + L.row0,
+                           p->d_cntHi, p->d_cntLo);
         hipLaunchKernelGGL(table_cols, dim3((unsigned)ceil_div(L.ncol, 4)), dim3(256), 0, s, segs,
                            p->d_cols + L.col0, L.ncol, p->d_cntHi, p->d_cntLo);
         hipLaunchKernelGGL(seg_search, dim3((unsigned)ceil_div(L.nwalk, 4)), dim3(256), 0, s, p->d_U, p->d_Ui,

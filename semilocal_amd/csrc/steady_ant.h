@@ -8,7 +8,8 @@ namespace sa {
 
 constexpr int kTile = 1024;      // elements per block of the split / small-solver kernels
 constexpr int kBlk = 1024;       // block size of the per-subproblem count tables
-constexpr int kSegSteps = 1024;  // walk steps per parallel walk segment
+constexpr int kSegSteps = 512;   // walk steps per parallel walk segment (large levels)
+constexpr int kMaxNb = 4096;     // count-table blocks per side: subproblems up to kMaxNb * kBlk
 
 // One product of the batch: elements [base, base + n) of every level array.
 // base is a multiple of kTile, so every tile belongs to exactly one problem.
@@ -21,7 +22,6 @@ struct SaProb {
 // One subproblem of a level (large levels only).
 struct SaSeg {
     int64_t x0;        // first element
-    int64_t tab_off;   // offset of its nb x nb histograms
     int64_t tab_off2;  // offset of its (nb+1) x (nb+1) count tables
     int32_t s;         // size
     int32_t tile0;     // first tile

@@ -202,38 +202,69 @@ __global__ __launch_bounds__(kTile) void split_scatter(const int32_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Small subproblems: one block solves every subproblem of size <= S0 <= kTile
-// inside its tile, from the split down to size 1 and back up, in LDS.
+// The ant passage walk, shared by the small solver and the large levels.
 // ---------------------------------------------------------------------------
-constexpr int kSmallLevels = 10;   // log2(kTile)
-
-// The reference's ant passage over one subproblem held in LDS (size s),
-// tracking delta only; good points are written over R.
-__device__ void walk_small(const int16_t *U, const uint8_t *hi, const int16_t *Ui, int s, int c_end, int16_t *R) {
-    int r = s - 1, c = 0;
-    int d = hi[r] ? 0 : -1;
-    const int t_end = s - 1 + c_end;
-    for (int t = 0; t <= t_end; ++t) {
-        const int urow = U[r];
-        const bool hrow = hi[r] != 0;
+// Runs at most `steps` evaluations of the reference's walk (steady_ant.h:
+// 186-265, tracking delta only) from (r, c) with d = delta(r, c), in a
+// subproblem of size s.  rowv(r) gives row r's point (col | hi bit 31),
+// colv(c) column c's point (row | hi bit 31); good points go to put(row, col).
+// Both candidates of the next move are read together, so a step costs one
+// memory round trip; the point of the current row and of column c-1 are
+// carried in registers.
+template <class RowF, class ColF, class PutF>
+__device__ __forceinline__ void ant_walk(int r, int c, int d, int s, int steps, RowF rowv, ColF colv, PutF put) {
+    uint32_t ur = rowv(r);
+    uint32_t left = c > 0 ? colv(c - 1) : 0u;
+    for (int t = 0; t < steps; ++t) {
+        const uint32_t up = r > 0 ? rowv(r - 1) : 0u;
+        const uint32_t nxt = c < s ? colv(c) : 0u;
         if (c > 0) {
-            const int pr = Ui[c - 1];
-            const int wcol = hi[pr] ? (pr < r) : (pr >= r);
-            const int wrow = hrow ? (urow < c) : (urow >= c);
-            if (d - wcol < 0 && d + wrow > 0) R[r] = (int16_t)(c - 1);
+            const int urow = (int)(ur & kValMask), pr = (int)(left & kValMask);
+            const int wcol = (left & kHiBit) ? (pr < r) : (pr >= r);
+            const int wrow = (ur & kHiBit) ? (urow < c) : (urow >= c);
+            if (d - wcol < 0 && d + wrow > 0) put(r, c - 1);
         }
         if (d < 0) {
             if (c >= s) break;
-            const int pr = Ui[c];
-            d += hi[pr] ? (pr < r) : (pr >= r);
+            const int pr = (int)(nxt & kValMask);
+            d += (nxt & kHiBit) ? (pr < r) : (pr >= r);
+            left = nxt;
             ++c;
         } else {
             if (r == 0) break;
             --r;
-            const int ur = U[r];
-            d -= hi[r] ? (ur < c) : (ur >= c);
+            const int u2 = (int)(up & kValMask);
+            d -= (up & kHiBit) ? (u2 < c) : (u2 >= c);
+            ur = up;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Small subproblems: one block solves every subproblem of size <= S0 <= kTile
+// inside its tile, from the split down to size 1 and back up, in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kSmallLevels = 10;   // log2(kTile)
+constexpr int kSmallSeg = 64;      // walk steps per segment inside the small solver
+constexpr uint32_t kHi16 = 0x8000u;
+
+// LDS points of the small solver: value | hi bit 15 (values < kTile).
+__device__ __forceinline__ uint32_t widen16(uint16_t x) {
+    return (uint32_t)(x & 0x7FFFu) | ((uint32_t)(x & kHi16) << 16);
+}
+
+// delta(r, c) of a subproblem held in LDS (s <= kTile rows), by one wave.
+__device__ __forceinline__ int wave_delta_lds(const uint16_t *U16, int s, int r, int c) {
+    const int lane = threadIdx.x & 63;
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < kTile / 64; ++k) {
+        const int rho = lane + 64 * k;
+        const uint32_t u = U16[rho < s ? rho : 0];
+        const int col = (int)(u & 0x7FFFu);
+        if (rho < s) acc += (u & kHi16) ? (rho < r && col < c) : -(rho >= r && col >= c);
+    }
+    return __builtin_amdgcn_readfirstlane(wave_sum(acc));
 }
 
 __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__ P0, const int32_t *__restrict__ Qi0,
@@ -242,9 +273,9 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
                                                      int32_t *__restrict__ Rout) {
     __shared__ int16_t sP[2][kTile], sQ[2][kTile];
     __shared__ int16_t sPR[kSmallLevels][kTile], sQC[kSmallLevels][kTile];
-    __shared__ int16_t sR[kTile], sU[kTile], sUi[kTile];
+    __shared__ int16_t sR[kTile];
+    __shared__ uint16_t sU[kTile], sUi[kTile];
     __shared__ int16_t segP[kTile], segQ[kTile];
-    __shared__ uint8_t sHi[kTile];
     __shared__ int sCend[kTile / 2];
     __shared__ int tmp[kTile / 64 + 1];
     const int t = blockIdx.x;
@@ -252,6 +283,7 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
     const int64_t base = (int64_t)t * kTile;
     const int len = (int)min((int64_t)kTile, pb.n - (base - pb.base));
     const int x = threadIdx.x;
+    const int lane = x & 63, wid = x >> 6;
     const bool in = x < len;
     if (in) {
         sP[0][x] = (int16_t)P0[base + x];
@@ -295,20 +327,55 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
         __syncthreads();
         if (in) {
             const int cid = x / h;
-            const int is_hi = cid & 1;
+            const uint16_t hb = (cid & 1) ? (uint16_t)kHi16 : (uint16_t)0;
             const int prow = sPR[lev][x];
             const int pcol = sQC[lev][cid * h + sR[x]];
-            sU[seg0 + prow] = (int16_t)pcol;
-            sHi[seg0 + prow] = (uint8_t)is_hi;
-            sUi[seg0 + pcol] = (int16_t)prow;
-            if (!is_hi) atomicMax(&sCend[x / S], pcol + 1);
+            sU[seg0 + prow] = (uint16_t)pcol | hb;
+            sUi[seg0 + pcol] = (uint16_t)prow | hb;
+            if (!hb) atomicMax(&sCend[x / S], pcol + 1);
         }
         __syncthreads();
-        if (in) sR[x] = sU[x];
+        if (in) sR[x] = (int16_t)(sU[x] & 0x7FFFu);
         __syncthreads();
-        if (in && x == seg0) {
-            const int s = min(S, len - x);
-            if (s > h) walk_small(sU + x, sHi + x, sUi + x, s, sCend[x / S], sR + x);
+        if (S < kSmallSeg) {
+            // short walks (< 2 kSmallSeg steps): one lane per subproblem
+            if (in && x == seg0) {
+                const int s = min(S, len - x);
+                if (s > h) {
+                    const uint16_t *U16 = sU + x, *Ui16 = sUi + x;
+                    int16_t *R = sR + x;
+                    ant_walk(s - 1, 0, (U16[s - 1] & kHi16) ? 0 : -1, s, s + sCend[x / S],
+                             [&](int rr) { return widen16(U16[rr]); }, [&](int cc) { return widen16(Ui16[cc]); },
+                             [&](int rr, int cc) { R[rr] = (int16_t)cc; });
+                }
+            }
+        } else {
+            // segments of kSmallSeg steps: start points by binary search on
+            // anti-diagonals (delta from an LDS scan), one wave per segment
+            const int nsps = (2 * S) / kSmallSeg;
+            const int nsub = (len + S - 1) / S;
+            for (int w = wid; w < nsub * nsps; w += kTile / 64) {
+                const int sub = w / nsps, k = w - sub * nsps;
+                const int s0 = sub * S, s = min(S, len - s0);
+                const int t_end = s - 1 + sCend[sub];
+                const int t0 = k * kSmallSeg;
+                if (s <= h || t0 > t_end) continue;   // uniform over the wave
+                const uint16_t *U16 = sU + s0, *Ui16 = sUi + s0;
+                int a = max(0, t0 - s + 1), b = min(t0, s);
+                while (a < b) {
+                    const int mid = (a + b) >> 1;
+                    if (wave_delta_lds(U16, s, s - t0 + mid, mid) >= 0) b = mid;   // delta(r(mid)+1, mid)
+                    else a = mid + 1;
+                }
+                const int c = a, r = s - 1 - t0 + c;
+                const int d = wave_delta_lds(U16, s, r, c);
+                if (lane == 0) {
+                    int16_t *R = sR + s0;
+                    ant_walk(r, c, d, s, min(kSmallSeg, t_end - t0 + 1),
+                             [&](int rr) { return widen16(U16[rr]); }, [&](int cc) { return widen16(Ui16[cc]); },
+                             [&](int rr, int cc) { R[rr] = (int16_t)cc; });
+                }
+            }
         }
         __syncthreads();
     }
@@ -320,87 +387,87 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
 // ---------------------------------------------------------------------------
 // Children (level S/2) -> parent (level S): U[parent row] = parent col | hi
 // bit, Ui[parent col] = parent row | hi bit, R = U without the bit, and
-// c_end[seg] = 1 + largest column of a lo point.
-__global__ void merge_map(const int32_t *__restrict__ Rc, const int32_t *__restrict__ PRc,
-                          const int32_t *__restrict__ QCc, const int32_t *__restrict__ tile_prob,
-                          const SaProb *__restrict__ probs, const int32_t *__restrict__ seg_first, int S,
-                          int64_t ntot, uint32_t *__restrict__ U, uint32_t *__restrict__ Ui,
-                          int32_t *__restrict__ R, int32_t *__restrict__ c_end) {
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= ntot) return;
+// c_end[seg] = 1 + largest column of a lo point (one atomic per block: a
+// block of 256 elements lies in one subproblem).
+__global__ __launch_bounds__(256) void merge_map(const int32_t *__restrict__ Rc, const int32_t *__restrict__ PRc,
+                                                 const int32_t *__restrict__ QCc,
+                                                 const int32_t *__restrict__ tile_prob,
+                                                 const SaProb *__restrict__ probs,
+                                                 const int32_t *__restrict__ seg_first, int S, int64_t ntot,
+                                                 uint32_t *__restrict__ U, uint32_t *__restrict__ Ui,
+                                                 int32_t *__restrict__ R, int32_t *__restrict__ c_end) {
+    __shared__ int wmax[4];
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // ntot is a multiple of 256
     const int pid = tile_prob[x / kTile];
     const SaProb pb = probs[pid];
     const int64_t lx = x - pb.base;
-    if (lx >= pb.n) return;
-    const int h = S >> 1;
-    const int64_t cid = lx / h;
-    const uint32_t bit = (cid & 1) ? kHiBit : 0u;
-    const int64_t seg0 = pb.base + (lx / S) * S;
-    const int prow = PRc[x];
-    const int pcol = QCc[pb.base + cid * h + Rc[x]];
-    U[seg0 + prow] = (uint32_t)pcol | bit;
-    Ui[seg0 + pcol] = (uint32_t)prow | bit;
-    R[seg0 + prow] = pcol;
-    if (!bit) atomicMax(&c_end[seg_first[pid] + lx / S], pcol + 1);
+    int cand = 0;
+    if (lx < pb.n) {
+        const int h = S >> 1;
+        const int64_t cid = lx / h;
+        const uint32_t bit = (cid & 1) ? kHiBit : 0u;
+        const int64_t seg0 = pb.base + (lx / S) * S;
+        const int prow = PRc[x];
+        const int pcol = QCc[pb.base + cid * h + Rc[x]];
+        U[seg0 + prow] = (uint32_t)pcol | bit;
+        Ui[seg0 + pcol] = (uint32_t)prow | bit;
+        R[seg0 + prow] = pcol;
+        if (!bit) cand = pcol + 1;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cand = max(cand, __shfl_xor(cand, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = cand;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int m = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        const int64_t lx0 = (int64_t)blockIdx.x * blockDim.x - pb.base;
+        if (m > 0) atomicMax(&c_end[seg_first[pid] + lx0 / S], m);
+    }
 }
 
-// hist*[seg][bi * nb + bj] counts the points with row in block bi and column
-// in block bj (split segments only: nb > 0).
-__global__ void table_hist(const uint32_t *__restrict__ U, const int32_t *__restrict__ tile_prob,
-                           const SaProb *__restrict__ probs, const int32_t *__restrict__ seg_first,
-                           const SaSeg *__restrict__ segs, int S, int64_t ntot, int32_t *__restrict__ histHi,
-                           int32_t *__restrict__ histLo) {
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= ntot) return;
-    const int pid = tile_prob[x / kTile];
-    const SaProb pb = probs[pid];
-    const int64_t lx = x - pb.base;
-    if (lx >= pb.n) return;
-    const SaSeg sg = segs[seg_first[pid] + lx / S];
-    if (sg.nb == 0) return;
-    const int r = (int)(lx % S);
-    const uint32_t u = U[x];
-    const int c = (int)(u & kValMask);
-    const int64_t cell = sg.tab_off + (int64_t)(r / kBlk) * sg.nb + c / kBlk;
-    atomicAdd((u & kHiBit) ? &histHi[cell] : &histLo[cell], 1);
-}
-
-// Row pass, one wave per (segment, block-row i): along block-columns j, hi
-// gets an exclusive prefix and lo an inclusive suffix.  Written transposed:
-// T[seg][j * (nb+1) + i] for j in [0, nb].
-__global__ void table_rows(const int32_t *__restrict__ histHi, const int32_t *__restrict__ histLo,
-                           const SaSeg *__restrict__ segs, const int2 *__restrict__ rows, int nrows,
-                           int32_t *__restrict__ cntHi, int32_t *__restrict__ cntLo) {
-    const int w = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
-    if (w >= nrows) return;
-    const int lane = threadIdx.x & 63;
-    const int2 job = rows[w];   // (segment, block-row)
+// Count tables of a split subproblem, row pass: one block per (subproblem,
+// block-row i).  The points of rows [i kBlk, (i+1) kBlk) are binned by
+// column block in LDS; along block-columns j, hi counts get an exclusive
+// prefix and lo counts an inclusive suffix, written transposed:
+// T[seg][j * (nb+1) + i], j in [0, nb].
+__global__ __launch_bounds__(256) void table_rows(const uint32_t *__restrict__ U, const SaSeg *__restrict__ segs,
+                                                  const int2 *__restrict__ rows, int32_t *__restrict__ cntHi,
+                                                  int32_t *__restrict__ cntLo) {
+    __shared__ int hh[kMaxNb], hl[kMaxNb];
+    __shared__ int tmp[256 / 64 + 1];
+    const int2 job = rows[blockIdx.x];   // (segment, block-row)
     const SaSeg sg = segs[job.x];
     const int nb = sg.nb, W = nb + 1, i = job.y;
-    const int32_t *hh = histHi + sg.tab_off + (int64_t)i * nb;
-    const int32_t *hl = histLo + sg.tab_off + (int64_t)i * nb;
+    for (int j = threadIdx.x; j < nb; j += 256) hh[j] = hl[j] = 0;
+    __syncthreads();
+    const int r0 = i * kBlk, r1 = min(sg.s, r0 + kBlk);
+    for (int rr = r0 + (int)threadIdx.x; rr < r1; rr += 256) {
+        const uint32_t u = U[sg.x0 + rr];
+        const int cb = (int)(u & kValMask) / kBlk;
+        atomicAdd((u & kHiBit) ? &hh[cb] : &hl[cb], 1);
+    }
+    __syncthreads();
     int32_t *ch = cntHi + sg.tab_off2;
     int32_t *cl = cntLo + sg.tab_off2;
-    int carry = 0;
-    for (int j0 = 0; j0 <= nb; j0 += 64) {
-        const int j = j0 + lane;
-        const int v = j < nb ? hh[j] : 0;
-        const int inc = wave_incl_scan(v);
-        if (j <= nb) ch[(int64_t)j * W + i] = carry + inc - v;
-        carry += __shfl(inc, 63);
+    int carryH = 0, carryL = 0, totalL = 0;
+    for (int j = threadIdx.x; j < nb; j += 256) totalL += hl[j];
+    {
+        int tt;
+        (void)block_excl_scan(totalL, tmp, tt);
+        totalL = tt;
     }
-    int total = 0;
-    for (int j0 = 0; j0 < nb; j0 += 64) {
-        const int j = j0 + lane;
-        total += wave_sum(j < nb ? hl[j] : 0);
-    }
-    carry = 0;
-    for (int j0 = 0; j0 <= nb; j0 += 64) {
-        const int j = j0 + lane;
-        const int v = j < nb ? hl[j] : 0;
-        const int inc = wave_incl_scan(v);
-        if (j <= nb) cl[(int64_t)j * W + i] = total - (carry + inc - v);
-        carry += __shfl(inc, 63);
+    for (int j0 = 0; j0 <= nb; j0 += 256) {
+        const int j = j0 + (int)threadIdx.x;
+        const int vh = j < nb ? hh[j] : 0, vl = j < nb ? hl[j] : 0;
+        int th, tl;
+        const int eh = block_excl_scan(vh, tmp, th);
+        const int el = block_excl_scan(vl, tmp, tl);
+        if (j <= nb) {
+            ch[(int64_t)j * W + i] = carryH + eh;
+            cl[(int64_t)j * W + i] = totalL - (carryL + el);
+        }
+        carryH += th;
+        carryL += tl;
     }
 }
 
@@ -443,7 +510,7 @@ __global__ void table_cols(const SaSeg *__restrict__ segs, const int2 *__restric
 }
 
 // delta(r, c) of one subproblem, evaluated by a whole wave (uniform result);
-// r, c in [0, s].
+// r, c in [0, s].  All loads are issued before the first use.
 __device__ int wave_delta(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
                           const int32_t *__restrict__ cntHi, const int32_t *__restrict__ cntLo, const SaSeg &sg,
                           int r, int c) {
@@ -451,30 +518,27 @@ __device__ int wave_delta(const uint32_t *__restrict__ U, const uint32_t *__rest
     const int i = r / kBlk, j = c / kBlk;
     const int r0 = i * kBlk, c0 = j * kBlk;
     const int64_t x0 = sg.x0;
-    int acc = 0;
-    // rows [r0, r): hi with col < c, lo with col >= c
-#pragma unroll
-    for (int k = 0; k < kBlk / 64; ++k) {
-        const int rr = r0 + k * 64 + lane;
-        if (rr < r) {
-            const uint32_t u = U[x0 + rr];
-            const int col = (int)(u & kValMask);
-            acc += (u & kHiBit) ? (col < c) : (col >= c);
-        }
-    }
-    // cols [c0, c): hi with row < r0, lo with row >= r0
-#pragma unroll
-    for (int k = 0; k < kBlk / 64; ++k) {
-        const int cc = c0 + k * 64 + lane;
-        if (cc < c) {
-            const uint32_t v = Ui[x0 + cc];
-            const int row = (int)(v & kValMask);
-            acc += (v & kHiBit) ? (row < r0) : (row >= r0);
-        }
-    }
-    acc = wave_sum(acc);
     const int64_t tix = sg.tab_off2 + (int64_t)j * (sg.nb + 1) + i;
-    return __builtin_amdgcn_readfirstlane(cntHi[tix] - cntLo[tix] + acc);
+    const int base = cntHi[tix] - cntLo[tix];
+    uint32_t u[kBlk / 64], v[kBlk / 64];
+#pragma unroll
+    for (int k = 0; k < kBlk / 64; ++k) {
+        const int rr = r0 + k * 64 + lane, cc = c0 + k * 64 + lane;
+        u[k] = U[x0 + (rr < r ? rr : 0)];
+        v[k] = Ui[x0 + (cc < c ? cc : 0)];
+    }
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < kBlk / 64; ++k) {
+        const int rr = r0 + k * 64 + lane, cc = c0 + k * 64 + lane;
+        // rows [r0, r): hi with col < c, lo with col >= c
+        const int col = (int)(u[k] & kValMask);
+        if (rr < r) acc += (u[k] & kHiBit) ? (col < c) : (col >= c);
+        // cols [c0, c): hi with row < r0, lo with row >= r0
+        const int row = (int)(v[k] & kValMask);
+        if (cc < c) acc += (v[k] & kHiBit) ? (row < r0) : (row >= r0);
+    }
+    return __builtin_amdgcn_readfirstlane(base + wave_sum(acc));
 }
 
 // Start point of each walk segment: one wave per entry (subproblem, k); walk
@@ -512,7 +576,8 @@ __global__ __launch_bounds__(64) void seg_walk(const uint32_t *__restrict__ U, c
                                                const SaSeg *__restrict__ segs, const int2 *__restrict__ walks,
                                                const int32_t *__restrict__ c_end,
                                                const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
-    __shared__ uint32_t rowsU[kSegSteps + 1], colsU[kSegSteps + 1];
+    constexpr int kStage = (kSegSteps + 1 + 63) / 64;
+    __shared__ uint32_t rowsU[kStage * 64], colsU[kStage * 64];
     const int w = blockIdx.x;
     const int lane = threadIdx.x;
     const int4 st = seg_start[w];
@@ -522,45 +587,29 @@ __global__ __launch_bounds__(64) void seg_walk(const uint32_t *__restrict__ U, c
     const int64_t x0 = sg.x0;
     const int s = sg.s;
     const int t_end = s - 1 + c_end[job.x];
-    // kSegSteps steps move up at most kSegSteps rows and right at most
+    // kSegSteps evaluations move up at most kSegSteps rows and right at most
     // kSegSteps columns; column c-1 is read before the first move.
     const int r_lo = max(0, st.x - kSegSteps);
     const int nrow = st.x - r_lo + 1;
-    const int c_lo = max(0, st.y - 1);
+    const int c_lo = max(0, st.y - 1);   // <= s - 1
     const int ncol = max(0, min(s, st.y + kSegSteps) - c_lo);
-    for (int k = lane; k < nrow; k += 64) rowsU[k] = U[x0 + r_lo + k];
-    for (int k = lane; k < ncol; k += 64) colsU[k] = Ui[x0 + c_lo + k];
+    uint32_t tr[kStage], tc[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+        const int q = lane + 64 * k;
+        tr[k] = U[x0 + r_lo + (q < nrow ? q : 0)];
+        tc[k] = Ui[x0 + c_lo + (q < ncol ? q : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+        rowsU[lane + 64 * k] = tr[k];
+        colsU[lane + 64 * k] = tc[k];
+    }
     __syncthreads();
-    if (lane != 0) return;
-    int r = st.x, c = st.y, d = st.z;
-    const int t_stop = min(t_end, st.w + kSegSteps - 1);
-    for (int t = st.w; t <= t_stop; ++t) {
-        const uint32_t ur = rowsU[r - r_lo];
-        const int urow = (int)(ur & kValMask);
-        const bool hrow = (ur & kHiBit) != 0;
-        if (c > 0) {
-            const uint32_t vc = colsU[c - 1 - c_lo];
-            const int pr = (int)(vc & kValMask);
-            const bool hp = (vc & kHiBit) != 0;
-            const int wcol = hp ? (pr < r) : (pr >= r);
-            const int wrow = hrow ? (urow < c) : (urow >= c);
-            if (d - wcol < 0 && d + wrow > 0) R[x0 + r] = c - 1;
-        }
-        if (d < 0) {
-            if (c >= s) break;
-            const uint32_t vc = colsU[c - c_lo];
-            const int pr = (int)(vc & kValMask);
-            const bool hp = (vc & kHiBit) != 0;
-            d += hp ? (pr < r) : (pr >= r);
-            ++c;
-        } else {
-            if (r == 0) break;
-            --r;
-            const uint32_t u2 = rowsU[r - r_lo];
-            const int ur2 = (int)(u2 & kValMask);
-            const bool h2 = (u2 & kHiBit) != 0;
-            d -= h2 ? (ur2 < c) : (ur2 >= c);
-        }
+    if (lane == 0) {
+        int32_t *Rs = R + x0;
+        ant_walk(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1), [&](int rr) { return rowsU[rr - r_lo]; },
+                 [&](int cc) { return colsU[cc - c_lo]; }, [&](int rr, int cc) { Rs[rr] = cc; });
     }
 }
 

@@ -37,3 +37,13 @@ def test_similar_pair_1e6_direct_combing_equals_reference_hybrid(gpu):
     perm = gpu.semi_local_comb(a, b)
     _check(perm, rec)
     assert gpu.llcs_from_kernel(perm, len(a), len(b)) == rec["llcs"]
+
+
+def test_similar_pair_1e6_hybrid_8x8_matches_reference_hybrid(gpu):
+    # the configuration's own path: 64 blocks combed in one batched launch,
+    # merged level by level with staggered steady-ant multiplications
+    rec = G.configs()["similar_pair_1e6"]
+    a, b = W.similar_pair()
+    perm = gpu.hybrid_iterative(a, b, 8, 8)
+    _check(perm, rec)
+    assert gpu.llcs_from_kernel(perm, len(a), len(b)) == rec["llcs"]
