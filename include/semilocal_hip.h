@@ -31,6 +31,7 @@ extern "C" {
 #define SL_ERR_NOMEM 3
 #define SL_ERR_TIMEOUT 4
 #define SL_ERR_UNSUPPORTED 5
+#define SL_PENDING 6          /* sl_server_result: the request is still queued or running */
 
 /* Symbol element width accepted by the entry points that take `sym_bytes`:
  * 1 = uint8_t symbols, 2 = uint16_t, 4 = int32_t (remapped internally to a
@@ -205,6 +206,59 @@ int sl_hybrid_time(sl_hybrid *h, int warmup, int iters, double *ms_per_run, doub
 /* Cells combed per run (sum of block areas) and number of merge levels. */
 int sl_hybrid_stats(const sl_hybrid *h, int64_t *cells, int *merge_levels);
 void sl_hybrid_destroy(sl_hybrid *h);
+
+/* ------------------------------------------------------------------------ */
+/* Request loop (serving): continuous batching of comparisons on one GPU.     */
+/* No reference counterpart (the reference answers one comparison per call   */
+/* and queries on a host Permutation, approximate_matching/utils.h:268-359).  */
+/* Comparisons that arrive while others run are admitted as soon as device    */
+/* memory (one pool carved at create time, at most 90% of free HBM) and a     */
+/* launch slot are free; each admitted micro-batch is combed by one           */
+/* persistent launch on its own stream, so micro-batches overlap on the       */
+/* device.  Braids stay resident until released (or evicted, least recently  */
+/* used first, when a new comparison needs the memory); follow-up queries are */
+/* answered from them without recomputation.  Results equal the one-shot      */
+/* entry points' on the same input.                                           */
+/* ------------------------------------------------------------------------ */
+typedef struct sl_server sl_server;
+
+#define SL_QUERY_H 0                 /* raw H(i, j), i in [-m, n], j in [0, m+n]     */
+#define SL_QUERY_STRING_SUBSTRING 1  /* LCS(a, b[i:j)),  0 <= i <= j <= n            */
+#define SL_QUERY_SUBSTRING_STRING 2  /* LCS(a[i:j), b),  0 <= i <= j <= m            */
+
+typedef struct sl_server_stats_t {
+    int64_t compares_done, queries_done, cells_done;
+    int64_t launches;          /* micro-batches launched (comparisons and queries) */
+    int64_t deferrals;         /* admissions postponed for lack of device memory */
+    int64_t evictions;         /* resident braids evicted to admit new work */
+    int64_t pool_bytes, pool_in_use, pool_peak;
+    int64_t resident_braids;
+    double busy_seconds;       /* first submission to last completion */
+} sl_server_stats_t;
+
+/* pool_bytes 0: 90% of the free HBM; max_inflight 0: 4 concurrent
+ * micro-batches; max_batch 0: 1024 requests per micro-batch; max_batch_cells
+ * 0: 4e10 cells per micro-batch. */
+int sl_server_create(sl_server **out, int64_t pool_bytes, int max_inflight, int max_batch,
+                     int64_t max_batch_cells);
+/* Queues the comparison of a (m symbols, rows) with b (n symbols, columns);
+ * the input is copied.  *id names the request and its braid.  keep == 0
+ * frees the braid as soon as the score is known.  Result: the LLCS. */
+int sl_server_submit_compare(sl_server *s, const void *a, int m, const void *b, int n, int sym_bytes,
+                             int keep, int64_t *id);
+/* Queues a query (SL_QUERY_*) on the braid of comparison `braid`; it runs
+ * once that braid is resident.  Result: the LCS (or H) value. */
+int sl_server_submit_query(sl_server *s, int64_t braid, int kind, int i, int j, int64_t *id);
+/* SL_OK with *result (and the submit-to-completion latency) when the request
+ * is done, its failure status if it failed, SL_PENDING while it is queued or
+ * running; wait != 0 blocks until it is done. */
+int sl_server_result(sl_server *s, int64_t id, int wait, int64_t *result, double *latency_ms);
+/* Copies a resident braid (row -> col, m + n entries) to the host. */
+int sl_server_download_braid(sl_server *s, int64_t braid, int32_t *row_to_col);
+/* Gives a braid up (freed once no queued query reads it). */
+int sl_server_release(sl_server *s, int64_t braid);
+int sl_server_stats(sl_server *s, sl_server_stats_t *stats);
+void sl_server_destroy(sl_server *s);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,11 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SEMILOCAL_LIB") or os.path.join(PKG, "libsemilocal_hip.so")
 
 SL_OK = 0
+SL_PENDING = 6
 SL_BATCH_WITH_COL_TO_ROW = 1
+SL_QUERY_H = 0
+SL_QUERY_STRING_SUBSTRING = 1
+SL_QUERY_SUBSTRING_STRING = 2
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
@@ -74,6 +78,15 @@ SIGNATURES = {
                         C.POINTER(C.c_double)], C.c_int),
     "sl_hybrid_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)], C.c_int),
     "sl_hybrid_destroy": ([_vp], None),
+    "sl_server_create": ([C.POINTER(_vp), C.c_int64, C.c_int, C.c_int, C.c_int64], C.c_int),
+    "sl_server_submit_compare": ([_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)],
+                                 C.c_int),
+    "sl_server_submit_query": ([_vp, C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)], C.c_int),
+    "sl_server_result": ([_vp, C.c_int64, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)], C.c_int),
+    "sl_server_download_braid": ([_vp, C.c_int64, _vp], C.c_int),
+    "sl_server_release": ([_vp, C.c_int64], C.c_int),
+    "sl_server_stats": ([_vp, _vp], C.c_int),
+    "sl_server_destroy": ([_vp], None),
 }
 
 
@@ -424,3 +437,73 @@ def hybrid_iterative(a, b, small_m: int, small_n: int) -> np.ndarray:
     check(lib().sl_hybrid_iterative(_ptr(a_arr) if m else None, m, _ptr(b_arr) if n else None, n, w, small_m,
                                     small_n, _ptr(out)))
     return out[:m + n]
+
+
+# ---------------------------------------------------------------------------
+# Request loop (serving).
+# ---------------------------------------------------------------------------
+class ServerStats(C.Structure):
+    _fields_ = [("compares_done", C.c_int64), ("queries_done", C.c_int64), ("cells_done", C.c_int64),
+                ("launches", C.c_int64), ("deferrals", C.c_int64), ("evictions", C.c_int64),
+                ("pool_bytes", C.c_int64), ("pool_in_use", C.c_int64), ("pool_peak", C.c_int64),
+                ("resident_braids", C.c_int64), ("busy_seconds", C.c_double)]
+
+
+class Server:
+    """Continuous-batching request loop (sl_server_* entry points)."""
+
+    STRING_SUBSTRING = SL_QUERY_STRING_SUBSTRING
+    SUBSTRING_STRING = SL_QUERY_SUBSTRING_STRING
+    H = SL_QUERY_H
+
+    def __init__(self, pool_bytes: int = 0, max_inflight: int = 0, max_batch: int = 0, max_batch_cells: int = 0):
+        h = _vp()
+        check(lib().sl_server_create(C.byref(h), int(pool_bytes), int(max_inflight), int(max_batch),
+                                     int(max_batch_cells)))
+        self.h = h
+
+    def submit_compare(self, a, b, keep: bool = True) -> int:
+        a_arr, b_arr, w = _pair_symbols(a, b)
+        rid = C.c_int64()
+        check(lib().sl_server_submit_compare(self.h, _ptr(a_arr) if len(a_arr) else None, len(a_arr),
+                                             _ptr(b_arr) if len(b_arr) else None, len(b_arr), w, 1 if keep else 0,
+                                             C.byref(rid)))
+        return rid.value
+
+    def submit_query(self, braid: int, kind: int, i: int, j: int) -> int:
+        rid = C.c_int64()
+        check(lib().sl_server_submit_query(self.h, int(braid), int(kind), int(i), int(j), C.byref(rid)))
+        return rid.value
+
+    def result(self, rid: int, wait: bool = True):
+        """(value, latency_ms), or None while pending (wait=False)."""
+        val, lat = C.c_int64(), C.c_double()
+        st = lib().sl_server_result(self.h, int(rid), 1 if wait else 0, C.byref(val), C.byref(lat))
+        if st == SL_PENDING:
+            return None
+        check(st)
+        return val.value, lat.value
+
+    def braid(self, rid: int, size: int) -> np.ndarray:
+        out = np.empty(max(size, 1), np.int32)
+        check(lib().sl_server_download_braid(self.h, int(rid), _ptr(out)))
+        return out[:size]
+
+    def release(self, rid: int):
+        check(lib().sl_server_release(self.h, int(rid)))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().sl_server_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sl_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -229,20 +229,33 @@ struct Batch {
     LaunchCtl *d_ctl = nullptr;    // [0] batched launch, [1+i] per-pair launch of pair i
     unsigned waves_cap = 0;
     size_t items_cap = 0;
+    DeviceArena *arena = nullptr;      // null: hipMalloc / hipFree
+    bool own_stream = true;
 };
+
+static hipError_t bt_alloc(Batch *bt, void **p, size_t bytes) {
+    *p = nullptr;
+    return bt->arena ? bt->arena->alloc(p, bytes) : hipMalloc(p, bytes);
+}
+
+static void bt_free(Batch *bt, void *p) {
+    if (!p) return;
+    if (bt->arena) bt->arena->release(p);
+    else (void)hipFree(p);
+}
 
 static void batch_free(Batch *b) {
     if (!b) return;
-    (void)hipFree(b->d_pairs);
-    (void)hipFree(b->d_items);
-    (void)hipFree(b->d_pool);
-    (void)hipFree(b->d_rc);
-    (void)hipFree(b->d_cr);
-    (void)hipFree(b->d_buf);
-    (void)hipFree(b->d_ctl);
+    bt_free(b, b->d_pairs);
+    bt_free(b, b->d_items);
+    bt_free(b, b->d_pool);
+    bt_free(b, b->d_rc);
+    bt_free(b, b->d_cr);
+    bt_free(b, b->d_buf);
+    bt_free(b, b->d_ctl);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream && b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -269,14 +282,14 @@ static void plan_items(Batch *bt) {
 // (Re)uploads the pair table and the queue; grows the device copies as needed.
 static hipError_t upload_items(Batch *bt) {
     hipError_t e = hipSuccess;
-    if (!bt->d_pairs) e = hipMalloc((void **)&bt->d_pairs, sizeof(PairDesc) * std::max<size_t>(bt->pairs.size(), 1));
+    if (!bt->d_pairs) e = bt_alloc(bt, (void **)&bt->d_pairs, sizeof(PairDesc) * std::max<size_t>(bt->pairs.size(), 1));
     if (e == hipSuccess && !bt->pairs.empty())
         e = hipMemcpy(bt->d_pairs, bt->pairs.data(), sizeof(PairDesc) * bt->pairs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && (!bt->d_items || bt->items.size() > bt->items_cap)) {
-        (void)hipFree(bt->d_items);
+        bt_free(bt, bt->d_items);
         bt->d_items = nullptr;
         bt->items_cap = std::max<size_t>(bt->items.size(), 1);
-        e = hipMalloc((void **)&bt->d_items, sizeof(WorkItem) * bt->items_cap);
+        e = bt_alloc(bt, (void **)&bt->d_items, sizeof(WorkItem) * bt->items_cap);
     }
     if (e == hipSuccess && !bt->items.empty())
         e = hipMemcpy(bt->d_items, bt->items.data(), sizeof(WorkItem) * bt->items.size(), hipMemcpyHostToDevice);
@@ -284,8 +297,8 @@ static hipError_t upload_items(Batch *bt) {
 }
 
 template <typename T>
-static hipError_t upload(T **dst, const T *src, size_t count) {
-    hipError_t e = hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(count, 1));
+static hipError_t upload(Batch *bt, T **dst, const T *src, size_t count) {
+    hipError_t e = bt_alloc(bt, (void **)dst, sizeof(T) * std::max<size_t>(count, 1));
     if (e != hipSuccess) return e;
     if (count) e = hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice);
     return e;
@@ -293,7 +306,7 @@ static hipError_t upload(T **dst, const T *src, size_t count) {
 
 static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
                         const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes, int flags,
-                        int rows_per_lane) {
+                        int rows_per_lane, DeviceArena *arena = nullptr, hipStream_t ext_stream = nullptr) {
     if (!out || npairs < 0 || (npairs > 0 && (!a_cat || !a_off || !a_len || !b_cat || !b_off || !b_len))) {
         set_error("sl_batch_create: null argument");
         return kErrArg;
@@ -322,6 +335,7 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     Batch *bt = new Batch();
     bt->npairs = npairs;
     bt->with_cr = (flags & SL_BATCH_WITH_COL_TO_ROW) != 0;
+    bt->arena = arena;
 
     std::vector<int64_t> a_pos, b_pos;
     PackedPool pool;
@@ -372,16 +386,24 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     }
     plan_items(bt);
 
-    hipError_t e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking);
+    hipError_t e = hipSuccess;
+    if (ext_stream) {
+        bt->stream = ext_stream;
+        bt->own_stream = false;
+    } else {
+        e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking);
+    }
     for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&bt->ev[k]);
     if (e == hipSuccess) e = upload_items(bt);
-    if (e == hipSuccess) e = upload((uint8_t **)&bt->d_pool, pool.bytes.data(), pool.bytes.size());
-    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_rc, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
+    if (e == hipSuccess) e = upload(bt, (uint8_t **)&bt->d_pool, pool.bytes.data(), pool.bytes.size());
+    if (e == hipSuccess)
+        e = bt_alloc(bt, (void **)&bt->d_rc, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
     if (e == hipSuccess && bt->with_cr)
-        e = hipMalloc((void **)&bt->d_cr, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
-    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_buf, sizeof(uint64_t) * std::max<int64_t>(bt->buf_total, 1));
+        e = bt_alloc(bt, (void **)&bt->d_cr, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
+    if (e == hipSuccess)
+        e = bt_alloc(bt, (void **)&bt->d_buf, sizeof(uint64_t) * std::max<int64_t>(bt->buf_total, 1));
     if (e == hipSuccess) e = hipMemset(bt->d_buf, 0, sizeof(uint64_t) * std::max<int64_t>(bt->buf_total, 1));
-    if (e == hipSuccess) e = hipMalloc((void **)&bt->d_ctl, sizeof(LaunchCtl) * (size_t)(npairs + 1));
+    if (e == hipSuccess) e = bt_alloc(bt, (void **)&bt->d_ctl, sizeof(LaunchCtl) * (size_t)(npairs + 1));
     if (e != hipSuccess) {
         set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
         batch_free(bt);
@@ -475,6 +497,11 @@ static int batch_sync(Batch *bt) {
 int batch_create_internal(Batch **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
                           const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes) {
     return batch_create(out, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes, 0, 0);
+}
+int batch_create_arena(Batch **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                       const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes,
+                       DeviceArena *arena, hipStream_t stream) {
+    return batch_create(out, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes, 0, 0, arena, stream);
 }
 int batch_launch_internal(Batch *bt) { return batch_run(bt); }
 int batch_wait_internal(Batch *bt) { return batch_sync(bt); }

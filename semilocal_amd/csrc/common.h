@@ -55,6 +55,14 @@ struct WorkItem {
     int32_t pass;
 };
 
+// Source of device memory for a combing batch: hipMalloc by default, or a
+// caller's arena (the request loop carves every batch out of one allocation).
+struct DeviceArena {
+    virtual ~DeviceArena() = default;
+    virtual hipError_t alloc(void **p, size_t bytes) = 0;
+    virtual void release(void *p) = 0;
+};
+
 // Device-side control block of one launch.
 struct LaunchCtl {
     uint32_t ticket;    // next queue entry to hand out

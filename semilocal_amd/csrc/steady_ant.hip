@@ -210,14 +210,17 @@ __global__ __launch_bounds__(kTile) void split_scatter(const int32_t *__restrict
 // colv(c) column c's point (row | hi bit 31); good points go to put(row, col).
 // Both candidates of the next move are read together, so a step costs one
 // memory round trip; the point of the current row and of column c-1 are
-// carried in registers.
+// carried in registers.  rowv must accept r in [r_min, r] and colv c in
+// [c_start - 1, min(c_start + steps, s) - 1] (c_start >= 1 or s >= 1).
 template <class RowF, class ColF, class PutF>
 __device__ __forceinline__ void ant_walk(int r, int c, int d, int s, int steps, RowF rowv, ColF colv, PutF put) {
     uint32_t ur = rowv(r);
     uint32_t left = c > 0 ? colv(c - 1) : 0u;
     for (int t = 0; t < steps; ++t) {
-        const uint32_t up = r > 0 ? rowv(r - 1) : 0u;
-        const uint32_t nxt = c < s ? colv(c) : 0u;
+        // unconditional (clamped) reads, so both are in flight together; the
+        // clamped values are never used
+        const uint32_t up = rowv(r > 0 ? r - 1 : r);
+        const uint32_t nxt = colv(c < s ? c : c - 1);
         if (c > 0) {
             const int urow = (int)(ur & kValMask), pr = (int)(left & kValMask);
             const int wcol = (left & kHiBit) ? (pr < r) : (pr >= r);
