@@ -75,6 +75,136 @@ def cpu_baseline(pairs, target_s: float = 12.0):
             "seconds": dt}
 
 
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def bench_serving(S, W, count: int, rate_hz: float, pool_gb: float):
+    """Config 5: replay a Poisson request stream in real time against the
+    request loop; latency = submission (at the scheduled arrival) to
+    completion.  30% of requests are follow-up queries on earlier braids."""
+    reqs = W.serving_stream(count=count, rate_hz=rate_hz)
+    srv = S.Server(pool_bytes=int(pool_gb * (1 << 30)))
+    ids = {}
+    t0 = time.perf_counter()
+    lag = 0.0
+    for r in reqs:
+        delay = r.arrival_s - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        else:
+            lag = max(lag, -delay)
+        if r.kind == "compare":
+            ids[r.rid] = srv.submit_compare(r.a, r.b)
+        else:
+            ids[r.rid] = srv.submit_query(ids[r.target], srv.STRING_SUBSTRING, *r.window)
+    lat = {"compare": [], "query": []}
+    vals = {}
+    for r in reqs:
+        v, ms = srv.result(ids[r.rid])
+        lat[r.kind].append(ms)
+        vals[r.rid] = v
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    # spot parity: a sample of comparisons and queries against the one-shot path
+    g = np.random.default_rng(0)
+    cmp_ids = [r for r in reqs if r.kind == "compare"]
+    checked = 0
+    for r in g.choice(cmp_ids, size=min(8, len(cmp_ids)), replace=False):
+        perm = S.semi_local_comb(r.a, r.b)
+        assert vals[r.rid] == S.llcs_from_kernel(perm, len(r.a), len(r.b)), "serving parity failure"
+        checked += 1
+    qs = [r for r in reqs if r.kind == "query"][:8]
+    for r in qs:
+        t = reqs[r.target]
+        perm = S.semi_local_comb(t.a, t.b)
+        lo, hi = r.window
+        h = hi - lo - int(np.count_nonzero(perm[lo + len(t.a):] < hi))
+        assert vals[r.rid] == h, "serving query parity failure"
+        checked += 1
+    srv.close()
+    allv = lat["compare"] + lat["query"]
+    return {
+        "workload": f"{count} requests, Poisson {rate_hz:g}/s, lengths log-normal (median 5000, sigma_ln 1.0) "
+                    f"clipped to [200, 200000], alphabet 4, 30% follow-up string-substring queries, seed 5",
+        "rate_hz": rate_hz, "requests": count,
+        "p50_ms": round(percentile(allv, 50), 3), "p99_ms": round(percentile(allv, 99), 3),
+        "compare_p50_ms": round(percentile(lat["compare"], 50), 3),
+        "compare_p99_ms": round(percentile(lat["compare"], 99), 3),
+        "query_p50_ms": round(percentile(lat["query"], 50), 3) if lat["query"] else None,
+        "query_p99_ms": round(percentile(lat["query"], 99), 3) if lat["query"] else None,
+        "gcups": round(st["cells_done"] / st["busy_seconds"] / 1e9, 2) if st["busy_seconds"] else None,
+        "offered_gcups": round(st["cells_done"] / reqs[-1].arrival_s / 1e9, 2),
+        "wall_s": round(wall, 3), "max_submit_lag_ms": round(lag * 1e3, 3),
+        "launches": st["launches"], "deferrals": st["deferrals"], "evictions": st["evictions"],
+        "pool_gb": round(st["pool_bytes"] / 2**30, 2), "pool_peak_mb": round(st["pool_peak"] / 2**20, 1),
+        "parity_spot_checks": checked,
+    }
+
+
+def bench_hybrid(S, W, golden):
+    """Config 3: 8x8 blocks combed in one launch, merged by 6 levels of batched
+    steady-ant multiplications."""
+    a, b = W.similar_pair()
+    h = S.Hybrid(a, b, 8, 8)
+    perm = h.run()
+    ok = W.sha256(perm) == golden["similar_pair_1e6"]["perm_sha256"]
+    tot, comb, merge = h.time(warmup=1, iters=3)
+    cells, levels = h.stats()
+    h.close()
+    return {"workload": "similar pair: a 1e6 symbols (alphabet 4), b = a with 5% subs and 1% indels (seed 3); "
+                        "8x8 blocks", "ms": round(tot, 3), "comb_ms": round(comb, 3), "merge_ms": round(merge, 3),
+            "merge_levels": levels, "gcups": round(cells / (tot * 1e-3) / 1e9, 2),
+            "parity": "bit-exact vs the reference hybrid's fixture" if ok else "MISMATCH"}
+
+
+def bench_multiply(S):
+    """Kernel (c) alone: one product of two random permutations of 2^20."""
+    n = 1 << 20
+    g = np.random.default_rng(0)
+    p, q = g.permutation(n).astype(np.int32), g.permutation(n).astype(np.int32)
+    ms, el = S.braid_multiply_time([(p, q)], warmup=1, iters=5)
+    # algorithmic bytes: read two int32 maps and write one per element per level
+    gbs = 12.0 * el / (ms * 1e-3) / 1e9
+    return {"workload": "R = P (.) Q, n = 2^20 random permutations", "ms": round(ms, 3),
+            "levels_x_elements": el, "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0,
+                                                  "unit": "GB/s", "frac": round(gbs / 8000.0, 5),
+                                                  "bytes_per_element_level": 12}}
+
+
+def bench_binary(S, W, golden):
+    """Config 2: bit-parallel LLCS of two 2^20-bit strings, and their full
+    permutation by kernel (a)."""
+    a, b = W.binary_pair()
+    bp = S.BitParallel(a, b)
+    llcs = bp.llcs()
+    ms = bp.time(warmup=1, iters=3)
+    bp.close()
+    rec = golden["binary_pair_2p20"]
+    t = time.perf_counter()
+    perm = S.semi_local_comb(a, b)
+    pms = (time.perf_counter() - t) * 1e3
+    cells = len(a) * len(b)
+    return {"workload": "binary pair, |a| = |b| = 2^20 (seed 2)",
+            "bitparallel_ms": round(ms, 3), "bitparallel_gcups": round(cells / (ms * 1e-3) / 1e9, 1),
+            "permutation_ms_incl_transfers": round(pms, 1),
+            "permutation_gcups": round(cells / (pms * 1e-3) / 1e9, 1),
+            "parity": "bit-exact" if (llcs == rec["llcs_bitparallel"] and W.sha256(perm) == rec["perm_sha256"])
+            else "MISMATCH"}
+
+
+def bench_plain(S, W, golden):
+    """Config 1: one 10k x 10k pair, one launch."""
+    a, b = W.plain_pair()
+    bt = S.Batch([(a, b)])
+    res = bt.run().results()[0]
+    ms, kms = bt.time(warmup=2, iters=10)
+    bt.close()
+    return {"workload": "plain pair 10000 x 10000, alphabet 4 (seed 1)", "ms": round(ms, 4),
+            "gcups": round(len(a) * len(b) / (ms * 1e-3) / 1e9, 1),
+            "parity": "bit-exact" if W.sha256(res) == golden["plain_pair_10k"]["perm_sha256"] else "MISMATCH"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +216,10 @@ def main():
     ap.add_argument("--rows-per-lane", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
+    ap.add_argument("--no-extras", action="store_true", help="only the ragged batch (configs 1, 2, 3, 5 skipped)")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=1000.0)
+    ap.add_argument("--serve-pool-gb", type=float, default=16.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -175,6 +309,17 @@ def main():
                      "ops_per_cell": VALU_OPS_PER_CELL},
     }
     batch.close()
+    if not args.no_extras:
+        golden = json.load(open(os.path.join(ROOT, "tests", "golden", "configs.json")))
+        for key, fn in (("plain_pair_10k", lambda: bench_plain(semilocal_amd, W, golden)),
+                        ("binary_pair_2p20", lambda: bench_binary(semilocal_amd, W, golden)),
+                        ("similar_pair_1e6_hybrid", lambda: bench_hybrid(semilocal_amd, W, golden)),
+                        ("braid_multiply_2p20", lambda: bench_multiply(semilocal_amd)),
+                        ("serving", lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
+                                                          args.serve_pool_gb))):
+            t0 = time.perf_counter()
+            out[key] = fn()
+            log(f"{key}: {out[key]}  ({time.perf_counter() - t0:.1f}s)")
     if not args.no_cpu_baseline:
         cb = cpu_baseline(pairs)
         log("cpu baseline:", cb)
