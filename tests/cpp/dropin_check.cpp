@@ -1,0 +1,236 @@
+// dropin_check.cpp -- the drop-in claim, checked at the reference's own call
+// sites.  TEST INFRASTRUCTURE ONLY: compiled by oracle/Makefile against the
+// reference's headers where they lie (nothing of the reference is copied)
+// and against include/semilocal_hip*.hpp; linked with libsemilocal_hip.so.
+//
+// Every check calls a reference entry point and the same call site switched
+// to the engine, on the same seeded input, and compares the results:
+//   flat API:   semi_local::sticky_braid_mpi / sticky_braid_sequential /
+//               hybrid_iterative, steady_ant::steady_ant_parallel_wrapper,
+//               steady_ant::staggered_sticky_multiplication<RowGlue, false>,
+//               binary::llcs_2symbol_smart_combing<uint64_t / uint32_t>
+//   namespaced: semi_local::OpenMPIterativeCombing, OpenMPHybridIterative,
+//               common::SequentialMemoryOptimizedStickBraidMultiplication,
+//               lcs::LCSBySemiLocal over either combing strategy.
+// Prints one line per check and "dropin_check: N checks, M failures"; exits
+// non-zero on any failure.  Needs a GPU to run.
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "semi_local.h"
+#include "prefix_lcs/bitwise/encoders_and_decoders.h"
+#include "prefix_lcs/bitwise/transposition_network_binary_alphabet.h"
+#include "include/LCS/LCS.h"
+#include "include/SemiLocal/SemiLocalLCS.h"
+
+#include "semilocal_hip_strategies.hpp"
+
+static int checks = 0, failures = 0;
+
+static void report(const char *what, bool ok) {
+    ++checks;
+    if (!ok) ++failures;
+    std::printf("%-64s %s\n", what, ok ? "ok" : "MISMATCH");
+}
+
+static bool same(const AbstractPermutation &x, const AbstractPermutation &y) {
+    if (x.row_size != y.row_size) return false;
+    for (int i = 0; i < x.row_size; ++i)
+        if (x.get_col_by_row(i) != y.get_col_by_row(i)) return false;
+    return true;
+}
+
+static bool same(const common::Permutation &x, const common::Permutation &y) {
+    if (x.rows != y.rows) return false;
+    for (int i = 0; i < x.rows; ++i)
+        if (x.getColByRow(i) != y.getColByRow(i)) return false;
+    return true;
+}
+
+static std::vector<int> symbols(std::mt19937_64 &g, int n, int sigma) {
+    std::vector<int> v((size_t)n);
+    for (auto &x : v) x = (int)(g() % (uint64_t)sigma);
+    return v;
+}
+
+static std::vector<int> shuffled(std::mt19937_64 &g, int n) {
+    std::vector<int> v((size_t)n);
+    for (int i = 0; i < n; ++i) v[(size_t)i] = i;
+    std::shuffle(v.begin(), v.end(), g);
+    return v;
+}
+
+int main() {
+    std::mt19937_64 g(20261203);
+    PrecalcMap map;
+    precalc(map, 5);
+    common::PrecalcMap cmap;   // the namespaced API's own base-case table
+    {
+        common::NaiveBraidMultiplication naive;
+        common::StickyBraidMultiplication::buildPrecalcMap(&naive, cmap, 5);
+    }
+    char name[160];
+
+    // ---- flat API: combing ------------------------------------------------
+    for (auto mn : {std::pair<int, int>{300, 500}, {777, 123}, {1, 900}, {2000, 2000}}) {
+        const int m = mn.first, n = mn.second;
+        auto a = symbols(g, m, 4), b = symbols(g, n, 4);
+        Permutation ref(m + n, m + n), hip(m + n, m + n);
+        semi_local::sticky_braid_mpi<int, false, true>(ref, a.data(), m, b.data(), n, 4);
+        semilocal_hip::sticky_braid_mpi<int, false, true>(hip, a.data(), m, b.data(), n, 4);
+        std::snprintf(name, sizeof name, "sticky_braid_mpi<int,false,true> %dx%d", m, n);
+        report(name, same(ref, hip));
+        Permutation ref_r(m + n, m + n), hip_r(m + n, m + n);
+        semi_local::sticky_braid_mpi<int, false, false>(ref_r, a.data(), m, b.data(), n, 1, true);
+        semilocal_hip::sticky_braid_mpi<int, false, false>(hip_r, a.data(), m, b.data(), n, 1, true);
+        std::snprintf(name, sizeof name, "sticky_braid_mpi is_reverse=true %dx%d", m, n);
+        report(name, same(ref_r, hip_r));
+        Permutation ref_s(m + n, m + n), hip_s(m + n, m + n);
+        semi_local::sticky_braid_sequential<int, false>(ref_s, a.data(), m, b.data(), n);
+        semilocal_hip::sticky_braid_sequential<int, false>(hip_s, a.data(), m, b.data(), n);
+        std::snprintf(name, sizeof name, "sticky_braid_sequential<int,false> %dx%d", m, n);
+        report(name, same(ref_s, hip_s));
+    }
+    {   // unsigned short symbols, as some reference call sites use
+        const int m = 900, n = 1100;
+        std::vector<unsigned short> a((size_t)m), b((size_t)n);
+        for (auto &x : a) x = (unsigned short)(g() % 300);
+        for (auto &x : b) x = (unsigned short)(g() % 300);
+        Permutation ref(m + n, m + n), hip(m + n, m + n);
+        semi_local::sticky_braid_mpi<unsigned short, false, true>(ref, a.data(), m, b.data(), n, 2);
+        semilocal_hip::sticky_braid_mpi<unsigned short, false, true>(hip, a.data(), m, b.data(), n, 2);
+        report("sticky_braid_mpi<unsigned short,...> 900x1100 sigma 300", same(ref, hip));
+    }
+
+    // ---- flat API: hybrid --------------------------------------------------
+    for (auto shape : {std::pair<int, int>{2, 2}, {3, 5}, {8, 8}, {1, 16}}) {
+        const int m = 3000, n = 2500;
+        auto a = symbols(g, m, 4), b = symbols(g, n, 4);
+        Permutation ref(m + n, m + n), hip(m + n, m + n);
+        semi_local::hybrid_iterative<int, false>(ref, a.data(), m, b.data(), n, map, shape.first, shape.second, 4);
+        semilocal_hip::hybrid_iterative<int, false>(hip, a.data(), m, b.data(), n, map, shape.first, shape.second, 4);
+        std::snprintf(name, sizeof name, "hybrid_iterative<int,false> 3000x2500 %dx%d blocks", shape.first,
+                      shape.second);
+        report(name, same(ref, hip));
+    }
+
+    // ---- flat API: steady ant and staggered --------------------------------
+    for (int n : {1, 7, 64, 1000, 5000, 100000}) {
+        auto pv = shuffled(g, n), qv = shuffled(g, n);
+        Permutation p(n, n), q(n, n), ref(n, n), hip(n, n);
+        for (int i = 0; i < n; ++i) {
+            p.set_point(i, pv[(size_t)i]);
+            q.set_point(i, qv[(size_t)i]);
+        }
+        Permutation p2(n, n), q2(n, n);   // the reference's routine may reuse its inputs
+        for (int i = 0; i < n; ++i) {
+            p2.set_point(i, pv[(size_t)i]);
+            q2.set_point(i, qv[(size_t)i]);
+        }
+        distance_unit_monge_product::steady_ant::steady_ant_parallel_wrapper(p, q, ref, map, 2);
+        semilocal_hip::steady_ant_parallel_wrapper(p2, q2, hip, map, 2);
+        std::snprintf(name, sizeof name, "steady_ant_parallel_wrapper n=%d", n);
+        report(name, same(ref, hip));
+    }
+    for (int rg = 0; rg < 2; ++rg) {
+        // block braids of two neighbouring blocks of one grid, as the hybrid glues them
+        const int m1 = 400, n1 = 700, m2 = 400, n2 = 350;
+        auto a = symbols(g, rg ? m1 + m2 : m1, 4), b = symbols(g, rg ? n1 : n1 + n2, 4);
+        const int pm = m1, pn = n1, qm = rg ? m2 : m1, qn = rg ? n1 : n2;
+        Permutation P(pm + pn, pm + pn), Q(qm + qn, qm + qn);
+        semi_local::sticky_braid_mpi<int, false, false>(P, a.data(), pm, b.data(), pn, 1);
+        semi_local::sticky_braid_mpi<int, false, false>(Q, rg ? a.data() + m1 : a.data(), qm,
+                                                        rg ? b.data() : b.data() + n1, qn, 1);
+        P.m = pm; P.n = pn; Q.m = qm; Q.n = qn;
+        const int k = rg ? pn : pm;
+        const int N = P.row_size + Q.row_size - k;
+        Permutation ref(N, N), hip(N, N);
+        if (rg) {
+            distance_unit_monge_product::steady_ant::staggered_sticky_multiplication<true, false>(&P, &Q, k, map, &ref);
+            semilocal_hip::staggered_sticky_multiplication<true, false>(&P, &Q, k, map, &hip);
+        } else {
+            distance_unit_monge_product::steady_ant::staggered_sticky_multiplication<false, false>(&P, &Q, k, map, &ref);
+            semilocal_hip::staggered_sticky_multiplication<false, false>(&P, &Q, k, map, &hip);
+        }
+        std::snprintf(name, sizeof name, "staggered_sticky_multiplication<%s,false>", rg ? "true" : "false");
+        report(name, same(ref, hip));
+    }
+
+    // ---- flat API: bit-parallel --------------------------------------------
+    {
+        const int bits = 1 << 14;
+        std::vector<int> a((size_t)bits), b((size_t)bits);
+        for (auto &x : a) x = (int)(g() & 1u);
+        for (auto &x : b) x = (int)(g() & 1u);
+        std::unordered_map<int, uint64_t> fwd{{0, 0}, {1, 1}};
+        std::unordered_map<uint64_t, int> rev{{0, 0}, {1, 1}};
+        auto ar = encode_reverse<int, uint64_t>(a, &fwd, &rev, 2);
+        auto bw = encode<int, uint64_t>(b, &fwd, &rev, 2);
+        const int want = prefix_lcs_via_semi_local::binary::llcs_2symbol_smart_combing<uint64_t>(
+            ar.first.first, ar.first.second, bw.first.first, bw.first.second, bits, 1);
+        const int got = semilocal_hip::llcs_2symbol_smart_combing<uint64_t>(ar.first.first, ar.first.second,
+                                                                             bw.first.first, bw.first.second, bits, 1);
+        report("llcs_2symbol_smart_combing<uint64_t> 2^14 bits", want == got);
+        std::unordered_map<int, uint32_t> fwd32{{0, 0}, {1, 1}};
+        std::unordered_map<uint32_t, int> rev32{{0, 0}, {1, 1}};
+        auto ar32 = encode_reverse<int, uint32_t>(a, &fwd32, &rev32, 2);
+        auto bw32 = encode<int, uint32_t>(b, &fwd32, &rev32, 2);
+        const int got32 = semilocal_hip::llcs_2symbol_smart_combing<uint32_t>(
+            ar32.first.first, ar32.first.second, bw32.first.first, bw32.first.second, bits, 1);
+        report("llcs_2symbol_smart_combing<uint32_t> 2^14 bits (repacked)", want == got32);
+        std::free(ar.first.first);
+        std::free(bw.first.first);
+        std::free(ar32.first.first);
+        std::free(bw32.first.first);
+    }
+
+    // ---- namespaced strategies ---------------------------------------------
+    {
+        const int m = 1500, n = 2600;
+        auto a = symbols(g, m, 4), b = symbols(g, n, 4);
+        common::Permutation ref, hip;
+        semi_local::OpenMPIterativeCombing<int, false> cpu(4);
+        semilocal_hip::HipIterativeCombing<int, false> dev(4);
+        cpu.compute(a.data(), m, b.data(), n, ref);
+        dev.compute(a.data(), m, b.data(), n, hip);
+        report("OpenMPIterativeCombing vs HipIterativeCombing 1500x2600", same(ref, hip));
+
+        common::SequentialMemoryOptimizedStickBraidMultiplication seqmul(cmap);
+        common::StaggeredStickyBraidMultiplication stag(&seqmul);
+        semi_local::OpenMPHybridIterative<int, false> cpu_h(4, 700, 500, &stag, &cpu);
+        semilocal_hip::HipHybridIterative<int, false> dev_h(4, 700, 500, nullptr, &dev);
+        common::Permutation ref_h, hip_h;
+        cpu_h.compute(a.data(), m, b.data(), n, ref_h);
+        dev_h.compute(a.data(), m, b.data(), n, hip_h);
+        report("OpenMPHybridIterative vs HipHybridIterative 1500x2600", same(ref_h, hip_h) && same(ref, hip_h));
+
+        lcs::LCSBySemiLocal<int> lcs_cpu(&cpu), lcs_dev(&dev);
+        semilocal_hip::HipLLCS<int> llcs_dev;
+        const int want = lcs_cpu.compute(a.data(), m, b.data(), n);
+        report("LCSBySemiLocal over the device combing strategy", lcs_dev.compute(a.data(), m, b.data(), n) == want);
+        report("HipLLCS", llcs_dev.compute(a.data(), m, b.data(), n) == want);
+    }
+    for (int n : {5, 300, 4096, 30000}) {
+        auto pv = shuffled(g, n), qv = shuffled(g, n);
+        common::Permutation p(n, n), q(n, n), ref, hip;
+        for (int i = 0; i < n; ++i) {
+            p.set(i, pv[(size_t)i]);
+            q.set(i, qv[(size_t)i]);
+        }
+        common::SequentialMemoryOptimizedStickBraidMultiplication cpu(cmap);
+        semilocal_hip::HipStickyBraid dev;
+        cpu.multiply(p, q, ref);
+        dev.multiply(p, q, hip);
+        std::snprintf(name, sizeof name, "SequentialMemoryOptimizedStickBraidMultiplication vs HipStickyBraid n=%d", n);
+        report(name, same(ref, hip));
+    }
+
+    std::printf("dropin_check: %d checks, %d failures\n", checks, failures);
+    return failures ? 1 : 0;
+}
