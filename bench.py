@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
     ap.add_argument("--no-extras", action="store_true", help="only the ragged batch (configs 1, 2, 3, 5 skipped)")
+    ap.add_argument("--only", choices=["plain", "binary", "hybrid", "multiply", "serving"],
+                    help="run one extra section alone (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--serve-pool-gb", type=float, default=16.0)
@@ -232,6 +234,16 @@ def main():
 
     dev = semilocal_amd.device_info()
     log("device:", dev)
+    if args.only:
+        golden = json.load(open(os.path.join(ROOT, "tests", "golden", "configs.json")))
+        fn = {"plain": lambda: bench_plain(semilocal_amd, W, golden),
+              "binary": lambda: bench_binary(semilocal_amd, W, golden),
+              "hybrid": lambda: bench_hybrid(semilocal_amd, W, golden),
+              "multiply": lambda: bench_multiply(semilocal_amd),
+              "serving": lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
+                                               args.serve_pool_gb)}[args.only]
+        print(json.dumps({"section": args.only, **fn()}), flush=True)
+        return
     t0 = time.perf_counter()
     pairs = W.ragged_batch()
     log(f"workload generated in {time.perf_counter() - t0:.1f}s")
@@ -267,19 +279,24 @@ def main():
         pms = float("nan")
     if args.copy_probe_mb:
         # a plain device copy of known size: the scale for the PMC byte counters
-        import torch
+        from semilocal_amd._lib import probe_copy
 
         nbytes = args.copy_probe_mb << 20
-        src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
-        torch.cuda.synchronize()
-        dst = torch.add(src, 1)   # elementwise kernel: reads nbytes, writes nbytes
-        torch.cuda.synchronize()
-        del dst
-        log(f"copy probe: {nbytes} bytes read, {nbytes} bytes written")
+        pms_copy = probe_copy(nbytes)
+        log(f"copy probe: {nbytes} bytes read, {nbytes} bytes written in {pms_copy:.3f} ms "
+            f"({2 * nbytes / pms_copy / 1e6:.0f} GB/s)")
 
     gcups = cells / (ms * 1e-3) / 1e9
     gcups_pp = cells / (pms * 1e-3) / 1e9
     peak_tops = dev["compute_units"] * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12
+    # HBM bytes per launch of this kernel, measured by two rocprofv3 --pmc
+    # passes and scaled by a copy kernel of known size (profiles/)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "comb_batch_kernel_ragged4096.traffic.json")
+    if os.path.exists(tf):
+        rec = json.load(open(tf))
+        if rec.get("kernel") == f"comb_batch_kernel<uint8_t,{tiles[0]}>":
+            traffic = rec["traffic_bytes_per_launch"]
     achieved_tops = cells * VALU_OPS_PER_CELL / (kms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -304,7 +321,7 @@ def main():
         "parity": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 2),
                      "unit": "Tops/s (int32 VALU lane-ops)", "frac": round(achieved_tops / peak_tops, 4),
-                     "traffic": None,
+                     "traffic": traffic,
                      "kernel": f"comb_batch_kernel<uint8_t,{tiles[0]}>", "kernel_ms": round(kms, 4),
                      "ops_per_cell": VALU_OPS_PER_CELL},
     }

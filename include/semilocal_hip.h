@@ -51,6 +51,12 @@ int sl_init(int device);
 /* Writes "name;compute_units;clock_khz;hbm_bytes" into buf. */
 int sl_device_info(char *buf, int len);
 
+/* Diagnostics (no reference counterpart): one plain device-to-device copy of
+ * `bytes` (a multiple of 16) by a streaming kernel, timed with HIP events.
+ * Used to scale the HBM byte counters of the profiler (bytes read = bytes
+ * written = `bytes`). */
+int sl_probe_copy(int64_t bytes, double *ms);
+
 /* ------------------------------------------------------------------------ */
 /* Plain semi-local LCS of one pair (anti-diagonal sticky-braid combing).     */
 /* Replaces: semi_local::sticky_braid_mpi<Input,WithIf,WithWait>(matrix, a,   */

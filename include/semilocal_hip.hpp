@@ -165,6 +165,33 @@ void sticky_braid_sequential(Perm &permutation, const Input *a, int a_size, cons
     detail::store(permutation, detail::comb(a, a_size, b, b_size));
 }
 
+// semi_local::first_and_third_phase_combined<Input, WithIf, WithWait> --
+// semi_local.h:292-295.  The phase split is a CPU barrier optimisation; the
+// kernel it produces is the same unique permutation.
+template <class Input, bool WithIf, bool WithWait, class Perm, class Map>
+void first_and_third_phase_combined(Perm &matrix, const Input *a, int a_size, const Input *b, int b_size, Map &map,
+                                    int nested_parall_regions = 0, int threads_num = 1) {
+    (void)map;
+    (void)nested_parall_regions;
+    (void)threads_num;
+    detail::store(matrix, detail::comb(a, a_size, b, b_size));
+}
+
+// semi_local::hybrid<Input, WithIf, WithWait, UseSumBound> (recursive) --
+// semi_local.h:452-456.  Its depth / sum bounds only choose where the CPU
+// recursion stops; the kernel is unique, so the device combs the pair whole.
+template <class Input, bool WithIf, bool WithWait, bool UseSumBound, class Perm, class Map>
+void hybrid(Perm &perm, const Input *a, int m, const Input *b, int n, Map &map, int thds_per_combing_algo,
+            int braid_mul_parall_depth, int depth, int sum_bound, int parallel_depth) {
+    (void)map;
+    (void)thds_per_combing_algo;
+    (void)braid_mul_parall_depth;
+    (void)depth;
+    (void)sum_bound;
+    (void)parallel_depth;
+    detail::store(perm, detail::comb(a, m, b, n));
+}
+
 // semi_local::hybrid_iterative<Input, WithIf> -- semi_local.h:577-580.
 template <class Input, bool WithIf, class Perm, class Map>
 void hybrid_iterative(Perm &perm, const Input *a, int m, const Input *b, int n, Map &map, int small_m, int small_n,

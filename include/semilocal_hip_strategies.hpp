@@ -76,6 +76,42 @@ class HipIterativeCombing : public semi_local::AntidiagonalIterativeCombing<Inpu
     int threads;   // accepted for signature parity; the device decides its own parallelism
 };
 
+// Replaces semi_local::OpenMPIterativeCombingCombined<Input, WithIf>
+// (SemiLocalLCS.h:308-476): same constructor; same unique kernel.
+template <class Input, bool WithIf = false>
+class HipIterativeCombingCombined : public HipIterativeCombing<Input, WithIf> {
+  public:
+    HipIterativeCombingCombined(int numThreads, common::StaggeredStickyBraidMultiplication *staggered = nullptr)
+        : HipIterativeCombing<Input, WithIf>(numThreads) {
+        (void)staggered;
+    }
+};
+
+// Replaces semi_local::OpenMPHybridRecursive<Input, WithIf, UseSumBound,
+// UseDepth> (SemiLocalLCS.h:478-585): same template parameters and
+// constructor.  Its thresholds only decide where the CPU recursion stops; the
+// device combs the pair whole (same unique kernel).
+template <class Input, bool WithIf = false, bool UseSumBound = true, bool UseDepth = true>
+class HipHybridRecursive : public semi_local::SemiLocalStrategy<Input> {
+  public:
+    HipHybridRecursive(int parallelThreshold, common::StaggeredStickyBraidMultiplication *staggered,
+                       int depthThreshold, int sumThreshold,
+                       semi_local::AntidiagonalIterativeCombing<Input, WithIf> *combing = nullptr) {
+        (void)parallelThreshold;
+        (void)staggered;
+        (void)depthThreshold;
+        (void)sumThreshold;
+        (void)combing;
+    }
+
+    void compute(const Input *a, int aSize, const Input *b, int bSize, common::Permutation &kernel) noexcept override {
+        inner.compute(a, aSize, b, bSize, kernel);
+    }
+
+  private:
+    HipIterativeCombing<Input, WithIf> inner;
+};
+
 // Replaces semi_local::OpenMPHybridIterative<Input, WithIf> (SemiLocalLCS.h:
 // 594-794): same constructor, same block heuristic (:606-646); blocks are
 // combed in one batched launch and merged by batched steady-ant levels.

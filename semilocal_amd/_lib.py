@@ -44,6 +44,7 @@ SIGNATURES = {
     "sl_last_error": ([], C.c_char_p),
     "sl_init": ([C.c_int], C.c_int),
     "sl_device_info": ([C.c_char_p, C.c_int], C.c_int),
+    "sl_probe_copy": ([C.c_int64, C.POINTER(C.c_double)], C.c_int),
     "sl_semi_local_comb": ([_vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "sl_llcs_from_kernel": ([_i32p, C.c_int, C.c_int], C.c_int),
     "sl_batch_create": ([C.POINTER(_vp), C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, C.c_int, C.c_int],
@@ -507,3 +508,10 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def probe_copy(nbytes: int) -> float:
+    """One plain device copy of nbytes (read nbytes, write nbytes); ms."""
+    ms = C.c_double()
+    check(lib().sl_probe_copy(int(nbytes), C.byref(ms)))
+    return ms.value

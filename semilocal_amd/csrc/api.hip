@@ -534,6 +534,48 @@ int sl_device_info(char *buf, int len) {
     return kOk;
 }
 
+// Plain streaming copy (16 B per lane per access), the byte-counter scale.
+__global__ void probe_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int sl_probe_copy(int64_t bytes, double *ms) {
+    if (bytes <= 0 || (bytes & 15)) {
+        set_error("sl_probe_copy: bytes must be a positive multiple of 16");
+        return kErrArg;
+    }
+    int rc = ensure_device();
+    if (rc) return rc;
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&src, (size_t)bytes);
+    if (e == hipSuccess) e = hipMalloc(&dst, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemset(src, 1, (size_t)bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(probe_copy_kernel, dim3(8192), dim3(256), 0, nullptr, (const uint4 *)src, (uint4 *)dst,
+                           (uint64_t)bytes / 16);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t;
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) {
+        set_error(std::string("sl_probe_copy: ") + hipGetErrorString(e));
+        return kErrHip;
+    }
+    return kOk;
+}
+
 int sl_llcs_from_kernel(const int32_t *row_to_col, int m, int n) {
     if (!row_to_col || m < 0 || n < 0) return -1;
     int acc = 0;
