@@ -108,6 +108,19 @@ int main() {
         report("sticky_braid_mpi<unsigned short,...> 900x1100 sigma 300", same(ref, hip));
     }
 
+    {   // first_and_third_phase_combined and the recursive hybrid
+        const int m = 1200, n = 1700;
+        auto a = symbols(g, m, 4), b = symbols(g, n, 4);
+        Permutation ref(m + n, m + n), hip(m + n, m + n);
+        semi_local::first_and_third_phase_combined<int, false, true>(ref, a.data(), m, b.data(), n, map, 0, 4);
+        semilocal_hip::first_and_third_phase_combined<int, false, true>(hip, a.data(), m, b.data(), n, map, 0, 4);
+        report("first_and_third_phase_combined<int,false,true> 1200x1700", same(ref, hip));
+        Permutation ref_h(m + n, m + n), hip_h(m + n, m + n);
+        semi_local::hybrid<int, false, true, true>(ref_h, a.data(), m, b.data(), n, map, 1, 0, 3, 400, 1);
+        semilocal_hip::hybrid<int, false, true, true>(hip_h, a.data(), m, b.data(), n, map, 1, 0, 3, 400, 1);
+        report("hybrid<int,false,true,true> (recursive) 1200x1700", same(ref_h, hip_h));
+    }
+
     // ---- flat API: hybrid --------------------------------------------------
     for (auto shape : {std::pair<int, int>{2, 2}, {3, 5}, {8, 8}, {1, 16}}) {
         const int m = 3000, n = 2500;
@@ -209,6 +222,19 @@ int main() {
         cpu_h.compute(a.data(), m, b.data(), n, ref_h);
         dev_h.compute(a.data(), m, b.data(), n, hip_h);
         report("OpenMPHybridIterative vs HipHybridIterative 1500x2600", same(ref_h, hip_h) && same(ref, hip_h));
+
+        semi_local::OpenMPIterativeCombingCombined<int, false> cpu_c(4, &stag);
+        semilocal_hip::HipIterativeCombingCombined<int, false> dev_c(4, &stag);
+        common::Permutation ref_c, hip_c;
+        cpu_c.compute(a.data(), m, b.data(), n, ref_c);
+        dev_c.compute(a.data(), m, b.data(), n, hip_c);
+        report("OpenMPIterativeCombingCombined vs HipIterativeCombingCombined", same(ref_c, hip_c));
+        semi_local::OpenMPHybridRecursive<int, false, true, true> cpu_r(1, &stag, 3, 800, &cpu);
+        semilocal_hip::HipHybridRecursive<int, false, true, true> dev_r(1, &stag, 3, 800, &dev);
+        common::Permutation ref_r, hip_r;
+        cpu_r.compute(a.data(), m, b.data(), n, ref_r);
+        dev_r.compute(a.data(), m, b.data(), n, hip_r);
+        report("OpenMPHybridRecursive vs HipHybridRecursive", same(ref_r, hip_r));
 
         lcs::LCSBySemiLocal<int> lcs_cpu(&cpu), lcs_dev(&dev);
         semilocal_hip::HipLLCS<int> llcs_dev;
