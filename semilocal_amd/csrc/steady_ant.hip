@@ -243,12 +243,103 @@ __device__ __forceinline__ void ant_walk(int r, int c, int d, int s, int steps, 
     }
 }
 
+// The same walk by a whole wave, 8 x 8 cells at a time.  Lane (i, j) =
+// (lane >> 3, lane & 7) evaluates delta at (r - i, c + j):
+//     delta(r - i, c + j) = d + sum_{j' < j} wcol(c + j', r)
+//                             - sum_{1 <= i' <= i} wrow(r - i', c + j),
+// where wcol(col, row) is delta's step when the walk moves right past column
+// col in that row and wrow(row, col) its step when it moves up into that row.
+// Both sums are population counts of two ballots under per-lane masks, so a
+// box costs one round trip of reads and no cross-lane shuffles.  A third
+// ballot, of (delta < 0), holds every move of the walk inside the box; a
+// scalar loop follows it one row at a time (count of trailing ones = the run
+// of right moves) until the path reaches the box's last row or column, and
+// every visited cell then tests the good-point condition at once.  rowv /
+// colv clamp out-of-range indices themselves: such values only reach cells
+// the walk never visits.
+template <class RowF, class ColF, class PutF>
+__device__ __forceinline__ void ant_walk_box(int r, int c, int d, int s, int steps, RowF rowv, ColF colv, PutF put) {
+    const int lane = threadIdx.x & 63, li = lane >> 3, lj = lane & 7;
+    const uint64_t colmask = 0x0101010101010101ull << lj;   // lanes of column j
+    const uint64_t upto = ~0ull >> (63 - lane);              // lanes 0 .. lane
+    const uint32_t jmask = (1u << lj) - 1u;                  // columns before j
+    int t = 0;
+    for (;;) {
+        const uint32_t ur = rowv(r - li);        // point of row r - i
+        const uint32_t uc = colv(c + lj);        // point of column c + j
+        const uint32_t ul = colv(c + lj - 1);    // point of column c + j - 1
+        const int rowc = (int)(uc & kValMask);
+        const bool wcol = (uc & kHiBit) ? (rowc < r) : (rowc >= r);
+        const int colr = (int)(ur & kValMask);
+        const bool wrow = (ur & kHiBit) ? (colr < c + lj) : (colr >= c + lj);
+        const uint64_t W = __builtin_amdgcn_ballot_w64(li == 0 && wcol);
+        const uint64_t Bm = __builtin_amdgcn_ballot_w64(li >= 1 && wrow);
+        const int delta = d + __builtin_popcount((uint32_t)W & jmask) - __builtin_popcountll(Bm & colmask & upto);
+        const uint64_t M = __builtin_amdgcn_ballot_w64(delta < 0);
+        int i = 0, j = 0, ni = 0, nj = 0;
+        uint64_t V = 0;
+        bool fin = false;
+        for (;;) {
+            const uint32_t rowb = (uint32_t)(M >> (8 * i)) & 0xFFu;
+            const int run = __builtin_ctz(~(rowb >> j));   // delta < 0 at columns j .. j + run - 1
+            const int jc = max(j, s - c);                  // moving right from column >= s ends the walk
+            int term, kind;                                // kind 0: ends, 1: leaves right, 2: moves up
+            if (run > 0 && jc <= min(j + run - 1, 6)) {
+                term = jc;
+                kind = 0;
+            } else if (j + run >= 7) {
+                term = 6;
+                kind = 1;
+            } else {
+                term = j + run;
+                kind = 2;
+            }
+            int cnt = term - j + 1;
+            if (cnt > steps - t) {
+                cnt = steps - t;
+                kind = 0;
+            }
+            if (cnt > 0) V |= (((1ull << cnt) - 1ull) << j) << (8 * i);
+            t += cnt;
+            if (kind == 0) {
+                fin = true;
+                break;
+            }
+            if (kind == 1) {
+                ni = i;
+                nj = 7;
+                break;
+            }
+            if (r - i == 0) {   // moving up from row 0 ends the walk
+                fin = true;
+                break;
+            }
+            ++i;
+            j = term;
+            if (i == 7) {
+                ni = 7;
+                nj = j;
+                break;
+            }
+        }
+        const int rowl = (int)(ul & kValMask);
+        const bool wleft = (ul & kHiBit) ? (rowl < r - li) : (rowl >= r - li);
+        if (((V >> lane) & 1ull) && c + lj > 0 && delta - (int)wleft < 0 && delta + (int)wrow > 0)
+            put(r - li, c + lj - 1);
+        if (fin) break;
+        d = __builtin_amdgcn_readlane(delta, ni * 8 + nj);
+        r -= ni;
+        c += nj;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Small subproblems: one block solves every subproblem of size <= S0 <= kTile
 // inside its tile, from the split down to size 1 and back up, in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kSmallLevels = 10;   // log2(kTile)
-constexpr int kSmallSeg = 64;      // walk steps per segment inside the small solver
+constexpr int kSmallSeg = 128;     // walk steps per segment inside the small solver
+constexpr int kSmallBoxS = 16;     // from this subproblem size on, whole waves walk (8 x 8 boxes)
 constexpr uint32_t kHi16 = 0x8000u;
 
 // LDS points of the small solver: value | hi bit 15 (values < kTile).
@@ -340,8 +431,8 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
         __syncthreads();
         if (in) sR[x] = (int16_t)(sU[x] & 0x7FFFu);
         __syncthreads();
-        if (S < kSmallSeg) {
-            // short walks (< 2 kSmallSeg steps): one lane per subproblem
+        if (S < kSmallBoxS) {
+            // short walks (< 2 kSmallBoxS steps): one lane per subproblem
             if (in && x == seg0) {
                 const int s = min(S, len - x);
                 if (s > h) {
@@ -353,9 +444,10 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
                 }
             }
         } else {
-            // segments of kSmallSeg steps: start points by binary search on
-            // anti-diagonals (delta from an LDS scan), one wave per segment
-            const int nsps = (2 * S) / kSmallSeg;
+            // segments of kSmallSeg steps walked by whole waves; the start of
+            // each is found by binary search on its anti-diagonal (delta from
+            // an LDS scan)
+            const int nsps = max(1, (2 * S) / kSmallSeg);
             const int nsub = (len + S - 1) / S;
             for (int w = wid; w < nsub * nsps; w += kTile / 64) {
                 const int sub = w / nsps, k = w - sub * nsps;
@@ -372,12 +464,11 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
                 }
                 const int c = a, r = s - 1 - t0 + c;
                 const int d = wave_delta_lds(U16, s, r, c);
-                if (lane == 0) {
-                    int16_t *R = sR + s0;
-                    ant_walk(r, c, d, s, min(kSmallSeg, t_end - t0 + 1),
-                             [&](int rr) { return widen16(U16[rr]); }, [&](int cc) { return widen16(Ui16[cc]); },
+                int16_t *R = sR + s0;
+                const int steps = nsps == 1 ? t_end + 1 : min(kSmallSeg, t_end - t0 + 1);
+                ant_walk_box(r, c, d, s, steps, [&](int rr) { return widen16(U16[max(rr, 0)]); },
+                             [&](int cc) { return widen16(Ui16[min(max(cc, 0), s - 1)]); },
                              [&](int rr, int cc) { R[rr] = (int16_t)cc; });
-                }
             }
         }
         __syncthreads();
@@ -574,7 +665,7 @@ __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__res
 }
 
 // Walk of one segment: the wave stages the rows and columns the walk can
-// reach in LDS, lane 0 walks and writes good points over R.
+// reach in LDS, then walks them in 8 x 8 boxes and writes good points over R.
 __global__ __launch_bounds__(64) void seg_walk(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
                                                const SaSeg *__restrict__ segs, const int2 *__restrict__ walks,
                                                const int32_t *__restrict__ c_end,
@@ -609,11 +700,11 @@ __global__ __launch_bounds__(64) void seg_walk(const uint32_t *__restrict__ U, c
         colsU[lane + 64 * k] = tc[k];
     }
     __syncthreads();
-    if (lane == 0) {
-        int32_t *Rs = R + x0;
-        ant_walk(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1), [&](int rr) { return rowsU[rr - r_lo]; },
-                 [&](int cc) { return colsU[cc - c_lo]; }, [&](int rr, int cc) { Rs[rr] = cc; });
-    }
+    int32_t *Rs = R + x0;
+    const int c_hi = c_lo + max(ncol, 1) - 1;
+    ant_walk_box(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1),
+                 [&](int rr) { return rowsU[max(rr, r_lo) - r_lo]; },
+                 [&](int cc) { return colsU[min(max(cc, c_lo), c_hi) - c_lo]; }, [&](int rr, int cc) { Rs[rr] = cc; });
 }
 
 __global__ void fill_i32(int32_t *__restrict__ p, int32_t v, int64_t count) {
