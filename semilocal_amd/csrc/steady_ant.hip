@@ -338,8 +338,9 @@ __device__ __forceinline__ void ant_walk_box(int r, int c, int d, int s, int ste
 // inside its tile, from the split down to size 1 and back up, in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kSmallLevels = 10;   // log2(kTile)
-constexpr int kSmallSeg = 128;     // walk steps per segment inside the small solver
-constexpr int kSmallBoxS = 16;     // from this subproblem size on, whole waves walk (8 x 8 boxes)
+constexpr int kSmallSeg = 64;      // walk steps per segment inside the small solver
+constexpr int kSmallBoxS = 16;     // from this subproblem size on, walks are cut into searched segments
+constexpr int kSmallMaxSeg = 64;   // segments per level and tile: max(kTile / kSmallBoxS, 2 kTile / kSmallSeg)
 constexpr uint32_t kHi16 = 0x8000u;
 
 // LDS points of the small solver: value | hi bit 15 (values < kTile).
@@ -371,6 +372,7 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
     __shared__ uint16_t sU[kTile], sUi[kTile];
     __shared__ int16_t segP[kTile], segQ[kTile];
     __shared__ int sCend[kTile / 2];
+    __shared__ int4 sStart[kSmallMaxSeg];
     __shared__ int tmp[kTile / 64 + 1];
     const int t = blockIdx.x;
     const SaProb pb = probs[tile_prob[t]];
@@ -432,7 +434,7 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
         if (in) sR[x] = (int16_t)(sU[x] & 0x7FFFu);
         __syncthreads();
         if (S < kSmallBoxS) {
-            // short walks (< 2 kSmallBoxS steps): one lane per subproblem
+            // short walks (< 2 kSmallBoxS steps): one lane per subproblem, from the start
             if (in && x == seg0) {
                 const int s = min(S, len - x);
                 if (s > h) {
@@ -444,31 +446,43 @@ __global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__
                 }
             }
         } else {
-            // segments of kSmallSeg steps walked by whole waves; the start of
-            // each is found by binary search on its anti-diagonal (delta from
-            // an LDS scan)
+            // segments of kSmallSeg steps.  Their start points are found by
+            // binary search on anti-diagonals, one wave per segment (delta from
+            // an LDS scan); then every segment is walked by its own lane, so
+            // the walks run side by side on the vector ALU.
             const int nsps = max(1, (2 * S) / kSmallSeg);
             const int nsub = (len + S - 1) / S;
-            for (int w = wid; w < nsub * nsps; w += kTile / 64) {
+            const int nseg = nsub * nsps;   // <= kSmallMaxSeg
+            for (int w = wid; w < nseg; w += kTile / 64) {
                 const int sub = w / nsps, k = w - sub * nsps;
                 const int s0 = sub * S, s = min(S, len - s0);
                 const int t_end = s - 1 + sCend[sub];
                 const int t0 = k * kSmallSeg;
-                if (s <= h || t0 > t_end) continue;   // uniform over the wave
-                const uint16_t *U16 = sU + s0, *Ui16 = sUi + s0;
-                int a = max(0, t0 - s + 1), b = min(t0, s);
-                while (a < b) {
-                    const int mid = (a + b) >> 1;
-                    if (wave_delta_lds(U16, s, s - t0 + mid, mid) >= 0) b = mid;   // delta(r(mid)+1, mid)
-                    else a = mid + 1;
+                int4 st = make_int4(-1, 0, 0, 0);
+                if (s > h && t0 <= t_end) {   // uniform over the wave
+                    const uint16_t *U16 = sU + s0;
+                    int a = max(0, t0 - s + 1), b = min(t0, s);
+                    while (a < b) {
+                        const int mid = (a + b) >> 1;
+                        if (wave_delta_lds(U16, s, s - t0 + mid, mid) >= 0) b = mid;   // delta(r(mid)+1, mid)
+                        else a = mid + 1;
+                    }
+                    const int c = a, r = s - 1 - t0 + c;
+                    st = make_int4(r, c, wave_delta_lds(U16, s, r, c),
+                                   nsps == 1 ? t_end + 1 : min(kSmallSeg, t_end - t0 + 1));
                 }
-                const int c = a, r = s - 1 - t0 + c;
-                const int d = wave_delta_lds(U16, s, r, c);
-                int16_t *R = sR + s0;
-                const int steps = nsps == 1 ? t_end + 1 : min(kSmallSeg, t_end - t0 + 1);
-                ant_walk_box(r, c, d, s, steps, [&](int rr) { return widen16(U16[max(rr, 0)]); },
-                             [&](int cc) { return widen16(Ui16[min(max(cc, 0), s - 1)]); },
-                             [&](int rr, int cc) { R[rr] = (int16_t)cc; });
+                if (lane == 0) sStart[w] = st;
+            }
+            __syncthreads();
+            if (x < nseg) {
+                const int4 st = sStart[x];
+                if (st.x >= 0) {
+                    const int sub = x / nsps, s0 = sub * S, s = min(S, len - s0);
+                    const uint16_t *U16 = sU + s0, *Ui16 = sUi + s0;
+                    int16_t *R = sR + s0;
+                    ant_walk(st.x, st.y, st.z, s, st.w, [&](int rr) { return widen16(U16[rr]); },
+                             [&](int cc) { return widen16(Ui16[cc]); }, [&](int rr, int cc) { R[rr] = (int16_t)cc; });
+                }
             }
         }
         __syncthreads();
@@ -664,47 +678,29 @@ __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__res
     if ((threadIdx.x & 63) == 0) seg_start[w] = out;
 }
 
-// Walk of one segment: the wave stages the rows and columns the walk can
-// reach in LDS, then walks them in 8 x 8 boxes and writes good points over R.
-__global__ __launch_bounds__(64) void seg_walk(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
-                                               const SaSeg *__restrict__ segs, const int2 *__restrict__ walks,
-                                               const int32_t *__restrict__ c_end,
-                                               const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
-    constexpr int kStage = (kSegSteps + 1 + 63) / 64;
-    __shared__ uint32_t rowsU[kStage * 64], colsU[kStage * 64];
-    const int w = blockIdx.x;
-    const int lane = threadIdx.x;
+// Walks: one lane per segment, reading the level's U / U^-1 directly (one
+// round trip per step, both candidates of the next move in flight).  Blocks
+// are mapped to XCDs in contiguous ranges of segments (the hardware deals
+// blocks to the 8 XCDs round-robin), so each XCD's L2 holds the rows and
+// columns of its own part of the diagonal range.
+__global__ __launch_bounds__(256) void seg_walk(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
+                                                const SaSeg *__restrict__ segs, const int2 *__restrict__ walks,
+                                                int nwalks, const int32_t *__restrict__ c_end,
+                                                const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
+    const int per_xcd = (int)gridDim.x / 8;   // the grid is a multiple of 8 blocks
+    const int b = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);   // XCD-contiguous block order
+    const int w = b * 256 + (int)threadIdx.x;
+    if (w >= nwalks) return;
     const int4 st = seg_start[w];
-    if (st.x < 0) return;   // uniform over the block (one wave)
+    if (st.x < 0) return;
     const int2 job = walks[w];
     const SaSeg sg = segs[job.x];
-    const int64_t x0 = sg.x0;
+    const uint32_t *Us = U + sg.x0, *Uis = Ui + sg.x0;
+    int32_t *Rs = R + sg.x0;
     const int s = sg.s;
     const int t_end = s - 1 + c_end[job.x];
-    // kSegSteps evaluations move up at most kSegSteps rows and right at most
-    // kSegSteps columns; column c-1 is read before the first move.
-    const int r_lo = max(0, st.x - kSegSteps);
-    const int nrow = st.x - r_lo + 1;
-    const int c_lo = max(0, st.y - 1);   // <= s - 1
-    const int ncol = max(0, min(s, st.y + kSegSteps) - c_lo);
-    uint32_t tr[kStage], tc[kStage];
-#pragma unroll
-    for (int k = 0; k < kStage; ++k) {
-        const int q = lane + 64 * k;
-        tr[k] = U[x0 + r_lo + (q < nrow ? q : 0)];
-        tc[k] = Ui[x0 + c_lo + (q < ncol ? q : 0)];
-    }
-#pragma unroll
-    for (int k = 0; k < kStage; ++k) {
-        rowsU[lane + 64 * k] = tr[k];
-        colsU[lane + 64 * k] = tc[k];
-    }
-    __syncthreads();
-    int32_t *Rs = R + x0;
-    const int c_hi = c_lo + max(ncol, 1) - 1;
-    ant_walk_box(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1),
-                 [&](int rr) { return rowsU[max(rr, r_lo) - r_lo]; },
-                 [&](int cc) { return colsU[min(max(cc, c_lo), c_hi) - c_lo]; }, [&](int rr, int cc) { Rs[rr] = cc; });
+    ant_walk(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1), [&](int rr) { return Us[rr]; },
+             [&](int cc) { return Uis[cc]; }, [&](int rr, int cc) { Rs[rr] = cc; });
 }
 
 __global__ void fill_i32(int32_t *__restrict__ p, int32_t v, int64_t count) {
