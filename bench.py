@@ -151,11 +151,15 @@ def bench_hybrid(S, W, golden):
     ok = W.sha256(perm) == golden["similar_pair_1e6"]["perm_sha256"]
     tot, comb, merge = h.time(warmup=1, iters=3)
     cells, levels = h.stats()
+    h.set_graph(True)                      # the same run captured once into a HIP graph, then replayed
+    ok = ok and W.sha256(h.run()) == golden["similar_pair_1e6"]["perm_sha256"]
+    gtot, _, _ = h.time(warmup=1, iters=3)
     h.close()
     return {"workload": "similar pair: a 1e6 symbols (alphabet 4), b = a with 5% subs and 1% indels (seed 3); "
                         "8x8 blocks", "ms": round(tot, 3), "comb_ms": round(comb, 3), "merge_ms": round(merge, 3),
-            "merge_levels": levels, "gcups": round(cells / (tot * 1e-3) / 1e9, 2),
-            "parity": "bit-exact vs the reference hybrid's fixture" if ok else "MISMATCH"}
+            "graph_ms": round(gtot, 3), "merge_levels": levels, "gcups": round(cells / (tot * 1e-3) / 1e9, 2),
+            "graph_gcups": round(cells / (gtot * 1e-3) / 1e9, 2),
+            "parity": "bit-exact vs the reference hybrid's fixture (eager and graph)" if ok else "MISMATCH"}
 
 
 def bench_multiply(S):

@@ -211,6 +211,11 @@ int sl_hybrid_time(sl_hybrid *h, int warmup, int iters, double *ms_per_run, doub
                    double *merge_ms);
 /* Cells combed per run (sum of block areas) and number of merge levels. */
 int sl_hybrid_stats(const sl_hybrid *h, int64_t *cells, int *merge_levels);
+/* enable != 0: the first run captures the whole pipeline (combing launch and
+ * every merge level) into a HIP graph on the handle's stream, and every run
+ * replays it with one graph launch.  Results are unchanged.  In graph mode
+ * sl_hybrid_time reports the total only (comb_ms and merge_ms are -1). */
+int sl_hybrid_set_graph(sl_hybrid *h, int enable);
 void sl_hybrid_destroy(sl_hybrid *h);
 
 /* ------------------------------------------------------------------------ */

@@ -79,6 +79,7 @@ SIGNATURES = {
                         C.POINTER(C.c_double)], C.c_int),
     "sl_hybrid_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)], C.c_int),
     "sl_hybrid_destroy": ([_vp], None),
+    "sl_hybrid_set_graph": ([_vp, C.c_int], C.c_int),
     "sl_server_create": ([C.POINTER(_vp), C.c_int64, C.c_int, C.c_int, C.c_int64], C.c_int),
     "sl_server_submit_compare": ([_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)],
                                  C.c_int),
@@ -412,6 +413,10 @@ class Hybrid:
         tot, comb, merge = C.c_double(), C.c_double(), C.c_double()
         check(lib().sl_hybrid_time(self.h, warmup, iters, C.byref(tot), C.byref(comb), C.byref(merge)))
         return tot.value, comb.value, merge.value
+
+    def set_graph(self, enable: bool = True):
+        """Capture the whole run into a HIP graph and replay it (same results)."""
+        check(lib().sl_hybrid_set_graph(self.h, 1 if enable else 0))
 
     def stats(self):
         cells, levels = C.c_int64(), C.c_int()

@@ -465,6 +465,20 @@ static int batch_run(Batch *bt) {
     return kOk;
 }
 
+// Launch meant for replay from a captured HIP graph.  The epoch is captured
+// into the kernel arguments and stays fixed, so the strand buffer is cleared
+// on the stream first (a fill kernel, not a memset node): a tag written by an
+// earlier replay can then never be taken for this replay's.
+static int batch_run_replayable(Batch *bt) {
+    const int rc = batch_check_ready(bt);
+    if (rc) return rc;
+    if (bt->epoch == 0) bt->epoch = 1;
+    SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_buf, 0u, (uint64_t)bt->buf_total * 2u, bt->stream));
+    SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
+    SL_HIP_TRY(launch_items(bt, bt->d_items, (uint32_t)bt->items.size(), bt->d_ctl));
+    return kOk;
+}
+
 static int batch_run_per_pair(Batch *bt) {
     int rc = batch_reset(bt);
     if (rc) return rc;
@@ -504,6 +518,7 @@ int batch_create_arena(Batch **out, int npairs, const void *a_cat, const int64_t
     return batch_create(out, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes, 0, 0, arena, stream);
 }
 int batch_launch_internal(Batch *bt) { return batch_run(bt); }
+int batch_launch_replayable_internal(Batch *bt) { return batch_run_replayable(bt); }
 int batch_wait_internal(Batch *bt) { return batch_sync(bt); }
 void batch_destroy_internal(Batch *bt) { batch_free(bt); }
 hipStream_t batch_stream_internal(const Batch *bt) { return bt->stream; }

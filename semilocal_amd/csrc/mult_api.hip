@@ -21,6 +21,7 @@ struct Batch;
 int batch_create_internal(Batch **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
                           const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes);
 int batch_launch_internal(Batch *bt);
+int batch_launch_replayable_internal(Batch *bt);
 int batch_wait_internal(Batch *bt);
 void batch_destroy_internal(Batch *bt);
 hipStream_t batch_stream_internal(const Batch *bt);
@@ -436,6 +437,9 @@ struct Hybrid {
     int64_t final_off = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     int64_t cells = 0;
+    hipGraph_t graph = nullptr;          // the whole run (combing + merge levels), captured once
+    hipGraphExec_t graph_exec = nullptr;
+    bool use_graph = false;
 };
 
 static void hybrid_free(Hybrid *h) {
@@ -448,6 +452,8 @@ static void hybrid_free(Hybrid *h) {
         (void)hipFree(L.d_qcr);
         (void)hipFree(L.d_moves);
     }
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
     (void)hipFree(h->d_pool[0]);
     (void)hipFree(h->d_pool[1]);
     for (auto &e : h->ev)
@@ -641,7 +647,45 @@ static int hybrid_merges(Hybrid *h) {
     return kOk;
 }
 
+// Captures one run (the combing launch and every merge level) into a HIP
+// graph on the batch's stream; later runs replay it with one launch.
+static int hybrid_capture(Hybrid *h) {
+    hipStream_t s = batch_stream_internal(h->batch);
+    SL_HIP_TRY(hipStreamSynchronize(s));
+    SL_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    int rc = batch_launch_replayable_internal(h->batch);
+    if (!rc) rc = hybrid_merges(h);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);   // ends the capture on every path
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return kErrHip;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        return kErrHip;
+    }
+    h->graph = g;
+    h->graph_exec = ex;
+    return kOk;
+}
+
 static int hybrid_run(Hybrid *h) {
+    if (h->use_graph) {
+        if (!h->graph_exec) {
+            const int rc = hybrid_capture(h);
+            if (rc) return rc;
+        }
+        SL_HIP_TRY(hipGraphLaunch(h->graph_exec, batch_stream_internal(h->batch)));
+        return kOk;
+    }
     int rc = batch_launch_internal(h->batch);
     if (!rc) rc = hybrid_merges(h);
     return rc;
@@ -851,6 +895,22 @@ int sl_hybrid_time(sl_hybrid *hh, int warmup, int iters, double *ms_per_run, dou
         if (rc) return rc;
     }
     double tot = 0, tc = 0, tm = 0;
+    if (h->use_graph) {   // one graph launch per run: no split into phases
+        SL_HIP_TRY(hipEventRecord(h->ev[0], s));
+        for (int i = 0; i < iters; ++i) {
+            const int rc = hybrid_run(h);
+            if (rc) return rc;
+        }
+        SL_HIP_TRY(hipEventRecord(h->ev[2], s));
+        const int rc = batch_wait_internal(h->batch);
+        if (rc) return rc;
+        float a = 0;
+        SL_HIP_TRY(hipEventElapsedTime(&a, h->ev[0], h->ev[2]));
+        *ms_per_run = a / iters;
+        if (comb_ms) *comb_ms = -1.0;
+        if (merge_ms) *merge_ms = -1.0;
+        return kOk;
+    }
     for (int i = 0; i < iters; ++i) {
         SL_HIP_TRY(hipEventRecord(h->ev[0], s));
         int rc = batch_launch_internal(h->batch);
@@ -882,6 +942,16 @@ int sl_hybrid_stats(const sl_hybrid *hh, int64_t *cells, int *merge_levels) {
     }
     if (cells) *cells = h->cells;
     if (merge_levels) *merge_levels = (int)h->levels.size();
+    return kOk;
+}
+
+int sl_hybrid_set_graph(sl_hybrid *hh, int enable) {
+    Hybrid *h = reinterpret_cast<Hybrid *>(hh);
+    if (!h) {
+        set_error("sl_hybrid_set_graph: null handle");
+        return kErrArg;
+    }
+    h->use_graph = enable != 0;
     return kOk;
 }
 

@@ -147,3 +147,25 @@ def test_resident_hybrid_repeats(gpu):
     cells, levels = h.stats()
     assert cells == len(a) * len(b) and levels == 4
     h.close()
+
+
+def test_hybrid_graph_replays_equal_the_eager_run(gpu):
+    # the whole pipeline captured into a HIP graph and replayed: every replay,
+    # with eager work and synchronisation in between, equals the eager run
+    g = _rng(37)
+    a = g.integers(0, 4, size=30000).astype(np.uint8)
+    b = g.integers(0, 4, size=26000).astype(np.uint8)
+    eager = gpu.Hybrid(a, b, 4, 4).run()
+    assert np.array_equal(eager, gpu.semi_local_comb(a, b))
+    h = gpu.Hybrid(a, b, 4, 4)
+    h.set_graph(True)
+    for k in range(4):
+        assert np.array_equal(h.run(), eager), k
+        x = gpu.semi_local_comb(a[:3000], b[:2000])            # unrelated eager work between replays
+        assert gpu.llcs_from_kernel(x, 3000, 2000) >= 0
+    tot, comb, merge = h.time(warmup=1, iters=3)
+    assert tot > 0 and comb == -1 and merge == -1
+    assert np.array_equal(h.run(), eager)
+    h.set_graph(False)                                        # back to eager launches on the same handle
+    assert np.array_equal(h.run(), eager)
+    h.close()
