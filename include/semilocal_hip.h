@@ -113,8 +113,9 @@ int sl_batch_time(sl_batch *bt, int mode, int warmup, int iters, double *ms_per_
  * Results never depend on it (the semi-local kernel is unique). */
 int sl_batch_set_tiles(sl_batch *bt, int rows_per_lane, int waves_per_cu);
 int sl_batch_get_tiles(const sl_batch *bt, int *rows_per_lane, int *waves_per_cu);
-/* Autotuner: times every tile shape on this batch (`iters` runs each, HIP
- * events) and keeps the fastest; reports it through the out-pointers. */
+/* Autotuner: times every tile shape -- rows per lane {4, 8, 16} x persistent
+ * waves per CU {8, 12, 16, 20, 24, 32} -- on this batch (`iters` runs each,
+ * HIP events) and keeps the fastest; reports it through the out-pointers. */
 int sl_batch_autotune(sl_batch *bt, int iters, int *rows_per_lane, int *waves_per_cu);
 /* Sum of m*n over the batch (cells), number of queue tiles, number of pairs. */
 int sl_batch_stats(const sl_batch *bt, int64_t *cells, int64_t *tiles, int64_t *pairs);

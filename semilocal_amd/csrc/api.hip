@@ -216,7 +216,7 @@ struct Batch {
     int64_t cells = 0;
     uint32_t spin_limit = 1u << 23;
     int rows_per_lane = 16;            // tile height of every pass of this batch
-    unsigned waves_per_cu = 16;
+    unsigned waves_per_cu = 24;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     PairDesc *d_pairs = nullptr;
@@ -735,7 +735,7 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
         set_error("sl_batch_autotune: bad argument");
         return kErrArg;
     }
-    const int wpc_cand[3] = {8, 12, 16};
+    const int wpc_cand[6] = {8, 12, 16, 20, 24, 32};
     double best = 1e30;
     int best_r = bt->rows_per_lane, best_w = (int)bt->waves_per_cu;
     for (int r : kTileHeights) {

@@ -103,18 +103,19 @@ __device__ __forceinline__ void wave_lds_sync() {
 // symbols match or the left strand already crossed the top one (l > t).
 // A match lifts l above every strand id (a tag bit above the id bits), so a
 // single unsigned min gives the new left strand; since {l', t'} = {l, t} as a
-// multiset, t' = l ^ t ^ l'.
+// multiset, t' = l ^ t ^ l', one v_bitop3_b32 (three-input XOR, truth table
+// 0x96) on gfx950.
 //  ONEHOT: A and B are one-hot symbol masks shifted above the id bits, so the
-//          tag is (A & B): v_and_or_b32, v_min_u32, v_xor_b32 x2 (4 VALU).
+//          tag is (A & B): v_and_or_b32, v_min_u32, v_bitop3_b32 (3 VALU).
 //  else:   A and B are symbol codes; ((A ^ B) - 1) has bit 31 set iff they are
-//          equal: v_xad_u32, v_and_or_b32, v_min_u32, v_xor_b32 x2 (5 VALU).
+//          equal: v_xad_u32, v_and_or_b32, v_min_u32, v_bitop3_b32 (4 VALU).
 template <bool ONEHOT>
 __device__ __forceinline__ void comb_cell(uint32_t A, uint32_t B, uint32_t &l, uint32_t &t) {
     uint32_t x;
     if (ONEHOT) x = (A & B) | l;
     else x = (((A ^ B) - 1u) & 0x80000000u) | l;
     const uint32_t nl = x < t ? x : t;
-    t = (l ^ t) ^ nl;
+    t = __builtin_amdgcn_bitop3_b32(l, t, nl, 0x96);
     l = nl;
 }
 
