@@ -257,8 +257,8 @@ def main():
     batch = semilocal_amd.Batch(pairs, rows_per_lane=args.rows_per_lane)
     if not args.rows_per_lane:
         t0 = time.perf_counter()
-        tiles = batch.autotune(iters=1)
-        log(f"autotuner picked rows_per_lane={tiles[0]} waves_per_cu={tiles[1]} "
+        tiles = batch.autotune(iters=2)
+        log(f"autotuner picked rows_per_lane={tiles[0]} waves_per_cu={tiles[1]} schedule={batch.schedule()} "
             f"in {time.perf_counter() - t0:.1f}s")
     tiles = batch.tiles()
     st = batch.stats()
@@ -321,7 +321,7 @@ def main():
         "config": {"workload": "ragged batch: 4096 pairs, lengths log-normal (median 2000, sigma_ln 1.0) "
                                "clipped to [100, 50000], alphabet 20, half mutated (10% subs, 2% indels), seed 4",
                    "pairs": st["pairs"], "cells": cells, "tiles": st["tiles"],
-                   "rows_per_lane": tiles[0], "waves_per_cu": tiles[1]},
+                   "rows_per_lane": tiles[0], "waves_per_cu": tiles[1], "queue_schedule": batch.schedule()},
         "one_pair_per_launch": ({"value": round(gcups_pp, 2), "unit": "GCUPS", "ms_per_step": round(pms, 3)}
                                 if args.per_pair_steps > 0 else None),
         "batched_vs_per_pair": round(gcups / gcups_pp, 2) if args.per_pair_steps > 0 else None,

@@ -57,6 +57,8 @@ SIGNATURES = {
     "sl_batch_set_tiles": ([_vp, C.c_int, C.c_int], C.c_int),
     "sl_batch_get_tiles": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "sl_batch_autotune": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "sl_batch_set_schedule": ([_vp, C.c_int], C.c_int),
+    "sl_batch_get_schedule": ([_vp, C.POINTER(C.c_int)], C.c_int),
     "sl_batch_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "sl_batch_destroy": ([_vp], None),
     "sl_llcs_bitparallel": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
@@ -226,6 +228,15 @@ class Batch:
         r, w = C.c_int(), C.c_int()
         check(lib().sl_batch_get_tiles(self.h, C.byref(r), C.byref(w)))
         return r.value, w.value
+
+    def set_schedule(self, schedule: int):
+        """Queue order of the batched launch: 0 pair by pair (largest first), 1 simulated."""
+        check(lib().sl_batch_set_schedule(self.h, int(schedule)))
+
+    def schedule(self) -> int:
+        out = C.c_int()
+        check(lib().sl_batch_get_schedule(self.h, C.byref(out)))
+        return out.value
 
     def autotune(self, iters: int = 1):
         r, w = C.c_int(), C.c_int()

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <queue>
 #include <unordered_map>
 #include <vector>
 
@@ -209,7 +210,9 @@ struct Batch {
     int sym_bytes = 1;
     bool with_cr = false;
     std::vector<PairDesc> pairs;       // caller order
-    std::vector<WorkItem> items;       // LPT order, passes of a pair consecutive
+    std::vector<WorkItem> items;       // passes pair by pair (largest pairs first)
+    std::vector<WorkItem> order;       // ticket order of the batched launch
+    int schedule = 0;                  // 0: `items` as they are, 1: simulated list schedule (schedule_items)
     std::vector<int64_t> perm_off;     // caller order
     int64_t perm_total = 0;
     int64_t buf_total = 0;
@@ -259,8 +262,58 @@ static void batch_free(Batch *b) {
     delete b;
 }
 
-// Splits every pair into passes of 64R rows and orders the queue: largest
-// pairs first (longest-processing-time first), passes of a pair consecutive.
+// Queue order of the persistent launch.  Waves take tickets in order, and a
+// pass can only run some way behind its predecessor (it reads the strands
+// the previous pass leaves, ~128 steps behind), so handing out the ~200
+// passes of a large pair back to back would park hundreds of waves on
+// waits.  The order comes from a list-scheduling simulation instead: W
+// waves; a pass lasts n + 64 steps; pass p of a pair becomes ready 128 steps
+// after pass p-1 started; a free wave takes the ready pass of the largest
+// pair.  Passes of a pair keep their order, so a pass only ever waits on a
+// pass handed out before it.
+static void schedule_items(Batch *bt) {
+    const size_t nitems = bt->items.size();
+    bt->order.clear();
+    bt->order.reserve(nitems);
+    if (nitems == 0) return;
+    const double lag = 128.0;
+    struct Ready {
+        int64_t cells;
+        int pair;
+        bool operator<(const Ready &o) const { return cells < o.cells || (cells == o.cells && pair > o.pair); }
+    };
+    using Pending = std::pair<double, int>;   // (ready time, pair)
+    std::priority_queue<Ready> ready;
+    std::priority_queue<Pending, std::vector<Pending>, std::greater<Pending>> pending;
+    std::priority_queue<double, std::vector<double>, std::greater<double>> waves;
+    std::vector<int> next((size_t)bt->npairs, 0);
+    for (int i = 0; i < bt->npairs; ++i)
+        if (bt->pairs[i].npasses > 0) ready.push(Ready{(int64_t)bt->pairs[i].m * bt->pairs[i].n, i});
+    const size_t w = std::max<size_t>(1, std::min<size_t>(nitems, bt->waves_cap));
+    for (size_t k = 0; k < w; ++k) waves.push(0.0);
+    while (bt->order.size() < nitems) {
+        double t = waves.top();
+        waves.pop();
+        if (ready.empty() && !pending.empty() && pending.top().first > t) t = pending.top().first;
+        while (!pending.empty() && pending.top().first <= t) {
+            const int i = pending.top().second;
+            pending.pop();
+            ready.push(Ready{(int64_t)bt->pairs[i].m * bt->pairs[i].n, i});
+        }
+        const Ready r = ready.top();
+        ready.pop();
+        const PairDesc &d = bt->pairs[r.pair];
+        bt->order.push_back(WorkItem{r.pair, next[r.pair]});
+        waves.push(t + (double)d.n + 64.0);
+        if (++next[r.pair] < d.npasses) pending.push(Pending{t + lag, r.pair});
+    }
+}
+
+// Splits every pair into passes of 64R rows.  `items` lists the passes pair
+// by pair (largest pairs first; the one-launch-per-pair form launches a
+// pair's consecutive run), `order` is the ticket order of the batched launch:
+// `items` itself (longest-processing-time first) or the simulated schedule;
+// the autotuner measures both.
 static void plan_items(Batch *bt) {
     const int64_t rpp = 64LL * bt->rows_per_lane;
     std::vector<int> order(bt->npairs);
@@ -277,22 +330,29 @@ static void plan_items(Batch *bt) {
         for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
     }
     bt->waves_cap = (unsigned)device_cus() * bt->waves_per_cu;
+    if (bt->schedule == 1) schedule_items(bt);
+    else bt->order = bt->items;
 }
 
-// (Re)uploads the pair table and the queue; grows the device copies as needed.
+// (Re)uploads the pair table and the queue (device copy: `items`, then
+// `order`); grows the device copies as needed.
 static hipError_t upload_items(Batch *bt) {
     hipError_t e = hipSuccess;
     if (!bt->d_pairs) e = bt_alloc(bt, (void **)&bt->d_pairs, sizeof(PairDesc) * std::max<size_t>(bt->pairs.size(), 1));
     if (e == hipSuccess && !bt->pairs.empty())
         e = hipMemcpy(bt->d_pairs, bt->pairs.data(), sizeof(PairDesc) * bt->pairs.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && (!bt->d_items || bt->items.size() > bt->items_cap)) {
+    const size_t need = 2 * bt->items.size();
+    if (e == hipSuccess && (!bt->d_items || need > bt->items_cap)) {
         bt_free(bt, bt->d_items);
         bt->d_items = nullptr;
-        bt->items_cap = std::max<size_t>(bt->items.size(), 1);
+        bt->items_cap = std::max<size_t>(need, 1);
         e = bt_alloc(bt, (void **)&bt->d_items, sizeof(WorkItem) * bt->items_cap);
     }
     if (e == hipSuccess && !bt->items.empty())
         e = hipMemcpy(bt->d_items, bt->items.data(), sizeof(WorkItem) * bt->items.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !bt->order.empty())
+        e = hipMemcpy(bt->d_items + bt->items.size(), bt->order.data(), sizeof(WorkItem) * bt->order.size(),
+                      hipMemcpyHostToDevice);
     return e;
 }
 
@@ -461,7 +521,7 @@ static hipError_t launch_items(Batch *bt, const WorkItem *items, uint32_t nitems
 static int batch_run(Batch *bt) {
     int rc = batch_reset(bt);
     if (rc) return rc;
-    SL_HIP_TRY(launch_items(bt, bt->d_items, (uint32_t)bt->items.size(), bt->d_ctl));
+    SL_HIP_TRY(launch_items(bt, bt->d_items + bt->items.size(), (uint32_t)bt->order.size(), bt->d_ctl));
     return kOk;
 }
 
@@ -475,7 +535,7 @@ static int batch_run_replayable(Batch *bt) {
     if (bt->epoch == 0) bt->epoch = 1;
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_buf, 0u, (uint64_t)bt->buf_total * 2u, bt->stream));
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
-    SL_HIP_TRY(launch_items(bt, bt->d_items, (uint32_t)bt->items.size(), bt->d_ctl));
+    SL_HIP_TRY(launch_items(bt, bt->d_items + bt->items.size(), (uint32_t)bt->order.size(), bt->d_ctl));
     return kOk;
 }
 
@@ -683,7 +743,7 @@ int sl_batch_time(sl_batch *h, int mode, int warmup, int iters, double *ms_per_r
             if (rc) return rc;
             SL_HIP_TRY(hipEventRecord(bt->ev[2], bt->stream));
             if (mode == 0) {
-                SL_HIP_TRY(launch_items(bt, bt->d_items, (uint32_t)bt->items.size(), bt->d_ctl));
+                SL_HIP_TRY(launch_items(bt, bt->d_items + bt->items.size(), (uint32_t)bt->order.size(), bt->d_ctl));
             } else {
                 for (int i = 0; i < bt->npairs; ++i) {
                     const PairDesc &d = bt->pairs[i];
@@ -729,6 +789,29 @@ int sl_batch_get_tiles(const sl_batch *h, int *rows_per_lane, int *waves_per_cu)
     return kOk;
 }
 
+int sl_batch_set_schedule(sl_batch *h, int schedule) {
+    Batch *bt = reinterpret_cast<Batch *>(h);
+    if (!bt || (schedule != 0 && schedule != 1)) {
+        set_error("sl_batch_set_schedule: schedule must be 0 or 1");
+        return kErrArg;
+    }
+    SL_HIP_TRY(hipStreamSynchronize(bt->stream));
+    bt->schedule = schedule;
+    plan_items(bt);
+    SL_HIP_TRY(upload_items(bt));
+    return kOk;
+}
+
+int sl_batch_get_schedule(const sl_batch *h, int *schedule) {
+    const Batch *bt = reinterpret_cast<const Batch *>(h);
+    if (!bt || !schedule) {
+        set_error("sl_batch_get_schedule: bad argument");
+        return kErrArg;
+    }
+    *schedule = bt->schedule;
+    return kOk;
+}
+
 int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per_cu) {
     Batch *bt = reinterpret_cast<Batch *>(h);
     if (!bt || iters <= 0) {
@@ -737,21 +820,26 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
     }
     const int wpc_cand[6] = {8, 12, 16, 20, 24, 32};
     double best = 1e30;
-    int best_r = bt->rows_per_lane, best_w = (int)bt->waves_per_cu;
-    for (int r : kTileHeights) {
-        for (int w : wpc_cand) {
-            int rc = sl_batch_set_tiles(h, r, w);
-            if (rc) return rc;
-            double ms = 0.0;
-            rc = sl_batch_time(h, 0, 1, iters, &ms, nullptr);
-            if (rc) return rc;
-            if (ms < best) {
-                best = ms;
-                best_r = r;
-                best_w = w;
+    int best_r = bt->rows_per_lane, best_w = (int)bt->waves_per_cu, best_s = bt->schedule;
+    for (int sched = 0; sched < 2; ++sched) {
+        bt->schedule = sched;
+        for (int r : kTileHeights) {
+            for (int w : wpc_cand) {
+                int rc = sl_batch_set_tiles(h, r, w);
+                if (rc) return rc;
+                double ms = 0.0;
+                rc = sl_batch_time(h, 0, 1, iters, &ms, nullptr);
+                if (rc) return rc;
+                if (ms < best) {
+                    best = ms;
+                    best_r = r;
+                    best_w = w;
+                    best_s = sched;
+                }
             }
         }
     }
+    bt->schedule = best_s;
     int rc = sl_batch_set_tiles(h, best_r, best_w);
     if (rc) return rc;
     if (rows_per_lane) *rows_per_lane = best_r;

@@ -111,15 +111,18 @@ def test_autotune_and_tiles_do_not_change_results(gpu, oracle):
     assert b.tiles() == (r, w)
     for got, exp in zip(b.run().results(), want):
         assert np.array_equal(got, exp)
-    for r2, w2 in [(4, 8), (8, 12), (16, 16)]:
-        b.set_tiles(r2, w2)
-        for got, exp in zip(b.run().results(), want):
-            assert np.array_equal(got, exp)
+    for sched in (0, 1):                      # queue order: pair by pair, or the simulated schedule
+        b.set_schedule(sched)
+        assert b.schedule() == sched
+        for r2, w2 in [(4, 8), (8, 12), (16, 16), (4, 32)]:
+            b.set_tiles(r2, w2)
+            for got, exp in zip(b.run().results(), want):
+                assert np.array_equal(got, exp)
     b.close()
 
 
 def test_large_alphabet_pair_uses_code_mode(gpu, oracle):
-    # 40 symbols and 60k strands exceed the one-hot tag budget: the 5-op cell path
+    # 40 symbols and 60k strands exceed the one-hot tag budget: the code-compare cell path
     g = np.random.default_rng(16)
     a = g.integers(0, 40, 20000).astype(np.uint8)
     b = g.integers(0, 40, 40000).astype(np.uint8)
