@@ -114,13 +114,14 @@ int sl_batch_time(sl_batch *bt, int mode, int warmup, int iters, double *ms_per_
 int sl_batch_set_tiles(sl_batch *bt, int rows_per_lane, int waves_per_cu);
 int sl_batch_get_tiles(const sl_batch *bt, int *rows_per_lane, int *waves_per_cu);
 /* Queue order of the batched launch: 0 = pair by pair, largest pairs first
- * (the default); 1 = a list-scheduling simulation that hands out a pair's
- * passes about when each can start behind the previous one.  Results never
- * depend on it. */
+ * (the default); s = 1, 2, 3 = a list-scheduling simulation that hands out a
+ * pair's passes about when each can start behind the previous one, assuming
+ * a pass starts 128 s steps after its predecessor.  Results never depend on
+ * it. */
 int sl_batch_set_schedule(sl_batch *bt, int schedule);
 int sl_batch_get_schedule(const sl_batch *bt, int *schedule);
 /* Autotuner: times every tile shape -- rows per lane {4, 8, 16} x persistent
- * waves per CU {8, 12, 16, 20, 24, 32} x queue order {0, 1} -- on this batch
+ * waves per CU {8, 12, 16, 20, 24, 32} x queue order {0..3} -- on this batch
  * (`iters` runs each, HIP events) and keeps the fastest; reports the shape
  * through the out-pointers (the order through sl_batch_get_schedule). */
 int sl_batch_autotune(sl_batch *bt, int iters, int *rows_per_lane, int *waves_per_cu);

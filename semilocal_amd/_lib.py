@@ -230,7 +230,7 @@ class Batch:
         return r.value, w.value
 
     def set_schedule(self, schedule: int):
-        """Queue order of the batched launch: 0 pair by pair (largest first), 1 simulated."""
+        """Queue order of the batched launch: 0 pair by pair (largest first), 1-3 simulated (lag 128 s)."""
         check(lib().sl_batch_set_schedule(self.h, int(schedule)))
 
     def schedule(self) -> int:

@@ -212,7 +212,7 @@ struct Batch {
     std::vector<PairDesc> pairs;       // caller order
     std::vector<WorkItem> items;       // passes pair by pair (largest pairs first)
     std::vector<WorkItem> order;       // ticket order of the batched launch
-    int schedule = 0;                  // 0: `items` as they are, 1: simulated list schedule (schedule_items)
+    int schedule = 0;                  // 0: `items` as they are; s >= 1: simulated list schedule, lag 128 s (schedule_items)
     std::vector<int64_t> perm_off;     // caller order
     int64_t perm_total = 0;
     int64_t buf_total = 0;
@@ -262,21 +262,23 @@ static void batch_free(Batch *b) {
     delete b;
 }
 
+constexpr int kMaxSchedule = 3;
+
 // Queue order of the persistent launch.  Waves take tickets in order, and a
 // pass can only run some way behind its predecessor (it reads the strands
-// the previous pass leaves, ~128 steps behind), so handing out the ~200
-// passes of a large pair back to back would park hundreds of waves on
+// the previous pass leaves, at least 64 steps behind), so handing out the
+// ~200 passes of a large pair back to back would park hundreds of waves on
 // waits.  The order comes from a list-scheduling simulation instead: W
-// waves; a pass lasts n + 64 steps; pass p of a pair becomes ready 128 steps
-// after pass p-1 started; a free wave takes the ready pass of the largest
-// pair.  Passes of a pair keep their order, so a pass only ever waits on a
-// pass handed out before it.
-static void schedule_items(Batch *bt) {
+// waves; a pass lasts n + 64 steps; pass p of a pair becomes ready `lag`
+// steps after pass p-1 started; a free wave takes the ready pass of the
+// largest pair.  Passes of a pair keep their order, so a pass only ever
+// waits on a pass handed out before it.  A longer lag interleaves more pairs
+// (fewer waits, a longer tail); the autotuner picks it (128, 256 or 384).
+static void schedule_items(Batch *bt, double lag) {
     const size_t nitems = bt->items.size();
     bt->order.clear();
     bt->order.reserve(nitems);
     if (nitems == 0) return;
-    const double lag = 128.0;
     struct Ready {
         int64_t cells;
         int pair;
@@ -330,7 +332,7 @@ static void plan_items(Batch *bt) {
         for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
     }
     bt->waves_cap = (unsigned)device_cus() * bt->waves_per_cu;
-    if (bt->schedule == 1) schedule_items(bt);
+    if (bt->schedule >= 1) schedule_items(bt, 128.0 * bt->schedule);
     else bt->order = bt->items;
 }
 
@@ -791,8 +793,8 @@ int sl_batch_get_tiles(const sl_batch *h, int *rows_per_lane, int *waves_per_cu)
 
 int sl_batch_set_schedule(sl_batch *h, int schedule) {
     Batch *bt = reinterpret_cast<Batch *>(h);
-    if (!bt || (schedule != 0 && schedule != 1)) {
-        set_error("sl_batch_set_schedule: schedule must be 0 or 1");
+    if (!bt || schedule < 0 || schedule > kMaxSchedule) {
+        set_error("sl_batch_set_schedule: schedule must be 0, 1, 2 or 3");
         return kErrArg;
     }
     SL_HIP_TRY(hipStreamSynchronize(bt->stream));
@@ -821,7 +823,7 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
     const int wpc_cand[6] = {8, 12, 16, 20, 24, 32};
     double best = 1e30;
     int best_r = bt->rows_per_lane, best_w = (int)bt->waves_per_cu, best_s = bt->schedule;
-    for (int sched = 0; sched < 2; ++sched) {
+    for (int sched = 0; sched <= kMaxSchedule; ++sched) {
         bt->schedule = sched;
         for (int r : kTileHeights) {
             for (int w : wpc_cand) {

@@ -111,7 +111,7 @@ def test_autotune_and_tiles_do_not_change_results(gpu, oracle):
     assert b.tiles() == (r, w)
     for got, exp in zip(b.run().results(), want):
         assert np.array_equal(got, exp)
-    for sched in (0, 1):                      # queue order: pair by pair, or the simulated schedule
+    for sched in (0, 1, 2, 3):                # queue order: pair by pair, or the simulated schedules
         b.set_schedule(sched)
         assert b.schedule() == sched
         for r2, w2 in [(4, 8), (8, 12), (16, 16), (4, 32)]:
