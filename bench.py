@@ -220,7 +220,10 @@ def main():
     ap.add_argument("--per-pair-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--rows-per-lane", type=int, default=0)
+    ap.add_argument("--rows-per-lane", type=int, default=0,
+                    help="fixed tile shape instead of the autotuner (with --waves-per-cu, --schedule)")
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--schedule", type=int, default=0, help="queue order 0-3 (with --rows-per-lane)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
     ap.add_argument("--no-extras", action="store_true", help="only the ragged batch (configs 1, 2, 3, 5 skipped)")
@@ -255,7 +258,11 @@ def main():
     pairs = W.ragged_batch()
     log(f"workload generated in {time.perf_counter() - t0:.1f}s")
     batch = semilocal_amd.Batch(pairs, rows_per_lane=args.rows_per_lane)
-    if not args.rows_per_lane:
+    if args.rows_per_lane:
+        batch.set_schedule(args.schedule)
+        if args.waves_per_cu:
+            batch.set_tiles(args.rows_per_lane, args.waves_per_cu)
+    else:
         t0 = time.perf_counter()
         tiles = batch.autotune(iters=2)
         log(f"autotuner picked rows_per_lane={tiles[0]} waves_per_cu={tiles[1]} schedule={batch.schedule()} "
@@ -279,9 +286,18 @@ def main():
 
     ms, kms = batch.time(per_pair=False, warmup=args.warmup, iters=args.steps)
     log(f"batched: {ms:.3f} ms/step (kernel {kms:.3f} ms)")
+    pp_rows = None
     if args.per_pair_steps > 0:
-        pms, pkms = batch.time(per_pair=True, warmup=1, iters=args.per_pair_steps)
-        log(f"one launch per pair: {pms:.3f} ms/step (kernels {pkms:.3f} ms)")
+        # the plain form at its own best tile height (one pair's passes run in
+        # parallel, so it favours short passes), then back to the tuned shape
+        pms = float("inf")
+        for r in (4, 8, 16):
+            batch.set_tiles(r, tiles[1])
+            t_r, k_r = batch.time(per_pair=True, warmup=1, iters=args.per_pair_steps)
+            log(f"one launch per pair, rows_per_lane={r}: {t_r:.3f} ms/step (kernels {k_r:.3f} ms)")
+            if t_r < pms:
+                pms, pp_rows = t_r, r
+        batch.set_tiles(tiles[0], tiles[1])
     else:
         pms = float("nan")
     if args.copy_probe_mb:
@@ -322,8 +338,8 @@ def main():
                                "clipped to [100, 50000], alphabet 20, half mutated (10% subs, 2% indels), seed 4",
                    "pairs": st["pairs"], "cells": cells, "tiles": st["tiles"],
                    "rows_per_lane": tiles[0], "waves_per_cu": tiles[1], "queue_schedule": batch.schedule()},
-        "one_pair_per_launch": ({"value": round(gcups_pp, 2), "unit": "GCUPS", "ms_per_step": round(pms, 3)}
-                                if args.per_pair_steps > 0 else None),
+        "one_pair_per_launch": ({"value": round(gcups_pp, 2), "unit": "GCUPS", "ms_per_step": round(pms, 3),
+                                 "rows_per_lane": pp_rows} if args.per_pair_steps > 0 else None),
         "batched_vs_per_pair": round(gcups / gcups_pp, 2) if args.per_pair_steps > 0 else None,
         "parity": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 2),

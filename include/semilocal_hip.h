@@ -127,6 +127,15 @@ int sl_batch_get_schedule(const sl_batch *bt, int *schedule);
 int sl_batch_autotune(sl_batch *bt, int iters, int *rows_per_lane, int *waves_per_cu);
 /* Sum of m*n over the batch (cells), number of queue tiles, number of pairs. */
 int sl_batch_stats(const sl_batch *bt, int64_t *cells, int64_t *tiles, int64_t *pairs);
+/* Pairs (with m, n > 0) per symbol test of the combing cell at the batch's
+ * current tile height: symbol codes (4 VALU ops per cell), one-hot tags above
+ * the strand-id bits (3 ops; needs bits(m+n-1) + sigma <= 32), or a per-lane
+ * LDS table of row-match masks (3 ops and one LDS read per step; needs
+ * sigma <= 32 and bits(m+n-1) <= 32 - R).  A launch with fewer waves than
+ * CUs (each launch of sl_batch_run_per_pair, a small batch) gives table pairs
+ * the code cell instead: one LDS read fewer per step, lower latency when few
+ * waves share a CU.  Results never depend on it. */
+int sl_batch_cell_modes(const sl_batch *bt, int64_t *code_pairs, int64_t *onehot_pairs, int64_t *table_pairs);
 void sl_batch_destroy(sl_batch *bt);
 
 /* One-shot convenience: create + run + sync + download + destroy. */

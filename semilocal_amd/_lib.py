@@ -60,6 +60,7 @@ SIGNATURES = {
     "sl_batch_set_schedule": ([_vp, C.c_int], C.c_int),
     "sl_batch_get_schedule": ([_vp, C.POINTER(C.c_int)], C.c_int),
     "sl_batch_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "sl_batch_cell_modes": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "sl_batch_destroy": ([_vp], None),
     "sl_llcs_bitparallel": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "sl_bitpar_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int], C.c_int),
@@ -247,6 +248,12 @@ class Batch:
         c, t, p = C.c_int64(), C.c_int64(), C.c_int64()
         check(lib().sl_batch_stats(self.h, C.byref(c), C.byref(t), C.byref(p)))
         return {"cells": c.value, "tiles": t.value, "pairs": p.value}
+
+    def cell_modes(self):
+        """Pairs per symbol test of the combing cell at the current tile height."""
+        c, o, t = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().sl_batch_cell_modes(self.h, C.byref(c), C.byref(o), C.byref(t)))
+        return {"code": c.value, "onehot": o.value, "table": t.value}
 
     def close(self):
         if getattr(self, "h", None):

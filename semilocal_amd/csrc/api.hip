@@ -20,7 +20,7 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStrea
 hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
                              const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
                              int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch, LaunchCtl *ctl,
-                             uint32_t spin_limit);
+                             uint32_t spin_limit, uint32_t table_words);
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -140,7 +140,8 @@ static int pack_pool_from(const T *a_cat, const int64_t *a_off, const int32_t *a
 }
 
 // Rewrites each pair's symbols (rows and columns share one map) to dense codes
-// 0..sigma_pair-1 in first-seen order and picks the cell mode.
+// 0..sigma_pair-1 in first-seen order (only equality matters, so relabeling
+// keeps the kernel) and records sigma and the strand-id bits.
 static void remap_pairs_dense(std::vector<PairDesc> &pairs, PackedPool &pool) {
     std::vector<int32_t> table(pool.sym_bytes == 1 ? 256 : 65536, -1);
     std::vector<uint32_t> touched;
@@ -166,7 +167,8 @@ static void remap_pairs_dense(std::vector<PairDesc> &pairs, PackedPool &pool) {
         int bits = 0;
         while (bits < 32 && (max_id >> bits) != 0) ++bits;
         d.tag_shift = bits;
-        d.onehot = (bits + sigma <= 32) ? 1 : 0;
+        d.sigma = sigma;
+        d.cell = kCellCode;
     }
 }
 
@@ -231,6 +233,7 @@ struct Batch {
     uint32_t epoch = 0;                // run counter; part of the strand tags
     LaunchCtl *d_ctl = nullptr;    // [0] batched launch, [1+i] per-pair launch of pair i
     unsigned waves_cap = 0;
+    uint32_t table_words = 0;          // LDS words per wave of the row-match table (kCellTable pairs)
     size_t items_cap = 0;
     DeviceArena *arena = nullptr;      // null: hipMalloc / hipFree
     bool own_stream = true;
@@ -311,6 +314,26 @@ static void schedule_items(Batch *bt, double lag) {
     }
 }
 
+// Symbol test of each pair's cell for the batch's tile height R: one-hot
+// tags when bits(m+n-1) + sigma <= 32; else the per-lane row-match table when
+// sigma <= kTableMaxSymbols and the R mask bits fit above the ids
+// (bits(m+n-1) <= 32 - R); else symbol codes.  The table takes
+// (sigma + 1) x 64 LDS words per wave, sized for the batch's largest sigma.
+static void choose_cells(Batch *bt) {
+    int32_t table_sigma = 0;
+    for (auto &d : bt->pairs) {
+        if (d.tag_shift + d.sigma <= 32) {
+            d.cell = kCellOneHot;
+        } else if (d.sigma <= kTableMaxSymbols && d.tag_shift <= 32 - bt->rows_per_lane) {
+            d.cell = kCellTable;
+            table_sigma = std::max(table_sigma, d.sigma);
+        } else {
+            d.cell = kCellCode;
+        }
+    }
+    bt->table_words = table_sigma ? (uint32_t)(table_sigma + 1) * 64u : 0u;
+}
+
 // Splits every pair into passes of 64R rows.  `items` lists the passes pair
 // by pair (largest pairs first; the one-launch-per-pair form launches a
 // pair's consecutive run), `order` is the ticket order of the batched launch:
@@ -332,6 +355,7 @@ static void plan_items(Batch *bt) {
         for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
     }
     bt->waves_cap = (unsigned)device_cus() * bt->waves_per_cu;
+    choose_cells(bt);
     if (bt->schedule >= 1) schedule_items(bt, 128.0 * bt->schedule);
     else bt->order = bt->items;
 }
@@ -437,9 +461,6 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
         m_max = std::max(m_max, d.m);
         n_max = std::max(n_max, d.n);
     }
-    // per-pair dense symbol codes (only equality matters, so relabeling keeps the
-    // kernel) and the cell mode: one-hot tags above the strand-id bits when
-    // bits(m+n-1) + sigma_pair <= 32
     remap_pairs_dense(bt->pairs, pool);
     bt->rows_per_lane = rows_per_lane ? rows_per_lane : model_rows_per_lane(bt->cells, m_max, n_max, device_cus());
     if (const char *env = std::getenv("SEMILOCAL_WAVES_PER_CU")) {   // tuning override
@@ -512,12 +533,16 @@ static int batch_reset(Batch *bt) {
     return kOk;
 }
 
+// One persistent launch over `nitems` queue entries.  A launch with fewer
+// waves than CUs (one pair's passes, a small batch) is latency-bound: it gives
+// kCellTable pairs the code cell, whose steps carry one LDS read fewer.
 static hipError_t launch_items(Batch *bt, const WorkItem *items, uint32_t nitems, LaunchCtl *ctl) {
     const unsigned waves = std::min<unsigned>(nitems, bt->waves_cap);
     const void *pool = bt->d_pool;
+    const uint32_t table_words = waves >= (unsigned)device_cus() ? bt->table_words : 0u;
     // per-pair pointers are resolved inside the kernel from PairDesc offsets
     return launch_comb_batch(bt->sym_bytes, bt->rows_per_lane, waves, bt->stream, bt->d_pairs, items, nitems, pool,
-                             bt->d_rc, bt->d_cr, bt->d_buf, bt->epoch, ctl, bt->spin_limit);
+                             bt->d_rc, bt->d_cr, bt->d_buf, bt->epoch, ctl, bt->spin_limit, table_words);
 }
 
 static int batch_run(Batch *bt) {
@@ -858,6 +883,21 @@ int sl_batch_stats(const sl_batch *h, int64_t *cells, int64_t *tiles, int64_t *p
     if (cells) *cells = bt->cells;
     if (tiles) *tiles = (int64_t)bt->items.size();
     if (pairs) *pairs = bt->npairs;
+    return kOk;
+}
+
+int sl_batch_cell_modes(const sl_batch *h, int64_t *code_pairs, int64_t *onehot_pairs, int64_t *table_pairs) {
+    const Batch *bt = reinterpret_cast<const Batch *>(h);
+    if (!bt) {
+        set_error("null batch");
+        return kErrArg;
+    }
+    int64_t count[3] = {0, 0, 0};
+    for (const auto &d : bt->pairs)
+        if (d.m > 0 && d.n > 0) ++count[d.cell];
+    if (code_pairs) *code_pairs = count[kCellCode];
+    if (onehot_pairs) *onehot_pairs = count[kCellOneHot];
+    if (table_pairs) *table_pairs = count[kCellTable];
     return kOk;
 }
 

@@ -14,13 +14,14 @@
 //    LDS window filled 64 columns at a time (coalesced loads); every lane
 //    writes its outgoing strand into an LDS slot indexed by column, so the
 //    slot ends up holding lane 63's value, which is written back coalesced.
-//    Per step: R cells of 4 (one-hot symbols) or 5 VALU ops, one DPP move,
-//    two LDS reads and one LDS write.
+//    Per step: R cells of 3 VALU ops (one-hot symbol tags, or a per-lane
+//    table of row-match masks) or 4 (symbol codes), one DPP move, two LDS
+//    reads (three with the table) and one LDS write.
 //  * passes of one pair run concurrently on different waves: pass p+1 reads
 //    the bottom strands pass p wrote to the pair's strand buffer, in place,
 //    as tagged 64-bit entries (strand + writer tag) with device-coherent
-//    accesses.  Queue tickets are handed out in (pair, pass) order, so a pass
-//    only waits on a pass that is already running; every wait is bounded
+//    accesses.  The queue hands out the passes of a pair in pass order, so a
+//    pass only waits on a pass that is already running; every wait is bounded
 //    (spin limit) and a timeout makes every wave of the launch leave.
 //
 // Output: row->col (and optionally col->row) of the (m+n)-permutation,
@@ -85,7 +86,11 @@ __device__ __forceinline__ void report_timeout(LaunchCtl *ctl) {
 
 // LDS words of one wave's private staging area: column values of the
 // 128-column window read by the lanes, the 64 top strands injected at lane 0,
-// and the 128-slot window the lanes' outgoing strands are written into.
+// and the 128-slot window the lanes' outgoing strands are written into.  The
+// row-match table (kCellTable pairs) follows it: (sigma + 1) rows of 64 words,
+// word s*64 + k = mask of lane k's rows that hold symbol s; row sigma is zero
+// (absent columns).  Symbol-major rows make every table read conflict-free:
+// lane k always reads bank k.
 constexpr int kWinB = 0;      // [0, 128):   column value of column 64j-64+x
 constexpr int kInjT = 128;    // [128, 192): top strand entering column 64j+x
 constexpr int kOutT = 192;    // [192, 320): strand leaving lane k at step u -> slot 64+u-k
@@ -105,8 +110,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // single unsigned min gives the new left strand; since {l', t'} = {l, t} as a
 // multiset, t' = l ^ t ^ l', one v_bitop3_b32 (three-input XOR, truth table
 // 0x96) on gfx950.
-//  ONEHOT: A and B are one-hot symbol masks shifted above the id bits, so the
-//          tag is (A & B): v_and_or_b32, v_min_u32, v_bitop3_b32 (3 VALU).
+//  ONEHOT: A & B is zero, or a bit above the id bits iff the symbols match:
+//          one-hot symbol masks shifted above the ids (kCellOneHot), or the
+//          row's own mask bit and the column symbol's table word
+//          (kCellTable).  v_and_or_b32, v_min_u32, v_bitop3_b32 (3 VALU).
 //  else:   A and B are symbol codes; ((A ^ B) - 1) has bit 31 set iff they are
 //          equal: v_xad_u32, v_and_or_b32, v_min_u32, v_bitop3_b32 (4 VALU).
 template <bool ONEHOT>
@@ -119,15 +126,18 @@ __device__ __forceinline__ void comb_cell(uint32_t A, uint32_t B, uint32_t &l, u
     l = nl;
 }
 
-template <bool ONEHOT>
+// Column value in the LDS window: the one-hot tag, the code, or the byte
+// offset of the symbol's table row (64 words).
+template <int MODE>
 __device__ __forceinline__ uint32_t sym_value(uint32_t code, uint32_t tag_shift) {
-    return ONEHOT ? (1u << (tag_shift + code)) : code;
+    return MODE == kCellOneHot ? (1u << (tag_shift + code)) : MODE == kCellTable ? code * 256u : code;
 }
 
-template <int R, bool ONEHOT, typename Sym>
+template <int R, int MODE, typename Sym>
 __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ pool, int32_t *__restrict__ rc,
                           int32_t *__restrict__ cr, uint64_t *__restrict__ buf, uint32_t epoch,
-                          LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W) {
+                          LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W, uint32_t *T) {
+    constexpr bool ONEHOT = MODE != kCellCode;
     const int lane = threadIdx.x & 63;
     const uint32_t m = (uint32_t)d.m, n = (uint32_t)d.n, size = m + n;
     const uint32_t tag_shift = (uint32_t)d.tag_shift;
@@ -143,8 +153,8 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
     const uint64_t my_tag = (uint64_t)strand_tag(epoch, (uint32_t)pass) << 32;
     const int64_t row0 = (int64_t)pass * 64 * R + (int64_t)lane * R;
     // values of padding rows / absent columns: never equal to anything
-    const uint32_t pad_row = ONEHOT ? 0u : 0x10000u;
-    const uint32_t pad_col = ONEHOT ? 0u : 0x20000u;
+    const uint32_t pad_row = MODE == kCellCode ? 0x10000u : 0u;
+    const uint32_t pad_col = MODE == kCellOneHot ? 0u : MODE == kCellTable ? (uint32_t)d.sigma * 256u : 0x20000u;
 
     // Left strand of grid row i is m-1-i (semi_local.h:70-82, 113-119).
     uint32_t l[R], A[R];
@@ -153,7 +163,22 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
         const int64_t i = row0 + q;
         const bool ok = i < (int64_t)m;
         l[q] = ok ? (uint32_t)(m - 1 - i) : 0u;
-        A[q] = ok ? sym_value<ONEHOT>((uint32_t)rows[i], tag_shift) : pad_row;
+        A[q] = ok ? (MODE == kCellTable ? (uint32_t)rows[i] : sym_value<MODE>((uint32_t)rows[i], tag_shift)) : pad_row;
+    }
+    if (MODE == kCellTable) {
+        // lane k's table words: bit 32-R+q set iff row q holds the symbol
+        // (ids fit below bit 32-R); row sigma stays zero for absent columns
+        const uint32_t sigma = (uint32_t)d.sigma;
+        for (uint32_t sym = 0; sym <= sigma; ++sym) {
+            uint32_t mask = 0;
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                if (row0 + q < (int64_t)m && A[q] == sym) mask |= 1u << (32 - R + q);
+            T[sym * 64u + (uint32_t)lane] = mask;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) A[q] = 1u << (32 - R + q);
+        wave_lds_sync();
     }
 
     const uint32_t nchunks = (n + 63u + 63u) / 64u;   // steps 0 .. n+62 inclusive
@@ -163,7 +188,7 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
         const uint32_t col = 64u * j + (uint32_t)lane;
         if (col < n) {
             tv = first ? (uint64_t)(m + col) : ld_coherent64(&sbuf[col]);   // top strand ids m+j (semi_local.h:77-80)
-            bin = sym_value<ONEHOT>((uint32_t)cols[col], tag_shift);
+            bin = sym_value<MODE>((uint32_t)cols[col], tag_shift);
         } else {
             tv = kNoStrand;
             bin = pad_col;
@@ -204,17 +229,50 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
         wave_lds_sync();
         const uint32_t *wb = W + kWinB + 64 - lane;   // wb[u]: column 64j+u-lane
         uint32_t *wo = W + kOutT + 64 - lane;         // wo[u]: slot of column 64j+u-lane
+        const char *tl = (const char *)(T + lane);    // this lane's table column
+        // Column values (row-match masks for kCellTable) and injected strands
+        // of a block of 8 steps are in registers before the block starts: the
+        // next block's reads are issued during the current one, so no LDS
+        // round trip sits on the step chain.  (The compiler cannot move them
+        // itself: it cannot tell these reads from the wo[] writes.)
+        auto lookup = [&](uint32_t (&v)[8]) {
+            if (MODE == kCellTable) {
+#pragma unroll
+                for (int uu = 0; uu < 8; ++uu) v[uu] = *(const uint32_t *)(tl + v[uu]);   // row-match mask
+            }
+        };
+        // One block of 8 steps from (bv, inj); reads the block at `un` into
+        // (bv_nx, inj_nx) on the way.
+        auto block = [&](int u0, int un, const uint32_t (&bv)[8], const uint32_t (&inj)[8], uint32_t (&bv_nx)[8],
+                         uint32_t (&inj_nx)[8]) {
+#pragma unroll
+            for (int uu = 0; uu < 8; ++uu) {
+                bv_nx[uu] = wb[un + uu];
+                inj_nx[uu] = W[kInjT + un + uu];
+            }
+#pragma unroll
+            for (int uu = 0; uu < 8; ++uu) {
+                if (uu == 4) lookup(bv_nx);
+                t = dpp_shr1(inj[uu], t);   // lane 0 takes column 64j+u, lane k>0 lane k-1's strand
+#pragma unroll
+                for (int q = 0; q < R; ++q) comb_cell<ONEHOT>(A[q], bv[uu], l[q], t);
+                wo[u0 + uu] = t;
+            }
+        };
+        // Steps [u_begin, u_end), a multiple of 16, two blocks per iteration
+        // with the register sets swapped (no copies between blocks).
         auto steps = [&](int u_begin, int u_end) {
-            for (int u0 = u_begin; u0 < u_end; u0 += 8) {
+            uint32_t b0[8], i0[8], b1[8], i1[8];
 #pragma unroll
-                for (int uu = 0; uu < 8; ++uu) {
-                    const int u = u0 + uu;
-                    const uint32_t bv = wb[u];
-                    t = dpp_shr1(W[kInjT + u], t);   // lane 0 takes column 64j+u, lane k>0 lane k-1's strand
-#pragma unroll
-                    for (int q = 0; q < R; ++q) comb_cell<ONEHOT>(A[q], bv, l[q], t);
-                    wo[u] = t;
-                }
+            for (int uu = 0; uu < 8; ++uu) {
+                b0[uu] = wb[u_begin + uu];
+                i0[uu] = W[kInjT + u_begin + uu];
+            }
+            lookup(b0);
+            for (int u0 = u_begin; u0 < u_end; u0 += 16) {
+                block(u0, u0 + 8, b0, i0, b1, i1);
+                const int un = u0 + 16 < u_end ? u0 + 16 : u0;   // last pair of blocks: re-reads valid slots
+                block(u0 + 8, un, b1, i1, b0, i0);
             }
         };
         steps(0, 32);
@@ -255,10 +313,12 @@ __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restr
                                                          const WorkItem *__restrict__ items, uint32_t nitems,
                                                          const Sym *__restrict__ pool, int32_t *__restrict__ rc,
                                                          int32_t *__restrict__ cr, uint64_t *__restrict__ buf,
-                                                         uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit) {
-    __shared__ uint32_t lds[kWavesPerBlock * kWaveLds];
+                                                         uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit,
+                                                         uint32_t table_words) {
+    extern __shared__ uint32_t lds[];   // kWavesPerBlock x (kWaveLds + table_words) words
     const int lane = threadIdx.x & 63;
-    uint32_t *W = lds + (threadIdx.x >> 6) * kWaveLds;   // this wave's private staging area
+    uint32_t *W = lds + (threadIdx.x >> 6) * (kWaveLds + table_words);   // this wave's private staging area
+    uint32_t *T = W + kWaveLds;                                          // its row-match table
     for (;;) {
         uint32_t tk = 0;
         if (lane == 0) tk = atomicAdd(&ctl->ticket, 1u);
@@ -266,10 +326,12 @@ __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restr
         if (tk >= nitems) break;     // every wave leaves once the queue is empty
         const WorkItem it = items[tk];
         const PairDesc d = pairs[it.pair];
-        if (d.onehot)
-            comb_pass<R, true, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W);
+        if (d.cell == kCellOneHot)
+            comb_pass<R, kCellOneHot, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+        else if (d.cell == kCellTable && table_words != 0)
+            comb_pass<R, kCellTable, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
         else
-            comb_pass<R, false, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W);
+            comb_pass<R, kCellCode, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
     }
 }
 
@@ -289,18 +351,24 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStrea
 template <typename Sym, int R>
 static void launch_one(unsigned blocks, hipStream_t stream, const PairDesc *pairs, const WorkItem *items,
                        uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch,
-                       LaunchCtl *ctl, uint32_t spin_limit) {
-    hipLaunchKernelGGL((comb_batch_kernel<Sym, R>), dim3(blocks), dim3(kWavesPerBlock * 64), 0, stream, pairs, items,
-                       nitems, (const Sym *)pool, rc, cr, buf, epoch, ctl, spin_limit);
+                       LaunchCtl *ctl, uint32_t spin_limit, uint32_t table_words) {
+    const size_t lds_bytes = sizeof(uint32_t) * kWavesPerBlock * (kWaveLds + table_words);
+    hipLaunchKernelGGL((comb_batch_kernel<Sym, R>), dim3(blocks), dim3(kWavesPerBlock * 64), lds_bytes, stream, pairs,
+                       items, nitems, (const Sym *)pool, rc, cr, buf, epoch, ctl, spin_limit, table_words);
 }
 
+// table_words: (largest sigma of the batch's kCellTable pairs + 1) x 64, or 0:
+// then kCellTable pairs take the code cell (one fewer LDS read per step, the
+// lower-latency choice for launches with few waves).
 hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
                              const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
                              int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch, LaunchCtl *ctl,
-                             uint32_t spin_limit) {
+                             uint32_t spin_limit, uint32_t table_words) {
     if (nitems == 0 || waves == 0) return hipSuccess;
+    if (table_words > (uint32_t)(kTableMaxSymbols + 1) * 64u) return hipErrorInvalidValue;
     const unsigned blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
-#define SL_LAUNCH(SYM, RR) launch_one<SYM, RR>(blocks, stream, pairs, items, nitems, pool, rc, cr, buf, epoch, ctl, spin_limit)
+#define SL_LAUNCH(SYM, RR) \
+    launch_one<SYM, RR>(blocks, stream, pairs, items, nitems, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words)
     if (sym_bytes == 1) {
         if (rows_per_lane == 4) SL_LAUNCH(uint8_t, 4);
         else if (rows_per_lane == 8) SL_LAUNCH(uint8_t, 8);

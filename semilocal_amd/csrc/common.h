@@ -44,9 +44,19 @@ struct PairDesc {
     int32_t item0;      // queue index of pass 0 of this pair
     int32_t npasses;    // ceil(m / rows_per_pass)
     int32_t rows_per_lane;  // R: rows held by one lane (rows_per_pass = 64 R)
-    int32_t onehot;     // 1: symbols are tested as one-hot tags above the strand-id bits
-    int32_t tag_shift;  // bit position of symbol code 0's tag (= bits of the largest strand id)
+    int32_t cell;       // how the cell tests symbols: kCellCode, kCellOneHot or kCellTable
+    int32_t tag_shift;  // bits of the largest strand id (one-hot tags start here)
+    int32_t sigma;      // distinct symbols of the pair (dense codes 0..sigma-1)
+    int32_t pad_;
 };
+
+// Symbol test of the combing cell, chosen per pair (and per tile height R).
+constexpr int32_t kCellCode = 0;     // codes compared in the cell: 4 VALU ops
+constexpr int32_t kCellOneHot = 1;   // one-hot tags above the id bits (bits + sigma <= 32): 3 ops
+constexpr int32_t kCellTable = 2;    // per-lane LDS table of row-match masks indexed by the
+                                     // column symbol (sigma <= kTableMaxSymbols, bits <= 32 - R):
+                                     // 3 ops + one LDS read per step
+constexpr int32_t kTableMaxSymbols = 32;
 
 // Queue entry: (pair, pass).  Entries of one pair are consecutive and in pass
 // order, so a pass only ever waits on an entry handed out before it.

@@ -127,3 +127,60 @@ def test_large_alphabet_pair_uses_code_mode(gpu, oracle):
     a = g.integers(0, 40, 20000).astype(np.uint8)
     b = g.integers(0, 40, 40000).astype(np.uint8)
     assert np.array_equal(gpu.semi_local_comb(a, b), oracle.comb(a, b))
+
+
+def test_cell_modes_every_tile_height(gpu, oracle):
+    # One batch that takes all three symbol tests of the cell: one-hot tags
+    # (bits(m+n-1) + sigma <= 32), the per-lane row-match table (sigma <= 32,
+    # bits(m+n-1) <= 32 - R) and symbol codes.  The 24000 x 42000 pair has
+    # 17 id bits, so it uses the table at R = 4 and 8 but codes at R = 16.
+    g = np.random.default_rng(17)
+
+    def pair(sigma, m, n):
+        x = g.integers(0, sigma, m).astype(np.uint8)
+        y = g.integers(0, sigma, n).astype(np.uint8)
+        return x, y
+
+    pairs = [pair(20, 3000, 7000),     # table
+             pair(32, 2500, 6000),     # table, the largest alphabet it takes
+             pair(33, 2000, 5000),     # codes
+             pair(20, 24000, 42000),   # table at R = 4, 8; codes at R = 16
+             pair(4, 3000, 3000),      # one-hot
+             pair(20, 200, 300),       # one-hot (9 id bits + 20 tags)
+             pair(20, 1, 5000),        # table, a single row
+             pair(20, 0, 700)]         # empty
+    y = pairs[3][1]
+    pairs[3] = (pairs[3][0], np.concatenate([y[:20000], pairs[3][0][:12000], y[32000:]]))   # similar stretch
+    # the table is used by launches with at least one wave per CU: 300 more table pairs
+    pairs += [pair(20, 300, 4000) for _ in range(300)]
+    want = [oracle.comb(x, y, threads=8) if len(x) and len(y) else np.arange(len(x) + len(y), dtype=np.int32)
+            for x, y in pairs]
+    expect_modes = {4: {"code": 1, "onehot": 2, "table": 304}, 8: {"code": 1, "onehot": 2, "table": 304},
+                    16: {"code": 2, "onehot": 2, "table": 303}}
+    b = gpu.Batch(pairs, rows_per_lane=4)
+    for R in (4, 8, 16):
+        for w in (8, 24):
+            b.set_tiles(R, w)
+            assert b.cell_modes() == expect_modes[R], R
+            for sched in (0, 2):
+                b.set_schedule(sched)
+                for k, (got, exp) in enumerate(zip(b.run().results(), want)):
+                    assert np.array_equal(got, exp), (R, w, sched, k)
+        for k, (got, exp) in enumerate(zip(b.run(per_pair=True).results(), want)):
+            assert np.array_equal(got, exp), (R, "per pair", k)
+    b.close()
+
+
+def test_cell_table_with_16bit_symbols(gpu, oracle):
+    # 25 distinct symbols spread over a 16-bit range: dense codes 0..24, table mode
+    g = np.random.default_rng(18)
+    alphabet = g.choice(np.arange(300, 60000), 25, replace=False).astype(np.int32)
+    pairs = [(alphabet[g.integers(0, 25, m)], alphabet[g.integers(0, 25, n)])
+             for m, n in [(4000, 9000), (700, 12000), (5000, 5000)] + [(300, 4500)] * 300]
+    b = gpu.Batch(pairs, rows_per_lane=8)
+    assert b.cell_modes()["table"] == 303
+    want = [oracle.comb(x, y, threads=8) for x, y in pairs]
+    for per_pair in (False, True):
+        for k, (got, exp) in enumerate(zip(b.run(per_pair=per_pair).results(), want)):
+            assert np.array_equal(got, exp), (per_pair, k)
+    b.close()
