@@ -147,6 +147,9 @@ def bench_hybrid(S, W, golden):
     steady-ant multiplications."""
     a, b = W.similar_pair()
     h = S.Hybrid(a, b, 8, 8)
+    t0 = time.perf_counter()
+    tiles = h.autotune(iters=1)            # the combing launch's tile shape and queue order
+    tune_s = time.perf_counter() - t0
     perm = h.run()
     ok = W.sha256(perm) == golden["similar_pair_1e6"]["perm_sha256"]
     tot, comb, merge = h.time(warmup=1, iters=3)
@@ -159,6 +162,7 @@ def bench_hybrid(S, W, golden):
                         "8x8 blocks", "ms": round(tot, 3), "comb_ms": round(comb, 3), "merge_ms": round(merge, 3),
             "graph_ms": round(gtot, 3), "merge_levels": levels, "gcups": round(cells / (tot * 1e-3) / 1e9, 2),
             "graph_gcups": round(cells / (gtot * 1e-3) / 1e9, 2),
+            "comb_tiles": {"rows_per_lane": tiles[0], "waves_per_cu": tiles[1], "autotune_s": round(tune_s, 1)},
             "parity": "bit-exact vs the reference hybrid's fixture (eager and graph)" if ok else "MISMATCH"}
 
 
@@ -317,8 +321,8 @@ def main():
     traffic = None
     tf = os.path.join(ROOT, "profiles", "comb_batch_kernel_ragged4096.traffic.json")
     if os.path.exists(tf):
-        rec = json.load(open(tf))
-        if rec.get("kernel") == f"comb_batch_kernel<uint8_t,{tiles[0]}>":
+        rec = json.load(open(tf)).get("kernels", {}).get(f"comb_batch_kernel<uint8_t,{tiles[0]}>")
+        if rec:
             traffic = rec["traffic_bytes_per_launch"]
     achieved_tops = cells * VALU_OPS_PER_CELL / (kms * 1e-3) / 1e12
     out = {

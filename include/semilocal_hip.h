@@ -109,19 +109,20 @@ int sl_batch_download(sl_batch *bt, int32_t *row_to_col_cat, int32_t *col_to_row
 int sl_batch_time(sl_batch *bt, int mode, int warmup, int iters, double *ms_per_run,
                   double *kernel_ms_out);
 /* Tile shape of the batch: rows per lane R (4, 8 or 16; one queue tile = one
- * pass of 64R rows of one pair) and persistent waves per compute unit.
- * Results never depend on it (the semi-local kernel is unique). */
+ * pass of 64R rows of one pair) and persistent waves per compute unit
+ * (default 8; R defaults to a cost model's choice).  Results never depend on
+ * it (the semi-local kernel is unique). */
 int sl_batch_set_tiles(sl_batch *bt, int rows_per_lane, int waves_per_cu);
 int sl_batch_get_tiles(const sl_batch *bt, int *rows_per_lane, int *waves_per_cu);
-/* Queue order of the batched launch: 0 = pair by pair, largest pairs first
- * (the default); s = 1, 2, 3 = a list-scheduling simulation that hands out a
- * pair's passes about when each can start behind the previous one, assuming
- * a pass starts 128 s steps after its predecessor.  Results never depend on
+/* Queue order of the batched launch: 0 = pair by pair, largest pairs first;
+ * s = 1, 2, 3 = a list-scheduling simulation that hands out a pair's passes
+ * about when each can start behind the previous one, assuming a pass starts
+ * 128 s steps after its predecessor (default 2).  Results never depend on
  * it. */
 int sl_batch_set_schedule(sl_batch *bt, int schedule);
 int sl_batch_get_schedule(const sl_batch *bt, int *schedule);
 /* Autotuner: times every tile shape -- rows per lane {4, 8, 16} x persistent
- * waves per CU {8, 12, 16, 20, 24, 32} x queue order {0..3} -- on this batch
+ * waves per CU {4, 8, 12, 16, 20, 24, 32} x queue order {0..3} -- on this batch
  * (`iters` runs each, HIP events) and keeps the fastest; reports the shape
  * through the out-pointers (the order through sl_batch_get_schedule). */
 int sl_batch_autotune(sl_batch *bt, int iters, int *rows_per_lane, int *waves_per_cu);
@@ -234,6 +235,10 @@ int sl_hybrid_stats(const sl_hybrid *h, int64_t *cells, int *merge_levels);
  * replays it with one graph launch.  Results are unchanged.  In graph mode
  * sl_hybrid_time reports the total only (comb_ms and merge_ms are -1). */
 int sl_hybrid_set_graph(sl_hybrid *h, int enable);
+/* Runs sl_batch_autotune on the hybrid's combing launch (its blocks as one
+ * ragged batch) and keeps the fastest shape; a captured graph is dropped and
+ * captured again on the next graph run.  Results never depend on it. */
+int sl_hybrid_autotune(sl_hybrid *h, int iters, int *rows_per_lane, int *waves_per_cu);
 void sl_hybrid_destroy(sl_hybrid *h);
 
 /* ------------------------------------------------------------------------ */

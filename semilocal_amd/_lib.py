@@ -83,6 +83,7 @@ SIGNATURES = {
     "sl_hybrid_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)], C.c_int),
     "sl_hybrid_destroy": ([_vp], None),
     "sl_hybrid_set_graph": ([_vp, C.c_int], C.c_int),
+    "sl_hybrid_autotune": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "sl_server_create": ([C.POINTER(_vp), C.c_int64, C.c_int, C.c_int, C.c_int64], C.c_int),
     "sl_server_submit_compare": ([_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)],
                                  C.c_int),
@@ -435,6 +436,12 @@ class Hybrid:
     def set_graph(self, enable: bool = True):
         """Capture the whole run into a HIP graph and replay it (same results)."""
         check(lib().sl_hybrid_set_graph(self.h, 1 if enable else 0))
+
+    def autotune(self, iters: int = 1):
+        """Tune the combing launch's tile shape and queue order (same results)."""
+        r, w = C.c_int(), C.c_int()
+        check(lib().sl_hybrid_autotune(self.h, iters, C.byref(r), C.byref(w)))
+        return r.value, w.value
 
     def stats(self):
         cells, levels = C.c_int64(), C.c_int()

@@ -179,20 +179,29 @@ static void remap_pairs_dense(std::vector<PairDesc> &pairs, PackedPool &pool) {
 // accesses per step are shared by R cells) but lengthens the critical path of
 // the largest pair (fewer, longer passes).  The default minimises
 // max(throughput time, critical path) with per-R rates measured on MI355X;
-// sl_batch_autotune replaces it with a measurement on the batch itself.
+// the critical path's step time counts the waves that share a SIMD (the
+// batch's passes per SIMD, 1 to 3), so one large pair with a single wave per
+// SIMD gets tall tiles.  sl_batch_autotune replaces the model with a
+// measurement on the batch itself.
 // ---------------------------------------------------------------------------
 static const int kTileHeights[3] = {4, 8, 16};
 
-int model_rows_per_lane(int64_t total_cells, int32_t m_max, int32_t n_max, int cus) {
+int model_rows_per_lane(const std::vector<PairDesc> &pairs, int64_t total_cells, int32_t m_max, int32_t n_max,
+                        int cus) {
     const double rate[3] = {6.2e12, 7.2e12, 8.0e12};   // single-pass cells/s on 256 CUs
     const double scale = cus > 0 ? cus / 256.0 : 1.0;
+    const double simds = 4.0 * (cus > 0 ? cus : 256);
     int best = 16;
     double best_t = 1e30;
     for (int k = 2; k >= 0; --k) {
         const int R = kTileHeights[k];
+        double total_passes = 0.0;
+        for (const auto &d : pairs)
+            if (d.m > 0 && d.n > 0) total_passes += std::ceil(d.m / (64.0 * R));
+        const double sharing = std::min(3.0, std::max(1.0, total_passes / simds));
         const double passes = std::ceil(m_max / (64.0 * R));
         const double steps = n_max + passes * 160.0;
-        const double clk_per_step = (R * 4.1 + 2.0) * 4.0 * 3.0;   // SIMD shared by ~3 resident waves
+        const double clk_per_step = (R * 4.1 + 2.0) * 4.0 * sharing;
         const double t_crit = steps * clk_per_step / 2.4e9;
         const double t_thru = (double)total_cells / (rate[k] * scale);
         const double t = std::max(t_crit, t_thru);
@@ -214,14 +223,14 @@ struct Batch {
     std::vector<PairDesc> pairs;       // caller order
     std::vector<WorkItem> items;       // passes pair by pair (largest pairs first)
     std::vector<WorkItem> order;       // ticket order of the batched launch
-    int schedule = 0;                  // 0: `items` as they are; s >= 1: simulated list schedule, lag 128 s (schedule_items)
+    int schedule = 2;                  // 0: `items` as they are; s >= 1: simulated list schedule, lag 128 s (schedule_items)
     std::vector<int64_t> perm_off;     // caller order
     int64_t perm_total = 0;
     int64_t buf_total = 0;
     int64_t cells = 0;
     uint32_t spin_limit = 1u << 23;
     int rows_per_lane = 16;            // tile height of every pass of this batch
-    unsigned waves_per_cu = 24;
+    unsigned waves_per_cu = 8;         // with schedule 2: the best default shape measured on MI355X
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     PairDesc *d_pairs = nullptr;
@@ -462,11 +471,8 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
         n_max = std::max(n_max, d.n);
     }
     remap_pairs_dense(bt->pairs, pool);
-    bt->rows_per_lane = rows_per_lane ? rows_per_lane : model_rows_per_lane(bt->cells, m_max, n_max, device_cus());
-    if (const char *env = std::getenv("SEMILOCAL_WAVES_PER_CU")) {   // tuning override
-        const long v = std::strtol(env, nullptr, 10);
-        if (v > 0 && v <= 64) bt->waves_per_cu = (unsigned)v;
-    }
+    bt->rows_per_lane =
+        rows_per_lane ? rows_per_lane : model_rows_per_lane(bt->pairs, bt->cells, m_max, n_max, device_cus());
     plan_items(bt);
 
     hipError_t e = hipSuccess;
@@ -845,7 +851,7 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
         set_error("sl_batch_autotune: bad argument");
         return kErrArg;
     }
-    const int wpc_cand[6] = {8, 12, 16, 20, 24, 32};
+    const int wpc_cand[] = {4, 8, 12, 16, 20, 24, 32};
     double best = 1e30;
     int best_r = bt->rows_per_lane, best_w = (int)bt->waves_per_cu, best_s = bt->schedule;
     for (int sched = 0; sched <= kMaxSchedule; ++sched) {
@@ -855,7 +861,7 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
                 int rc = sl_batch_set_tiles(h, r, w);
                 if (rc) return rc;
                 double ms = 0.0;
-                rc = sl_batch_time(h, 0, 1, iters, &ms, nullptr);
+                rc = sl_batch_time(h, 0, 0, iters, &ms, nullptr);   // each run is a whole launch: no warm-up
                 if (rc) return rc;
                 if (ms < best) {
                     best = ms;

@@ -955,6 +955,21 @@ int sl_hybrid_set_graph(sl_hybrid *hh, int enable) {
     return kOk;
 }
 
+int sl_hybrid_autotune(sl_hybrid *hh, int iters, int *rows_per_lane, int *waves_per_cu) {
+    Hybrid *h = reinterpret_cast<Hybrid *>(hh);
+    if (!h || iters <= 0) {
+        set_error("sl_hybrid_autotune: bad argument");
+        return kErrArg;
+    }
+    SL_HIP_TRY(hipStreamSynchronize(batch_stream_internal(h->batch)));
+    // a new shape can move the queue's device buffers: drop the captured graph
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    return sl_batch_autotune(reinterpret_cast<sl_batch *>(h->batch), iters, rows_per_lane, waves_per_cu);
+}
+
 void sl_hybrid_destroy(sl_hybrid *hh) { hybrid_free(reinterpret_cast<Hybrid *>(hh)); }
 
 int sl_hybrid_iterative(const void *a, int m, const void *b, int n, int sym_bytes, int small_m, int small_n,

@@ -169,3 +169,23 @@ def test_hybrid_graph_replays_equal_the_eager_run(gpu):
     h.set_graph(False)                                        # back to eager launches on the same handle
     assert np.array_equal(h.run(), eager)
     h.close()
+
+
+def test_hybrid_autotune_keeps_results_and_recaptures_the_graph(gpu):
+    # tuning the combing launch changes tile shape and queue order (and can
+    # move the queue's device buffers): eager runs and a graph captured before
+    # the tuning must both still give the eager result
+    g = _rng(41)
+    a = g.integers(0, 4, size=20000).astype(np.uint8)
+    b = g.integers(0, 4, size=17000).astype(np.uint8)
+    want = gpu.semi_local_comb(a, b)
+    h = gpu.Hybrid(a, b, 4, 2)
+    h.set_graph(True)
+    assert np.array_equal(h.run(), want)                      # graph captured at the default shape
+    r, w = h.autotune(iters=1)
+    assert r in (4, 8, 16) and 1 <= w <= 64
+    for _ in range(2):
+        assert np.array_equal(h.run(), want)                  # captured again at the tuned shape
+    h.set_graph(False)
+    assert np.array_equal(h.run(), want)
+    h.close()
