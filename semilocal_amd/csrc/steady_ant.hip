@@ -212,34 +212,95 @@ __global__ __launch_bounds__(kTile) void split_scatter(const int32_t *__restrict
 // memory round trip; the point of the current row and of column c-1 are
 // carried in registers.  rowv must accept r in [r_min, r] and colv c in
 // [c_start - 1, min(c_start + steps, s) - 1] (c_start >= 1 or s >= 1).
+//
+// One step from (r, c): ur / left are the points of row r / column c-1, up /
+// nxt those of row r-1 / column c (clamped reads where out of range; such a
+// value is never used).  Tests (r, c-1) for a good point, then moves right
+// (delta < 0) or up, updating delta by the moved-past point.  Returns false
+// where the walk leaves the square.  Branch-free apart from the store of a
+// good point and the exit: lanes walking different segments take different
+// directions, and divergent if/else arms cost every lane both arms plus the
+// exec-mask bookkeeping.
+template <class PutF>
+__device__ __forceinline__ bool ant_step(int &r, int &c, int &d, int s, uint32_t &ur, uint32_t &left, uint32_t up,
+                                         uint32_t nxt, bool &right, PutF put) {
+    {
+        const int wcol = ((int)(left & kValMask) >= r) ^ (int)(left >> 31);   // hi: row < r; lo: row >= r
+        const int wrow = ((int)(ur & kValMask) >= c) ^ (int)(ur >> 31);       // hi: col < c; lo: col >= c
+        if (c > 0 && d - wcol < 0 && d + wrow > 0) put(r, c - 1);
+    }
+    // written with masks, not selects, so the compiler keeps it straight-line
+    const int rt = (int)((uint32_t)d >> 31);                 // 1: move right (delta < 0), 0: up
+    const uint32_t rm = 0u - (uint32_t)rt;                   // all ones when moving right
+    const int stop = (rt & (c >= s)) | ((rt ^ 1) & (r == 0));
+    right = rt != 0;
+    if (stop) return false;
+    const uint32_t v = (nxt & rm) | (up & ~rm);              // the point the move passes
+    const int thr = c + ((r - c) & (int)rm);                 // right: compare its row with r; up: its column with c
+    const int dd = ((int)(v & kValMask) >= thr) ^ (int)(v >> 31);
+    const int up1 = rt ^ 1;
+    d += (dd ^ -up1) + up1;                                  // right: +dd, up: -dd
+    left = (v & rm) | (left & ~rm);
+    ur = (ur & rm) | (v & ~rm);
+    r -= up1;
+    c += rt;
+    return true;
+}
+
 template <class RowF, class ColF, class PutF>
 __device__ __forceinline__ void ant_walk(int r, int c, int d, int s, int steps, RowF rowv, ColF colv, PutF put) {
     uint32_t ur = rowv(r);
     uint32_t left = c > 0 ? colv(c - 1) : 0u;
     for (int t = 0; t < steps; ++t) {
-        // unconditional (clamped) reads, so both are in flight together; the
-        // clamped values are never used
+        // unconditional (clamped) reads, so both are in flight together
         const uint32_t up = rowv(r > 0 ? r - 1 : r);
         const uint32_t nxt = colv(c < s ? c : c - 1);
-        if (c > 0) {
-            const int urow = (int)(ur & kValMask), pr = (int)(left & kValMask);
-            const int wcol = (left & kHiBit) ? (pr < r) : (pr >= r);
-            const int wrow = (ur & kHiBit) ? (urow < c) : (urow >= c);
-            if (d - wcol < 0 && d + wrow > 0) put(r, c - 1);
+        bool right;
+        if (!ant_step(r, c, d, s, ur, left, up, nxt, right, put)) break;
+    }
+}
+
+// The same walk with its reads taken K steps at a time.  A walk moves one
+// row up or one column right per step, so its next K steps read only rows
+// r-1 .. r-K and columns c .. c+K-1: the lane reads that window in one batch
+// of 2K independent loads into its own LDS slots (lw[k * 64], interleaved with
+// the other lanes of its wave, so every access is conflict-free), then takes
+// up to K steps from LDS.  One memory round trip per K steps instead of one
+// per step; the values are the same as ant_walk's, so the walk is too.
+// Out-of-range window entries are clamped reads the walk never uses.
+template <int K, class RowF, class ColF, class PutF>
+__device__ __forceinline__ void ant_walk_window(int r, int c, int d, int s, int steps, RowF rowv, ColF colv,
+                                                PutF put, uint32_t *lw) {
+    uint32_t ur = rowv(r);
+    uint32_t left = c > 0 ? colv(c - 1) : 0u;
+    int t = 0;
+    while (t < steps) {
+        uint32_t wr[K], wc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            wr[k] = rowv(r - 1 - k >= 0 ? r - 1 - k : 0);
+            wc[k] = colv(c + k < s ? c + k : s - 1);
         }
-        if (d < 0) {
-            if (c >= s) break;
-            const int pr = (int)(nxt & kValMask);
-            d += (nxt & kHiBit) ? (pr < r) : (pr >= r);
-            left = nxt;
-            ++c;
-        } else {
-            if (r == 0) break;
-            --r;
-            const int u2 = (int)(up & kValMask);
-            d -= (up & kHiBit) ? (u2 < c) : (u2 >= c);
-            ur = up;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            lw[k * 64] = wr[k];
+            lw[(K + k) * 64] = wc[k];
         }
+        const int t_stop = t + K < steps ? t + K : steps;
+        int iu = 0, ic = 0;
+        bool done = false;
+        for (; t < t_stop; ++t) {
+            const uint32_t up = lw[iu * 64];         // row r - 1
+            const uint32_t nxt = lw[(K + ic) * 64];  // column c
+            bool right;
+            if (!ant_step(r, c, d, s, ur, left, up, nxt, right, put)) {
+                done = true;
+                break;
+            }
+            ic += right ? 1 : 0;
+            iu += right ? 0 : 1;
+        }
+        if (done) break;
     }
 }
 
@@ -362,7 +423,7 @@ __device__ __forceinline__ int wave_delta_lds(const uint16_t *U16, int s, int r,
     return __builtin_amdgcn_readfirstlane(wave_sum(acc));
 }
 
-__global__ __launch_bounds__(kTile) void solve_small(const int32_t *__restrict__ P0, const int32_t *__restrict__ Qi0,
+__global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restrict__ P0, const int32_t *__restrict__ Qi0,
                                                      const int32_t *__restrict__ tile_prob,
                                                      const SaProb *__restrict__ probs, int S0,
                                                      int32_t *__restrict__ Rout) {
@@ -618,7 +679,12 @@ __global__ void table_cols(const SaSeg *__restrict__ segs, const int2 *__restric
 }
 
 // delta(r, c) of one subproblem, evaluated by a whole wave (uniform result);
-// r, c in [0, s].  All loads are issued before the first use.
+// r, c in [0, s].  The two partial scans read 16 bytes per lane and load:
+// block starts are multiples of kBlk from a 1024-aligned subproblem base, and
+// every problem's region is a whole number of kTile tiles, so a group of 4
+// that starts below r (or c) lies inside the level arrays; groups past it are
+// not read, and entries past r (or c) inside a group are masked.  All loads
+// are issued before the first use.
 __device__ int wave_delta(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
                           const int32_t *__restrict__ cntHi, const int32_t *__restrict__ cntLo, const SaSeg &sg,
                           int r, int c) {
@@ -628,23 +694,31 @@ __device__ int wave_delta(const uint32_t *__restrict__ U, const uint32_t *__rest
     const int64_t x0 = sg.x0;
     const int64_t tix = sg.tab_off2 + (int64_t)j * (sg.nb + 1) + i;
     const int base = cntHi[tix] - cntLo[tix];
-    uint32_t u[kBlk / 64], v[kBlk / 64];
+    const uint4 *U4 = reinterpret_cast<const uint4 *>(U + x0 + r0);
+    const uint4 *Ui4 = reinterpret_cast<const uint4 *>(Ui + x0 + c0);
+    constexpr int kGroups = kBlk / 256;   // 4-element groups per lane
+    uint4 u[kGroups], v[kGroups];
 #pragma unroll
-    for (int k = 0; k < kBlk / 64; ++k) {
-        const int rr = r0 + k * 64 + lane, cc = c0 + k * 64 + lane;
-        u[k] = U[x0 + (rr < r ? rr : 0)];
-        v[k] = Ui[x0 + (cc < c ? cc : 0)];
+    for (int k = 0; k < kGroups; ++k) {
+        const int g = k * 256 + lane * 4;   // first element of the lane's group
+        u[k] = r0 + g < r ? U4[k * 64 + lane] : make_uint4(0u, 0u, 0u, 0u);
+        v[k] = c0 + g < c ? Ui4[k * 64 + lane] : make_uint4(0u, 0u, 0u, 0u);
     }
     int acc = 0;
 #pragma unroll
-    for (int k = 0; k < kBlk / 64; ++k) {
-        const int rr = r0 + k * 64 + lane, cc = c0 + k * 64 + lane;
-        // rows [r0, r): hi with col < c, lo with col >= c
-        const int col = (int)(u[k] & kValMask);
-        if (rr < r) acc += (u[k] & kHiBit) ? (col < c) : (col >= c);
-        // cols [c0, c): hi with row < r0, lo with row >= r0
-        const int row = (int)(v[k] & kValMask);
-        if (cc < c) acc += (v[k] & kHiBit) ? (row < r0) : (row >= r0);
+    for (int k = 0; k < kGroups; ++k) {
+        const int g = k * 256 + lane * 4;
+        const uint32_t uu[4] = {u[k].x, u[k].y, u[k].z, u[k].w};
+        const uint32_t vv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            // rows [r0, r): hi with col < c, lo with col >= c
+            const int col = (int)(uu[e] & kValMask);
+            if (r0 + g + e < r) acc += ((col >= c) ^ (int)(uu[e] >> 31));
+            // cols [c0, c): hi with row < r0, lo with row >= r0
+            const int row = (int)(vv[e] & kValMask);
+            if (c0 + g + e < c) acc += ((row >= r0) ^ (int)(vv[e] >> 31));
+        }
     }
     return __builtin_amdgcn_readfirstlane(base + wave_sum(acc));
 }
@@ -678,18 +752,22 @@ __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__res
     if ((threadIdx.x & 63) == 0) seg_start[w] = out;
 }
 
-// Walks: one lane per segment, reading the level's U / U^-1 directly (one
-// round trip per step, both candidates of the next move in flight).  Blocks
-// are mapped to XCDs in contiguous ranges of segments (the hardware deals
-// blocks to the 8 XCDs round-robin), so each XCD's L2 holds the rows and
-// columns of its own part of the diagonal range.
-__global__ __launch_bounds__(256) void seg_walk(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
-                                                const SaSeg *__restrict__ segs, const int2 *__restrict__ walks,
-                                                int nwalks, const int32_t *__restrict__ c_end,
-                                                const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
+// Walks: one lane per segment, reading the level's U / U^-1 a window of
+// kWalkWindow rows and columns per round trip (ant_walk_window).  Blocks of
+// kWalkBlock lanes spread the few thousand segments of a level over many
+// CUs, and are mapped to XCDs in contiguous ranges of segments (the hardware
+// deals blocks to the 8 XCDs round-robin), so each XCD's L2 holds the rows
+// and columns of its own part of the diagonal range.
+__global__ __launch_bounds__(kWalkBlock) void seg_walk(const uint32_t *__restrict__ U,
+                                                       const uint32_t *__restrict__ Ui,
+                                                       const SaSeg *__restrict__ segs,
+                                                       const int2 *__restrict__ walks, int nwalks,
+                                                       const int32_t *__restrict__ c_end,
+                                                       const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
+    __shared__ uint32_t win[kWalkBlock * 2 * kWalkWindow];
     const int per_xcd = (int)gridDim.x / 8;   // the grid is a multiple of 8 blocks
     const int b = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);   // XCD-contiguous block order
-    const int w = b * 256 + (int)threadIdx.x;
+    const int w = b * kWalkBlock + (int)threadIdx.x;
     if (w >= nwalks) return;
     const int4 st = seg_start[w];
     if (st.x < 0) return;
@@ -699,8 +777,10 @@ __global__ __launch_bounds__(256) void seg_walk(const uint32_t *__restrict__ U, 
     int32_t *Rs = R + sg.x0;
     const int s = sg.s;
     const int t_end = s - 1 + c_end[job.x];
-    ant_walk(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1), [&](int rr) { return Us[rr]; },
-             [&](int cc) { return Uis[cc]; }, [&](int rr, int cc) { Rs[rr] = cc; });
+    uint32_t *lw = win + (threadIdx.x >> 6) * (64 * 2 * kWalkWindow) + (threadIdx.x & 63);
+    ant_walk_window<kWalkWindow>(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1),
+                                 [&](int rr) { return Us[rr]; }, [&](int cc) { return Uis[cc]; },
+                                 [&](int rr, int cc) { Rs[rr] = cc; }, lw);
 }
 
 __global__ void fill_i32(int32_t *__restrict__ p, int32_t v, int64_t count) {
