@@ -233,8 +233,10 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
     }
     PLAN_TRY(dalloc(&p->d_PR, nt * std::max<size_t>(nl, 1)));
     PLAN_TRY(dalloc(&p->d_QC, nt * std::max<size_t>(nl, 1)));
-    PLAN_TRY(dalloc(&p->d_U, nt));
-    PLAN_TRY(dalloc(&p->d_Ui, nt));
+    // slack past the last tile: the walk reads its window in aligned 16-byte
+    // groups that can reach a few entries past a subproblem's end
+    PLAN_TRY(dalloc(&p->d_U, nt + kWalkSlack));
+    PLAN_TRY(dalloc(&p->d_Ui, nt + kWalkSlack));
     PLAN_TRY(dalloc(&p->d_cntP, (size_t)p->ntiles));
     PLAN_TRY(dalloc(&p->d_cntQ, (size_t)p->ntiles));
     PLAN_TRY(dalloc(&p->d_cend, (size_t)p->max_seg));

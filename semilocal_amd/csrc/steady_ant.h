@@ -11,6 +11,8 @@ constexpr int kBlk = 1024;       // block size of the per-subproblem count table
 constexpr int kSegSteps = 256;   // walk steps per parallel walk segment (large levels)
 constexpr int kWalkBlock = 64;   // lanes (segments) per block of the large-level walk kernel
 constexpr int kWalkWindow = 16;  // rows and columns a large-level walk reads ahead per round trip
+constexpr int kWalkGroups = kWalkWindow / 4 + 1;   // aligned 4-entry groups covering one window side
+constexpr int kWalkSlack = 64;   // entries allocated past the end of U and U^-1 for those groups
 constexpr int kMaxNb = 4096;     // count-table blocks per side: subproblems up to kMaxNb * kBlk
 
 // One product of the batch: elements [base, base + n) of every level array.

@@ -263,42 +263,56 @@ __device__ __forceinline__ void ant_walk(int r, int c, int d, int s, int steps, 
 // The same walk with its reads taken K steps at a time.  A walk moves one
 // row up or one column right per step, so its next K steps read only rows
 // r-1 .. r-K and columns c .. c+K-1: the lane reads that window in one batch
-// of 2K independent loads into its own LDS slots (lw[k * 64], interleaved with
+// of independent loads into its own LDS slots (lw[k * 64], interleaved with
 // the other lanes of its wave, so every access is conflict-free), then takes
 // up to K steps from LDS.  One memory round trip per K steps instead of one
 // per step; the values are the same as ant_walk's, so the walk is too.
-// Out-of-range window entries are clamped reads the walk never uses.
-template <int K, class RowF, class ColF, class PutF>
-__device__ __forceinline__ void ant_walk_window(int r, int c, int d, int s, int steps, RowF rowv, ColF colv,
-                                                PutF put, uint32_t *lw) {
-    uint32_t ur = rowv(r);
-    uint32_t left = c > 0 ? colv(c - 1) : 0u;
+//
+// ant_walk_window over the level arrays themselves (rows U, columns Ui of
+// the subproblem), with each window side read as kWalkGroups aligned 16-byte
+// groups: rows from a_r = (r - K) rounded down to a multiple of 4 (at least
+// 0), columns from a_c = c rounded down.  Those cover rows r-K .. r-1 and
+// columns c .. c+K-1, and reach at most 19 entries past the subproblem's end
+// (its tile padding, the next subproblem or the arrays' slack; such entries
+// are never used).  A step indexes the window by its row and column.
+template <class PutF>
+__device__ __forceinline__ void ant_walk_window4(int r, int c, int d, int s, int steps, const uint32_t *U,
+                                                 const uint32_t *Ui, PutF put, uint32_t *lw) {
+    constexpr int K = kWalkWindow, G = kWalkGroups;
+    uint32_t ur = U[r];
+    uint32_t left = c > 0 ? Ui[c - 1] : 0u;
     int t = 0;
     while (t < steps) {
-        uint32_t wr[K], wc[K];
+        const int ar = r - K > 0 ? (r - K) & ~3 : 0;
+        const int ac = c & ~3;
+        uint4 wr[G], wc[G];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            wr[k] = rowv(r - 1 - k >= 0 ? r - 1 - k : 0);
-            wc[k] = colv(c + k < s ? c + k : s - 1);
+        for (int g = 0; g < G; ++g) {
+            wr[g] = *reinterpret_cast<const uint4 *>(U + ar + 4 * g);
+            wc[g] = *reinterpret_cast<const uint4 *>(Ui + ac + 4 * g);
         }
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            lw[k * 64] = wr[k];
-            lw[(K + k) * 64] = wc[k];
+        for (int g = 0; g < G; ++g) {
+            lw[(4 * g + 0) * 64] = wr[g].x;
+            lw[(4 * g + 1) * 64] = wr[g].y;
+            lw[(4 * g + 2) * 64] = wr[g].z;
+            lw[(4 * g + 3) * 64] = wr[g].w;
+            lw[(4 * G + 4 * g + 0) * 64] = wc[g].x;
+            lw[(4 * G + 4 * g + 1) * 64] = wc[g].y;
+            lw[(4 * G + 4 * g + 2) * 64] = wc[g].z;
+            lw[(4 * G + 4 * g + 3) * 64] = wc[g].w;
         }
         const int t_stop = t + K < steps ? t + K : steps;
-        int iu = 0, ic = 0;
         bool done = false;
         for (; t < t_stop; ++t) {
-            const uint32_t up = lw[iu * 64];         // row r - 1
-            const uint32_t nxt = lw[(K + ic) * 64];  // column c
+            const int ir = r - 1 - ar, ic = c - ac;          // in [0, 4G) for up to K steps
+            const uint32_t up = lw[(ir >= 0 ? ir : 0) * 64];   // row r - 1 (unused at r = 0)
+            const uint32_t nxt = lw[(4 * G + ic) * 64];       // column c
             bool right;
             if (!ant_step(r, c, d, s, ur, left, up, nxt, right, put)) {
                 done = true;
                 break;
             }
-            ic += right ? 1 : 0;
-            iu += right ? 0 : 1;
         }
         if (done) break;
     }
@@ -764,7 +778,7 @@ __global__ __launch_bounds__(kWalkBlock) void seg_walk(const uint32_t *__restric
                                                        const int2 *__restrict__ walks, int nwalks,
                                                        const int32_t *__restrict__ c_end,
                                                        const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
-    __shared__ uint32_t win[kWalkBlock * 2 * kWalkWindow];
+    __shared__ uint32_t win[kWalkBlock * 8 * kWalkGroups];
     const int per_xcd = (int)gridDim.x / 8;   // the grid is a multiple of 8 blocks
     const int b = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);   // XCD-contiguous block order
     const int w = b * kWalkBlock + (int)threadIdx.x;
@@ -777,10 +791,9 @@ __global__ __launch_bounds__(kWalkBlock) void seg_walk(const uint32_t *__restric
     int32_t *Rs = R + sg.x0;
     const int s = sg.s;
     const int t_end = s - 1 + c_end[job.x];
-    uint32_t *lw = win + (threadIdx.x >> 6) * (64 * 2 * kWalkWindow) + (threadIdx.x & 63);
-    ant_walk_window<kWalkWindow>(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1),
-                                 [&](int rr) { return Us[rr]; }, [&](int cc) { return Uis[cc]; },
-                                 [&](int rr, int cc) { Rs[rr] = cc; }, lw);
+    uint32_t *lw = win + (threadIdx.x >> 6) * (64 * 8 * kWalkGroups) + (threadIdx.x & 63);
+    ant_walk_window4(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1), Us, Uis,
+                     [&](int rr, int cc) { Rs[rr] = cc; }, lw);
 }
 
 __global__ void fill_i32(int32_t *__restrict__ p, int32_t v, int64_t count) {
