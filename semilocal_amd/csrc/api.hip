@@ -2,6 +2,7 @@
 // stream management, ragged-batch planning (orientation, length-bucket tile
 // table, LPT queue order), uploads, launches and timing.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -246,6 +247,10 @@ struct Batch {
     size_t items_cap = 0;
     DeviceArena *arena = nullptr;      // null: hipMalloc / hipFree
     bool own_stream = true;
+    // Host sources of the stream-ordered uploads: kept for the batch's life
+    // (the copies run on the batch's stream, after the host returns).
+    std::vector<uint8_t> host_pool;
+    std::vector<int32_t> host_ident;
 };
 
 static hipError_t bt_alloc(Batch *bt, void **p, size_t bytes) {
@@ -370,13 +375,22 @@ static void plan_items(Batch *bt) {
 }
 
 // (Re)uploads the pair table and the queue (device copy: `items`, then
-// `order`); grows the device copies as needed.
+// `order`); grows the device copies as needed.  Every upload of a batch runs
+// on the batch's own stream, so it is ordered before the batch's next launch
+// (copies through the null stream are not ordered with a non-blocking
+// stream).  The host vectors are only rewritten after a sync of that stream.
 static hipError_t upload_items(Batch *bt) {
     hipError_t e = hipSuccess;
     if (!bt->d_pairs) e = bt_alloc(bt, (void **)&bt->d_pairs, sizeof(PairDesc) * std::max<size_t>(bt->pairs.size(), 1));
     if (e == hipSuccess && !bt->pairs.empty())
-        e = hipMemcpy(bt->d_pairs, bt->pairs.data(), sizeof(PairDesc) * bt->pairs.size(), hipMemcpyHostToDevice);
-    const size_t need = 2 * bt->items.size();
+        e = hipMemcpyAsync(bt->d_pairs, bt->pairs.data(), sizeof(PairDesc) * bt->pairs.size(), hipMemcpyHostToDevice,
+                           bt->stream);
+    // sized once for the most passes any tile height gives (R = 4), so a
+    // change of tile shape never reallocates the queue
+    size_t most = 0;
+    for (const auto &d : bt->pairs)
+        if (d.m > 0 && d.n > 0) most += (size_t)((d.m + 255) / 256);
+    const size_t need = 2 * std::max(bt->items.size(), most);
     if (e == hipSuccess && (!bt->d_items || need > bt->items_cap)) {
         bt_free(bt, bt->d_items);
         bt->d_items = nullptr;
@@ -384,18 +398,20 @@ static hipError_t upload_items(Batch *bt) {
         e = bt_alloc(bt, (void **)&bt->d_items, sizeof(WorkItem) * bt->items_cap);
     }
     if (e == hipSuccess && !bt->items.empty())
-        e = hipMemcpy(bt->d_items, bt->items.data(), sizeof(WorkItem) * bt->items.size(), hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(bt->d_items, bt->items.data(), sizeof(WorkItem) * bt->items.size(), hipMemcpyHostToDevice,
+                           bt->stream);
     if (e == hipSuccess && !bt->order.empty())
-        e = hipMemcpy(bt->d_items + bt->items.size(), bt->order.data(), sizeof(WorkItem) * bt->order.size(),
-                      hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(bt->d_items + bt->items.size(), bt->order.data(), sizeof(WorkItem) * bt->order.size(),
+                           hipMemcpyHostToDevice, bt->stream);
     return e;
 }
 
+// Allocates and uploads on the batch's stream; `src` must outlive the copy.
 template <typename T>
 static hipError_t upload(Batch *bt, T **dst, const T *src, size_t count) {
     hipError_t e = bt_alloc(bt, (void **)dst, sizeof(T) * std::max<size_t>(count, 1));
     if (e != hipSuccess) return e;
-    if (count) e = hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice);
+    if (count) e = hipMemcpyAsync(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice, bt->stream);
     return e;
 }
 
@@ -484,37 +500,53 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     }
     for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&bt->ev[k]);
     if (e == hipSuccess) e = upload_items(bt);
-    if (e == hipSuccess) e = upload(bt, (uint8_t **)&bt->d_pool, pool.bytes.data(), pool.bytes.size());
+    bt->host_pool = std::move(pool.bytes);
+    if (e == hipSuccess) e = upload(bt, (uint8_t **)&bt->d_pool, bt->host_pool.data(), bt->host_pool.size());
     if (e == hipSuccess)
         e = bt_alloc(bt, (void **)&bt->d_rc, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
     if (e == hipSuccess && bt->with_cr)
         e = bt_alloc(bt, (void **)&bt->d_cr, sizeof(int32_t) * std::max<int64_t>(bt->perm_total, 1));
     if (e == hipSuccess)
         e = bt_alloc(bt, (void **)&bt->d_buf, sizeof(uint64_t) * std::max<int64_t>(bt->buf_total, 1));
-    if (e == hipSuccess) e = hipMemset(bt->d_buf, 0, sizeof(uint64_t) * std::max<int64_t>(bt->buf_total, 1));
+    // cleared on the batch's own stream, so the clear is ordered before every
+    // combing launch (a null-stream memset is not ordered with a non-blocking
+    // stream, and the memory may hold another batch's tagged entries)
+    if (e == hipSuccess)
+        e = launch_fill_u32((uint32_t *)bt->d_buf, 0u, 2u * (uint64_t)std::max<int64_t>(bt->buf_total, 1), bt->stream);
     if (e == hipSuccess) e = bt_alloc(bt, (void **)&bt->d_ctl, sizeof(LaunchCtl) * (size_t)(npairs + 1));
+    // the control blocks hold the error bits batch_sync reports: cleared
+    // here, so a sync before the first run reads no memory left by others
+    if (e == hipSuccess)
+        e = launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(npairs + 1) * 2u, bt->stream);
     if (e != hipSuccess) {
         set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
         batch_free(bt);
         return e == hipErrorOutOfMemory ? kErrNoMem : kErrHip;
     }
-    // degenerate pairs (m == 0 or n == 0) have no tiles: their kernel is the identity
-    std::vector<int32_t> ident;
-    for (int i = 0; i < npairs; ++i) {
+    // degenerate pairs (m == 0 or n == 0) have no tiles: their kernel is the
+    // identity, copied from one host array that lives as long as the batch
+    int64_t ident_len = 0;
+    for (const auto &d : bt->pairs)
+        if (d.npasses == 0) ident_len = std::max<int64_t>(ident_len, (int64_t)d.m + d.n);
+    bt->host_ident.resize((size_t)ident_len);
+    std::iota(bt->host_ident.begin(), bt->host_ident.end(), 0);
+    for (int i = 0; i < npairs && e == hipSuccess; ++i) {
         const PairDesc &d = bt->pairs[i];
         if (d.npasses == 0 && d.m + d.n > 0) {
-            ident.resize((size_t)(d.m + d.n));
-            std::iota(ident.begin(), ident.end(), 0);
-            e = hipMemcpy(bt->d_rc + d.perm_off, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice);
+            const size_t bytes = sizeof(int32_t) * (size_t)(d.m + d.n);
+            e = hipMemcpyAsync(bt->d_rc + d.perm_off, bt->host_ident.data(), bytes, hipMemcpyHostToDevice, bt->stream);
             if (e == hipSuccess && bt->with_cr)
-                e = hipMemcpy(bt->d_cr + d.perm_off, ident.data(), sizeof(int32_t) * ident.size(),
-                              hipMemcpyHostToDevice);
-            if (e != hipSuccess) {
-                set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
-                batch_free(bt);
-                return kErrHip;
-            }
+                e = hipMemcpyAsync(bt->d_cr + d.perm_off, bt->host_ident.data(), bytes, hipMemcpyHostToDevice,
+                                   bt->stream);
         }
+    }
+    // a batch on its own stream is complete when created; one on a caller's
+    // stream is ordered on that stream
+    if (e == hipSuccess && bt->own_stream) e = hipStreamSynchronize(bt->stream);
+    if (e != hipSuccess) {
+        set_error(std::string("sl_batch_create: ") + hipGetErrorString(e));
+        batch_free(bt);
+        return kErrHip;
     }
     *out = bt;
     return kOk;
@@ -530,11 +562,22 @@ static int batch_check_ready(const Batch *bt) {
     return kOk;
 }
 
+// Run epochs come from one counter for the whole process, so no two runs of
+// any two batches tag their strand hand-offs alike (until the 16-bit counter
+// wraps, 65,535 runs later); memory a batch inherits from another can then
+// never hold an entry that matches.  Tag 0 is a cleared buffer's content.
+static std::atomic<uint32_t> g_epoch{0};
+uint32_t next_run_epoch() {
+    for (;;) {
+        const uint32_t v = (g_epoch.fetch_add(1u) + 1u) & 0xFFFFu;
+        if (v != 0) return v;
+    }
+}
+
 static int batch_reset(Batch *bt) {
     const int rc = batch_check_ready(bt);
     if (rc) return rc;
-    bt->epoch = (bt->epoch + 1u) & 0xFFFFu;   // new tags for this run's strand hand-offs
-    if (bt->epoch == 0) bt->epoch = 1;         // tag 0 is the buffer's initial content
+    bt->epoch = next_run_epoch();   // new tags for this run's strand hand-offs
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
     return kOk;
 }
@@ -565,7 +608,7 @@ static int batch_run(Batch *bt) {
 static int batch_run_replayable(Batch *bt) {
     const int rc = batch_check_ready(bt);
     if (rc) return rc;
-    if (bt->epoch == 0) bt->epoch = 1;
+    if (bt->epoch == 0) bt->epoch = next_run_epoch();
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_buf, 0u, (uint64_t)bt->buf_total * 2u, bt->stream));
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
     SL_HIP_TRY(launch_items(bt, bt->d_items + bt->items.size(), (uint32_t)bt->order.size(), bt->d_ctl));
@@ -737,10 +780,14 @@ int sl_batch_download(sl_batch *h, int32_t *rc_cat, int32_t *cr_cat) {
         set_error("batch was created without SL_BATCH_WITH_COL_TO_ROW");
         return kErrArg;
     }
-    if (bt->perm_total) {
-        SL_HIP_TRY(hipMemcpy(rc_cat, bt->d_rc, sizeof(int32_t) * bt->perm_total, hipMemcpyDeviceToHost));
-        if (cr_cat) SL_HIP_TRY(hipMemcpy(cr_cat, bt->d_cr, sizeof(int32_t) * bt->perm_total, hipMemcpyDeviceToHost));
+    if (bt->perm_total) {   // ordered after the batch's launches on its stream
+        SL_HIP_TRY(hipMemcpyAsync(rc_cat, bt->d_rc, sizeof(int32_t) * bt->perm_total, hipMemcpyDeviceToHost,
+                                  bt->stream));
+        if (cr_cat)
+            SL_HIP_TRY(hipMemcpyAsync(cr_cat, bt->d_cr, sizeof(int32_t) * bt->perm_total, hipMemcpyDeviceToHost,
+                                      bt->stream));
     }
+    SL_HIP_TRY(hipStreamSynchronize(bt->stream));
     return kOk;
 }
 
@@ -755,8 +802,10 @@ int sl_batch_time(sl_batch *h, int mode, int warmup, int iters, double *ms_per_r
         int rc = run();
         if (rc) return rc;
     }
-    int rc = batch_sync(bt);
+    // the warm-up runs' errors; without warm-up runs only the stream is idle
+    int rc = warmup > 0 ? batch_sync(bt) : kOk;
     if (rc) return rc;
+    SL_HIP_TRY(hipStreamSynchronize(bt->stream));
     SL_HIP_TRY(hipEventRecord(bt->ev[0], bt->stream));
     for (int it = 0; it < iters; ++it) {
         rc = run();
@@ -862,7 +911,11 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
                 if (rc) return rc;
                 double ms = 0.0;
                 rc = sl_batch_time(h, 0, 0, iters, &ms, nullptr);   // each run is a whole launch: no warm-up
-                if (rc) return rc;
+                if (rc) {
+                    set_error("sl_batch_autotune: rows_per_lane " + std::to_string(r) + ", waves_per_cu " +
+                              std::to_string(w) + ", order " + std::to_string(sched) + ": " + last_error());
+                    return rc;
+                }
                 if (ms < best) {
                     best = ms;
                     best_r = r;

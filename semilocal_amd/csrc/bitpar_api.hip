@@ -47,8 +47,7 @@ static int bitplan_run(BitPlan *p) {
         set_error("bit-parallel plan is missing a device buffer (internal defect); nothing launched");
         return kErrHip;
     }
-    p->d.epoch = (p->d.epoch + 1u) & 0xFFFFu;
-    if (p->d.epoch == 0) p->d.epoch = 1;   // tag 0 is the buffer's initial content
+    p->d.epoch = next_run_epoch();   // unique tags for this run's hand-offs; tag 0 is a cleared buffer
     SL_HIP_TRY(launch_fill_u32(p->d_scratch, 0u, 4, p->stream));
     SL_HIP_TRY(launch_bitpar(p->d, p->waves, reinterpret_cast<LaunchCtl *>(p->d_scratch), p->spin_limit, p->stream));
     return kOk;
@@ -90,8 +89,13 @@ int sl_bitpar_create(sl_bitpar **out, const uint64_t *a_rev, int a_words, const 
     if (e == hipSuccess) e = hipMemcpy(p->d_words, a_rev, sizeof(uint64_t) * a_words, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_words + M, b, sizeof(uint64_t) * b_words, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_buf, sizeof(uint64_t) * (size_t)N);
-    if (e == hipSuccess) e = hipMemset(p->d_buf, 0, sizeof(uint64_t) * (size_t)N);
+    // cleared on the handle's own stream, ordered before every launch
+    if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t *>(p->d_buf), 0u, 2u * (uint64_t)N, p->stream);
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_scratch, sizeof(uint32_t) * 4);
+    if (e == hipSuccess) e = launch_fill_u32(p->d_scratch, 0u, 4, p->stream);   // no bits left by others
+    // the word uploads go through the null stream, which is not ordered with
+    // the handle's non-blocking stream: finish them before the first launch
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_error(std::string("sl_bitpar_create: ") + hipGetErrorString(e));
         bitplan_free(p);

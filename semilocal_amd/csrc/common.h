@@ -21,6 +21,7 @@ void set_error(const std::string &msg);
 const char *last_error();
 int ensure_device(int device = -1);   // selects (and on first use initialises) the HIP device
 int device_cus();                     // compute units of the selected device
+uint32_t next_run_epoch();            // nonzero 16-bit run tag, unique across every handle of the process
 
 #define SL_HIP_TRY(expr)                                                              \
     do {                                                                              \

@@ -247,6 +247,9 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
         PLAN_TRY(dalloc(&p->d_map_row, nt));
         PLAN_TRY(dalloc(&p->d_map_col, nt));
     }
+    // the uploads above go through the null stream, which is not ordered with
+    // the non-blocking streams the plan later runs on: finish them here
+    PLAN_TRY(hipDeviceSynchronize());
     *out = p;
     return kOk;
 }

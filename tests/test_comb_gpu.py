@@ -184,3 +184,46 @@ def test_cell_table_with_16bit_symbols(gpu, oracle):
         for k, (got, exp) in enumerate(zip(b.run(per_pair=per_pair).results(), want)):
             assert np.array_equal(got, exp), (per_pair, k)
     b.close()
+
+
+def test_new_batches_in_freed_memory_match_the_oracle(gpu, oracle):
+    # The strand buffer of a new batch lands in memory freed by earlier
+    # batches, which holds their tagged hand-off entries.  Run tags are unique
+    # across the process and a new buffer is cleared on the batch's own
+    # stream, so no entry left behind is taken for a new run's.
+    g = np.random.default_rng(19)
+    big = [(g.integers(0, 4, 20000).astype(np.uint8), g.integers(0, 4, 30000).astype(np.uint8)),
+           (g.integers(0, 4, 15000).astype(np.uint8), g.integers(0, 4, 40000).astype(np.uint8))]
+    b = gpu.Batch(big, rows_per_lane=16)
+    for _ in range(3):
+        b.run()
+    b.close()
+    for k, R in enumerate((4, 8, 16, 4, 8, 16)):
+        pairs = [(g.integers(0, 4, int(g.integers(2000, 9000))).astype(np.uint8),
+                  g.integers(0, 4, int(g.integers(2000, 12000))).astype(np.uint8)) for _ in range(6)]
+        b = gpu.Batch(pairs, rows_per_lane=R)
+        for i, ((x, y), got) in enumerate(zip(pairs, b.run().results())):
+            assert np.array_equal(got, oracle.comb(x, y, threads=8)), (k, R, i)
+        b.close()
+
+
+def test_timing_before_any_run_reports_no_stale_error(gpu, oracle):
+    # A new batch's control blocks may sit in memory another handle just
+    # freed (here a bit-parallel handle's scratch words).  They are cleared at
+    # creation, so a timing call without warm-up as the batch's first call
+    # (what the autotuner does) reports only its own runs.
+    g = np.random.default_rng(20)
+    for _ in range(3):
+        x = g.integers(0, 2, 1 << 16).astype(np.uint8)
+        y = g.integers(0, 2, 1 << 16).astype(np.uint8)
+        bp = gpu.BitParallel(x, y)
+        bp.llcs()
+        bp.time(warmup=1, iters=2)
+        bp.close()
+        pairs = [(g.integers(0, 4, 3000).astype(np.uint8), g.integers(0, 4, 5000).astype(np.uint8)) for _ in range(4)]
+        b = gpu.Batch(pairs)
+        b.set_tiles(4, 4)
+        b.time(warmup=0, iters=1)
+        for (u, v), got in zip(pairs, b.results()):
+            assert np.array_equal(got, oracle.comb(u, v, threads=8))
+        b.close()
