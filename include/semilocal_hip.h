@@ -137,6 +137,15 @@ int sl_batch_stats(const sl_batch *bt, int64_t *cells, int64_t *tiles, int64_t *
  * the code cell instead: one LDS read fewer per step, lower latency when few
  * waves share a CU.  Results never depend on it. */
 int sl_batch_cell_modes(const sl_batch *bt, int64_t *code_pairs, int64_t *onehot_pairs, int64_t *table_pairs);
+/* Host-only planning, no device needed: the queue a batch of pairs with
+ * these string lengths gets at tile height `rows_per_lane`, `waves_per_cu`
+ * waves on `cus` compute units and queue order `schedule` (as in
+ * sl_batch_set_schedule).  Writes the (pair, pass) tickets in launch order
+ * (pair_out / pass_out may be NULL to ask for *nitems only; otherwise they
+ * hold *nitems entries: one per pass of 64 * rows_per_lane rows of every
+ * pair with both lengths > 0). */
+int sl_plan_queue(int npairs, const int32_t *a_len, const int32_t *b_len, int rows_per_lane, int waves_per_cu,
+                  int cus, int schedule, int32_t *pair_out, int32_t *pass_out, int64_t *nitems);
 void sl_batch_destroy(sl_batch *bt);
 
 /* One-shot convenience: create + run + sync + download + destroy. */

@@ -7,11 +7,11 @@ Python binding used by the tests and the benchmark.
 """
 from ._lib import (Batch, BitParallel, Hybrid, SemiLocalError, braid_multiply, braid_multiply_batch,
                    braid_multiply_time, comb_batch, device_info, hybrid_iterative, lib, llcs_bitparallel,
-                   semi_local_comb, staggered_multiply, Server)
+                   plan_queue, semi_local_comb, staggered_multiply, Server)
 
 __all__ = ["Batch", "BitParallel", "Hybrid", "SemiLocalError", "braid_multiply", "braid_multiply_batch",
            "braid_multiply_time", "comb_batch", "device_info", "hybrid_iterative", "lib", "llcs_bitparallel",
-           "semi_local_comb", "staggered_multiply", "Server", "llcs_from_kernel"]
+           "plan_queue", "semi_local_comb", "staggered_multiply", "Server", "llcs_from_kernel"]
 
 
 def llcs_from_kernel(perm, m: int, n: int) -> int:

@@ -61,6 +61,8 @@ SIGNATURES = {
     "sl_batch_get_schedule": ([_vp, C.POINTER(C.c_int)], C.c_int),
     "sl_batch_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "sl_batch_cell_modes": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "sl_plan_queue": ([C.c_int, _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
+                       C.POINTER(C.c_int64)], C.c_int),
     "sl_batch_destroy": ([_vp], None),
     "sl_llcs_bitparallel": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "sl_bitpar_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int], C.c_int),
@@ -188,6 +190,21 @@ def _split(flat, a_len, b_len):
         out.append(flat[s:s + m + n].copy())
         s += m + n
     return out
+
+
+def plan_queue(a_lens, b_lens, rows_per_lane: int, waves_per_cu: int = 8, cus: int = 256, schedule: int = 2):
+    """The (pair, pass) tickets a batch of pairs with these lengths gets, in
+    launch order (host-only planning: sl_plan_queue, no device needed)."""
+    a_len = np.ascontiguousarray(a_lens, np.int32)
+    b_len = np.ascontiguousarray(b_lens, np.int32)
+    n = C.c_int64()
+    check(lib().sl_plan_queue(len(a_len), a_len, b_len, rows_per_lane, waves_per_cu, cus, schedule, None, None,
+                              C.byref(n)))
+    pair = np.empty(max(n.value, 1), np.int32)
+    pas = np.empty(max(n.value, 1), np.int32)
+    check(lib().sl_plan_queue(len(a_len), a_len, b_len, rows_per_lane, waves_per_cu, cus, schedule,
+                              pair.ctypes.data, pas.ctypes.data, C.byref(n)))
+    return pair[:n.value], pas[:n.value]
 
 
 class Batch:

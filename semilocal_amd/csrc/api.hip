@@ -243,6 +243,7 @@ struct Batch {
     uint32_t epoch = 0;                // run counter; part of the strand tags
     LaunchCtl *d_ctl = nullptr;    // [0] batched launch, [1+i] per-pair launch of pair i
     unsigned waves_cap = 0;
+    int cus = 0;                       // CUs the queue is planned for; 0: the device's
     uint32_t table_words = 0;          // LDS words per wave of the row-match table (kCellTable pairs)
     size_t items_cap = 0;
     DeviceArena *arena = nullptr;      // null: hipMalloc / hipFree
@@ -368,7 +369,7 @@ static void plan_items(Batch *bt) {
         d.item0 = (int32_t)bt->items.size();
         for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
     }
-    bt->waves_cap = (unsigned)device_cus() * bt->waves_per_cu;
+    bt->waves_cap = (unsigned)(bt->cus > 0 ? bt->cus : device_cus()) * bt->waves_per_cu;
     choose_cells(bt);
     if (bt->schedule >= 1) schedule_items(bt, 128.0 * bt->schedule);
     else bt->order = bt->items;
@@ -961,6 +962,41 @@ int sl_batch_cell_modes(const sl_batch *h, int64_t *code_pairs, int64_t *onehot_
 }
 
 void sl_batch_destroy(sl_batch *h) { batch_free(reinterpret_cast<Batch *>(h)); }
+
+int sl_plan_queue(int npairs, const int32_t *a_len, const int32_t *b_len, int rows_per_lane, int waves_per_cu, int cus,
+                  int schedule, int32_t *pair_out, int32_t *pass_out, int64_t *nitems) {
+    if (npairs < 0 || (npairs > 0 && (!a_len || !b_len)) || !nitems ||
+        (rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16) || waves_per_cu < 1 || waves_per_cu > 64 ||
+        cus < 1 || schedule < 0 || schedule > kMaxSchedule) {
+        set_error("sl_plan_queue: bad argument");
+        return kErrArg;
+    }
+    Batch bt;   // host planning only: no device state is touched
+    bt.npairs = npairs;
+    bt.pairs.resize((size_t)npairs);
+    for (int i = 0; i < npairs; ++i) {
+        if (a_len[i] < 0 || b_len[i] < 0) {
+            set_error("sl_plan_queue: negative length");
+            return kErrArg;
+        }
+        PairDesc &d = bt.pairs[(size_t)i];
+        d = PairDesc{};
+        d.m = std::min(a_len[i], b_len[i]);   // rows = the shorter string, as in a batch
+        d.n = std::max(a_len[i], b_len[i]);
+        d.sigma = 1;
+    }
+    bt.rows_per_lane = rows_per_lane;
+    bt.waves_per_cu = (unsigned)waves_per_cu;
+    bt.cus = cus;
+    bt.schedule = schedule;
+    plan_items(&bt);
+    *nitems = (int64_t)bt.order.size();
+    for (size_t k = 0; k < bt.order.size(); ++k) {
+        if (pair_out) pair_out[k] = bt.order[k].pair;
+        if (pass_out) pass_out[k] = bt.order[k].pass;
+    }
+    return kOk;
+}
 
 int sl_comb_batch(int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len, const void *b_cat,
                   const int64_t *b_off, const int32_t *b_len, int sym_bytes, int32_t *rc_cat, int32_t *cr_cat) {
