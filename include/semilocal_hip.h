@@ -269,6 +269,20 @@ typedef struct sl_server sl_server;
 #define SL_QUERY_STRING_SUBSTRING 1  /* LCS(a, b[i:j)),  0 <= i <= j <= n            */
 #define SL_QUERY_SUBSTRING_STRING 2  /* LCS(a[i:j), b),  0 <= i <= j <= m            */
 
+/* Resident-braid index: a wavelet matrix over a kernel (row -> col, size
+ * m+n) that answers every query kind above in O(log(m+n)) rank steps (one
+ * lane per query) instead of an O(m+n) scan.  No reference counterpart as an
+ * index; the counts are those of approximate_matching/utils.h:268-359.
+ * Memory: about 0.19 * (m+n) * log2(m+n) bytes. */
+typedef struct sl_braid_index sl_braid_index;
+int sl_braid_index_create(sl_braid_index **out, const int32_t *row_to_col, int m, int n);
+/* nq queries: kind[k] in SL_QUERY_*, arguments (qi[k], qj[k]) as above;
+ * out[k] = the answer (LCS length, or H(i, j)). */
+int sl_braid_index_query(sl_braid_index *ix, int nq, const int32_t *kind, const int32_t *qi, const int32_t *qj,
+                         int64_t *out);
+int sl_braid_index_stats(const sl_braid_index *ix, int *levels, int64_t *bytes);
+void sl_braid_index_destroy(sl_braid_index *ix);
+
 typedef struct sl_server_stats_t {
     int64_t compares_done, queries_done, cells_done;
     int64_t launches;          /* micro-batches launched (comparisons and queries) */

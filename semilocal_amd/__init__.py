@@ -5,11 +5,11 @@ the C ABI in include/semilocal_hip.h, plus the C++ drop-in strategies in
 include/semilocal_hip.hpp).  This package builds it and exposes a thin
 Python binding used by the tests and the benchmark.
 """
-from ._lib import (Batch, BitParallel, Hybrid, SemiLocalError, braid_multiply, braid_multiply_batch,
+from ._lib import (Batch, BitParallel, BraidIndex, Hybrid, SemiLocalError, braid_multiply, braid_multiply_batch,
                    braid_multiply_time, comb_batch, device_info, hybrid_iterative, lib, llcs_bitparallel,
                    plan_queue, semi_local_comb, staggered_multiply, Server)
 
-__all__ = ["Batch", "BitParallel", "Hybrid", "SemiLocalError", "braid_multiply", "braid_multiply_batch",
+__all__ = ["Batch", "BitParallel", "BraidIndex", "Hybrid", "SemiLocalError", "braid_multiply", "braid_multiply_batch",
            "braid_multiply_time", "comb_batch", "device_info", "hybrid_iterative", "lib", "llcs_bitparallel",
            "plan_queue", "semi_local_comb", "staggered_multiply", "Server", "llcs_from_kernel"]
 

@@ -63,6 +63,12 @@ SIGNATURES = {
     "sl_batch_cell_modes": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "sl_plan_queue": ([C.c_int, _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
                        C.POINTER(C.c_int64)], C.c_int),
+    "sl_braid_index_create": ([C.POINTER(_vp), _i32p, C.c_int, C.c_int], C.c_int),
+    "sl_braid_index_query": ([_vp, C.c_int, _i32p, _i32p, _i32p, np.ctypeslib.ndpointer(dtype=np.int64,
+                                                                                       flags="C_CONTIGUOUS")],
+                             C.c_int),
+    "sl_braid_index_stats": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int64)], C.c_int),
+    "sl_braid_index_destroy": ([_vp], None),
     "sl_batch_destroy": ([_vp], None),
     "sl_llcs_bitparallel": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "sl_bitpar_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int], C.c_int),
@@ -205,6 +211,45 @@ def plan_queue(a_lens, b_lens, rows_per_lane: int, waves_per_cu: int = 8, cus: i
     check(lib().sl_plan_queue(len(a_len), a_len, b_len, rows_per_lane, waves_per_cu, cus, schedule,
                               pair.ctypes.data, pas.ctypes.data, C.byref(n)))
     return pair[:n.value], pas[:n.value]
+
+
+class BraidIndex:
+    """Wavelet-matrix index over a kernel (sl_braid_index_*): O(log(m+n))
+    H(i, j) / string-substring / substring-string queries."""
+
+    H, STRING_SUBSTRING, SUBSTRING_STRING = 0, 1, 2
+
+    def __init__(self, perm, m: int, n: int):
+        p = np.ascontiguousarray(perm, np.int32)
+        if len(p) != m + n:
+            raise ValueError("perm must have m + n entries")
+        h = _vp()
+        check(lib().sl_braid_index_create(C.byref(h), p if len(p) else np.zeros(1, np.int32), m, n))
+        self.h = h
+
+    def query(self, kinds, qi, qj) -> np.ndarray:
+        k = np.ascontiguousarray(kinds, np.int32)
+        i = np.ascontiguousarray(qi, np.int32)
+        j = np.ascontiguousarray(qj, np.int32)
+        out = np.empty(max(len(k), 1), np.int64)
+        check(lib().sl_braid_index_query(self.h, len(k), k, i, j, out))
+        return out[:len(k)]
+
+    def stats(self):
+        lv, b = C.c_int(), C.c_int64()
+        check(lib().sl_braid_index_stats(self.h, C.byref(lv), C.byref(b)))
+        return {"levels": lv.value, "bytes": b.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sl_braid_index_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:
