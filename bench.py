@@ -142,6 +142,62 @@ def bench_serving(S, W, count: int, rate_hz: float, pool_gb: float):
     }
 
 
+def bench_index(S, W):
+    """Resident-braid queries on one large braid (200k x 200k, alphabet 4):
+    20,000 string-substring queries through the request loop with the braid
+    scanned per query vs answered from its wavelet-matrix index, and the
+    index's own build and query rates."""
+    g = np.random.default_rng(9)
+    m = n = 200_000
+    a = g.integers(0, 4, m).astype(np.uint8)
+    b = a.copy()
+    sub = g.random(n) < 0.05                      # a similar string: 5% substitutions
+    b[sub] = g.integers(0, 4, int(np.count_nonzero(sub))).astype(np.uint8)
+    nq = 20_000
+    lo = g.integers(0, n, nq)
+    hi = np.minimum(n, lo + g.integers(1, 50_000, nq))
+    out = {}
+    answers = {}
+    for mode, index_min in (("scan", 0), ("index", 1)):
+        srv = S.Server(pool_bytes=8 << 30)
+        srv.set_index(index_min)
+        rid = srv.submit_compare(a, b)
+        srv.result(rid)
+        t0 = time.perf_counter()
+        while index_min and srv.stats()["indexed_braids"] < 1 and time.perf_counter() - t0 < 30:
+            time.sleep(0.0005)
+        build_ms = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        qids = [srv.submit_query(rid, srv.STRING_SUBSTRING, int(x), int(y)) for x, y in zip(lo, hi)]
+        vals = [srv.result(q)[0] for q in qids]
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        answers[mode] = vals
+        out[mode] = {"queries": nq, "seconds": round(dt, 3), "queries_per_s": round(nq / dt, 1),
+                     "index_queries": st["index_queries"]}
+        if index_min:
+            out[mode]["index_build_ms"] = round(build_ms, 2)
+        srv.close()
+    perm = S.semi_local_comb(a, b)
+    ix = S.BraidIndex(perm, m, n)
+    kinds = np.full(100_000, 1, np.int32)
+    qi = g.integers(0, n, 100_000).astype(np.int32)
+    qj = np.minimum(n, qi + g.integers(0, 50_000, 100_000)).astype(np.int32)
+    ix.query(kinds[:10], qi[:10], qj[:10])
+    t0 = time.perf_counter()
+    ix.query(kinds, qi, qj)
+    qdt = time.perf_counter() - t0
+    st = ix.stats()
+    ix.close()
+    same = answers["scan"] == answers["index"]
+    return {"workload": "one resident braid of a 200,000 x 200,000 pair (alphabet 4); 20,000 string-substring "
+                        "queries through the request loop",
+            "scan": out["scan"], "index": out["index"],
+            "standalone_index": {"levels": st["levels"], "bytes": st["bytes"], "queries": 100_000,
+                                 "ms_incl_transfers": round(qdt * 1e3, 2)},
+            "parity": "index answers equal scan answers" if same else "MISMATCH"}
+
+
 def bench_hybrid(S, W, golden):
     """Config 3: 8x8 blocks combed in one launch, merged by 6 levels of batched
     steady-ant multiplications."""
@@ -231,7 +287,7 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
     ap.add_argument("--no-extras", action="store_true", help="only the ragged batch (configs 1, 2, 3, 5 skipped)")
-    ap.add_argument("--only", choices=["plain", "binary", "hybrid", "multiply", "serving"],
+    ap.add_argument("--only", choices=["plain", "binary", "hybrid", "multiply", "index", "serving"],
                     help="run one extra section alone (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
@@ -254,6 +310,7 @@ def main():
               "binary": lambda: bench_binary(semilocal_amd, W, golden),
               "hybrid": lambda: bench_hybrid(semilocal_amd, W, golden),
               "multiply": lambda: bench_multiply(semilocal_amd),
+              "index": lambda: bench_index(semilocal_amd, W),
               "serving": lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
                                                args.serve_pool_gb)}[args.only]
         print(json.dumps({"section": args.only, **fn()}), flush=True)
@@ -359,6 +416,7 @@ def main():
                         ("binary_pair_2p20", lambda: bench_binary(semilocal_amd, W, golden)),
                         ("similar_pair_1e6_hybrid", lambda: bench_hybrid(semilocal_amd, W, golden)),
                         ("braid_multiply_2p20", lambda: bench_multiply(semilocal_amd)),
+                        ("braid_index_queries", lambda: bench_index(semilocal_amd, W)),
                         ("serving", lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
                                                           args.serve_pool_gb))):
             t0 = time.perf_counter()

@@ -291,6 +291,8 @@ typedef struct sl_server_stats_t {
     int64_t pool_bytes, pool_in_use, pool_peak;
     int64_t resident_braids;
     double busy_seconds;       /* first submission to last completion */
+    int64_t indexed_braids;    /* index builds completed (sl_server_set_index) */
+    int64_t index_queries;     /* queries answered from an index */
 } sl_server_stats_t;
 
 /* pool_bytes 0: 90% of the free HBM; max_inflight 0: 4 concurrent
@@ -298,6 +300,15 @@ typedef struct sl_server_stats_t {
  * 0: 4e10 cells per micro-batch. */
 int sl_server_create(sl_server **out, int64_t pool_bytes, int max_inflight, int max_batch,
                      int64_t max_batch_cells);
+/* Kept braids with m + n >= min_size (default: off) get a wavelet-matrix
+ * index (as sl_braid_index_*), built on the device on a stream of its own
+ * once their comparison's result is out; queries on a braid whose index is
+ * built take O(log(m+n)) rank steps instead of an O(m+n) scan (earlier ones
+ * scan).  min_size <= 0 turns indexing off.  Applies to comparisons launched
+ * after the call.  The index lives in the pool with its braid.  A build is
+ * ~3 kernels per bit of m+n: it pays for braids that get many queries, and
+ * costs latency when most braids get few. */
+int sl_server_set_index(sl_server *s, int64_t min_size);
 /* Queues the comparison of a (m symbols, rows) with b (n symbols, columns);
  * the input is copied.  *id names the request and its braid.  keep == 0
  * frees the braid as soon as the score is known.  Result: the LLCS. */

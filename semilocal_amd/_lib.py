@@ -100,6 +100,7 @@ SIGNATURES = {
     "sl_server_download_braid": ([_vp, C.c_int64, _vp], C.c_int),
     "sl_server_release": ([_vp, C.c_int64], C.c_int),
     "sl_server_stats": ([_vp, _vp], C.c_int),
+    "sl_server_set_index": ([_vp, C.c_int64], C.c_int),
     "sl_server_destroy": ([_vp], None),
 }
 
@@ -539,7 +540,8 @@ class ServerStats(C.Structure):
     _fields_ = [("compares_done", C.c_int64), ("queries_done", C.c_int64), ("cells_done", C.c_int64),
                 ("launches", C.c_int64), ("deferrals", C.c_int64), ("evictions", C.c_int64),
                 ("pool_bytes", C.c_int64), ("pool_in_use", C.c_int64), ("pool_peak", C.c_int64),
-                ("resident_braids", C.c_int64), ("busy_seconds", C.c_double)]
+                ("resident_braids", C.c_int64), ("busy_seconds", C.c_double),
+                ("indexed_braids", C.c_int64), ("index_queries", C.c_int64)]
 
 
 class Server:
@@ -554,6 +556,10 @@ class Server:
         check(lib().sl_server_create(C.byref(h), int(pool_bytes), int(max_inflight), int(max_batch),
                                      int(max_batch_cells)))
         self.h = h
+
+    def set_index(self, min_size: int):
+        """Braids with m + n >= min_size get a wavelet-matrix index (<= 0: none)."""
+        check(lib().sl_server_set_index(self.h, int(min_size)))
 
     def submit_compare(self, a, b, keep: bool = True) -> int:
         a_arr, b_arr, w = _pair_symbols(a, b)

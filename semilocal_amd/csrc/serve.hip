@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/semilocal_hip.h"
+#include "braid_index.h"
 #include "common.h"
 
 namespace slhip {
@@ -63,6 +64,7 @@ struct QueryDesc {
     int32_t kind;      // SL_QUERY_*
     int32_t i, j;
     int32_t pad;
+    BraidIndexView ix; // levels > 0: the braid's wavelet-matrix index
 };
 
 __device__ __forceinline__ int block_sum(int v, int *tmp) {
@@ -96,6 +98,10 @@ __global__ __launch_bounds__(256) void score_and_keep(const ScoreDesc *__restric
 __global__ __launch_bounds__(256) void answer_queries(const QueryDesc *__restrict__ q, int64_t *__restrict__ out) {
     __shared__ int tmp[4];
     const QueryDesc qd = q[blockIdx.x];
+    if (qd.ix.levels > 0) {   // indexed braid: O(log(m+n)) rank walk by one lane
+        if (threadIdx.x == 0) out[blockIdx.x] = wm_query(qd.ix, qd.kind, qd.i, qd.j);
+        return;
+    }
     int hi_, hj;   // H arguments
     if (qd.kind == SL_QUERY_STRING_SUBSTRING) {         // LCS(a, b[i:j)) = H(i, j)
         hi_ = qd.i;
@@ -219,6 +225,10 @@ enum BraidState { kBraidPending = 0, kBraidResident = 1, kBraidGone = 2 };
 struct Braid {
     int state = kBraidPending;
     int32_t *d_rc = nullptr;
+    void *d_ix = nullptr;       // wavelet-matrix index (braids of at least index_min entries)
+    BraidIndexView ix{};
+    bool ix_ready = false;      // the index is built: queries use it
+    int64_t ix_job = 0;         // nonzero while its build is in flight
     int m = 0, n = 0;
     int refs = 0;               // unfinished queries that read it
     bool released = false;      // the client gave it up
@@ -239,12 +249,25 @@ struct Flight {
     int64_t cells = 0;
 };
 
+// Index builds of one micro-batch, on the index stream after the batch's
+// results.  Memory of a braid dropped while its build runs is returned when
+// the build is done.
+struct IndexJob {
+    hipEvent_t done = nullptr;
+    std::vector<int64_t> braids;
+    std::vector<void *> release_when_done;   // scratch, and memory of braids dropped meanwhile
+};
+
 struct Server {
     Arena arena;
     int max_inflight = 4;
     int max_batch = 1024;
     int64_t max_batch_cells = 0;
+    int64_t index_min = 0;           // kept braids with m+n >= index_min get an index (<= 0: none)
     std::vector<Flight> flights;
+    hipStream_t ix_stream = nullptr;  // index builds, off the comparisons' streams
+    std::map<int64_t, IndexJob> ix_jobs;
+    int64_t next_ix_job = 1;
     std::mutex mu;
     std::condition_variable cv;      // wakes the scheduler
     std::condition_variable done_cv; // wakes waiting clients
@@ -258,6 +281,7 @@ struct Server {
     std::string fatal;   // a device-side failure that stopped the loop
     // statistics
     int64_t n_compares = 0, n_queries = 0, cells_done = 0, deferrals = 0, evictions = 0, launches = 0;
+    int64_t n_indexed = 0, n_index_queries = 0;
     Clock::time_point t_first_submit, t_last_done;
     bool any_submit = false;
 };
@@ -271,10 +295,56 @@ static void finish(Server *S, Req *r, int status, int64_t result, const std::str
     S->t_last_done = r->t_done;
 }
 
+// Returns a braid's device memory (its kernel and its index) to the pool,
+// or, while its index build still reads the kernel, once the build is done.
+static void braid_drop(Server *S, Braid &b) {
+    auto job = b.ix_job ? S->ix_jobs.find(b.ix_job) : S->ix_jobs.end();
+    if (job != S->ix_jobs.end()) {
+        job->second.release_when_done.push_back(b.d_rc);
+        if (b.d_ix) job->second.release_when_done.push_back(b.d_ix);
+    } else {
+        S->arena.release(b.d_rc);
+        if (b.d_ix) S->arena.release(b.d_ix);
+    }
+    b.d_rc = nullptr;
+    b.d_ix = nullptr;
+    b.ix = BraidIndexView{};
+    b.ix_ready = false;
+    b.ix_job = 0;
+}
+
+// Finished index builds: their braids' queries switch to the index.
+static bool complete_index_jobs(Server *S) {
+    bool any = false;
+    for (auto it = S->ix_jobs.begin(); it != S->ix_jobs.end();) {
+        const hipError_t q = hipEventQuery(it->second.done);
+        if (q == hipErrorNotReady) {
+            ++it;
+            continue;
+        }
+        if (q != hipSuccess) {
+            S->fatal = std::string("request loop: index build failed: ") + hipGetErrorString(q);
+            return any;
+        }
+        for (int64_t id : it->second.braids) {
+            auto bi = S->braids.find(id);
+            if (bi != S->braids.end() && bi->second.ix_job == it->first) {
+                bi->second.ix_ready = true;
+                bi->second.ix_job = 0;
+                ++S->n_indexed;
+            }
+        }
+        for (void *p : it->second.release_when_done) S->arena.release(p);
+        (void)hipEventDestroy(it->second.done);
+        it = S->ix_jobs.erase(it);
+        any = true;
+    }
+    return any;
+}
+
 static void braid_free_if_unused(Server *S, int64_t id, Braid &b) {
     if (b.state == kBraidResident && b.refs == 0 && b.released) {
-        S->arena.release(b.d_rc);
-        b.d_rc = nullptr;
+        braid_drop(S, b);
         b.state = kBraidGone;
         b.gone_reason = "released";
     }
@@ -287,15 +357,14 @@ static bool evict_one(Server *S) {
     uint64_t best = UINT64_MAX;
     for (auto &kv : S->braids) {
         Braid &b = kv.second;
-        if (b.state == kBraidResident && b.refs == 0 && b.last_use < best) {
+        if (b.state == kBraidResident && b.refs == 0 && b.ix_job == 0 && b.last_use < best) {
             best = b.last_use;
             victim = kv.first;
         }
     }
     if (!victim) return false;
     Braid &b = S->braids[victim];
-    S->arena.release(b.d_rc);
-    b.d_rc = nullptr;
+    braid_drop(S, b);
     b.state = kBraidGone;
     b.gone_reason = "evicted to admit new work";
     ++S->evictions;
@@ -312,8 +381,7 @@ static void complete_flight(Server *S, Flight &F) {
             Req *r = F.reqs[k];
             Braid &b = S->braids[r->id];
             if (rc) {
-                S->arena.release(b.d_rc);
-                b.d_rc = nullptr;
+                braid_drop(S, b);
                 b.state = kBraidGone;
                 b.gone_reason = "comparison failed";
                 finish(S, r, rc, 0, last_error());
@@ -453,17 +521,57 @@ static bool launch_compares(Server *S, Flight &F, std::unique_lock<std::mutex> &
         if (e == hipSuccess)
             e = hipMemcpyAsync(F.h_out, F.d_out, sizeof(int64_t) * take.size(), hipMemcpyDeviceToHost, F.stream);
         if (e == hipSuccess) e = hipEventRecord(F.done, F.stream);
+        // indexes of the large braids: built from their resident copies on the
+        // index stream once the micro-batch's results are out, so no result
+        // waits for an index (a braid whose index does not fit keeps the scan)
+        std::vector<void *> ixmem(take.size(), nullptr);
+        std::vector<BraidIndexView> ixview(take.size());
+        IndexJob job;
+        bool waited = false;
+        for (size_t k = 0; k < take.size() && e == hipSuccess; ++k) {
+            const int64_t size = (int64_t)take[k]->m + take[k]->n;
+            if (S->index_min <= 0 || size < S->index_min || !take[k]->keep) continue;
+            void *mem = nullptr, *scratch = nullptr;
+            if (S->arena.alloc(&mem, braid_index_bytes(size)) != hipSuccess) continue;
+            if (S->arena.alloc(&scratch, braid_index_scratch_bytes(size)) != hipSuccess) {
+                S->arena.release(mem);
+                continue;
+            }
+            if (!waited) e = hipStreamWaitEvent(S->ix_stream, F.done, 0);
+            waited = true;
+            if (e == hipSuccess) e = braid_index_build(keep[k], take[k]->m, take[k]->n, mem, scratch, S->ix_stream,
+                                                       &ixview[k]);
+            ixmem[k] = mem;
+            job.braids.push_back(take[k]->id);
+            job.release_when_done.push_back(scratch);
+        }
+        if (e == hipSuccess && waited) e = hipEventCreateWithFlags(&job.done, hipEventDisableTiming);
+        if (e == hipSuccess && waited) e = hipEventRecord(job.done, S->ix_stream);
         lk.lock();
         if (e != hipSuccess) {
             S->fatal = std::string("request loop launch failed: ") + hipGetErrorString(e);
             for (int32_t *p : keep) S->arena.release(p);
+            for (void *p : ixmem) S->arena.release(p);
+            for (void *p : job.release_when_done) S->arena.release(p);
+            if (job.done) (void)hipEventDestroy(job.done);
             batch_destroy_internal(bt);
             return false;
+        }
+        int64_t job_id = 0;
+        if (waited) {
+            job_id = S->next_ix_job++;
+            S->ix_jobs.emplace(job_id, std::move(job));
         }
         for (size_t k = 0; k < take.size(); ++k) {
             S->cmp_queue.pop_front();
             Braid &b = S->braids[take[k]->id];
             b.d_rc = keep[k];
+            if (ixmem[k]) {
+                b.d_ix = ixmem[k];
+                b.ix = ixview[k];
+                b.ix_ready = false;
+                b.ix_job = job_id;
+            }
             b.m = take[k]->m;
             b.n = take[k]->n;
             take[k]->state = kRunning;
@@ -504,7 +612,9 @@ static bool launch_queries(Server *S, Flight &F, std::unique_lock<std::mutex> &l
     if (take.empty()) return resolved;
     for (size_t k = 0; k < take.size(); ++k) {
         const Braid &b = S->braids[take[k]->braid];
-        F.h_qd[k] = QueryDesc{b.d_rc, b.m, b.n, take[k]->kind, take[k]->qi, take[k]->qj, 0};
+        F.h_qd[k] = QueryDesc{b.d_rc, b.m, b.n, take[k]->kind, take[k]->qi, take[k]->qj, 0,
+                              b.ix_ready ? b.ix : BraidIndexView{}};
+        if (b.ix_ready && b.ix.levels > 0) ++S->n_index_queries;
         take[k]->state = kRunning;
     }
     F.busy = true;
@@ -540,6 +650,7 @@ static void run_loop(Server *S) {
                 break;
             }
         }
+        if (!S->ix_jobs.empty() && complete_index_jobs(S)) progress = true;
         if (progress) S->done_cv.notify_all();
         if (!S->fatal.empty()) break;
         // queries first (short), then comparisons, each on a free flight
@@ -581,6 +692,10 @@ static void server_free(Server *S) {
         if (F.done) (void)hipEventDestroy(F.done);
         if (F.stream) (void)hipStreamDestroy(F.stream);
     }
+    if (S->ix_stream) (void)hipStreamSynchronize(S->ix_stream);
+    for (auto &kv : S->ix_jobs)
+        if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+    if (S->ix_stream) (void)hipStreamDestroy(S->ix_stream);
     for (auto &kv : S->reqs) delete kv.second;
     delete S;
 }
@@ -610,6 +725,7 @@ int sl_server_create(sl_server **out, int64_t pool_bytes, int max_inflight, int 
     S->max_batch = max_batch > 0 ? max_batch : 1024;
     S->max_batch_cells = max_batch_cells > 0 ? max_batch_cells : (int64_t)4e10;
     hipError_t e = S->arena.init(cap);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->ix_stream, hipStreamNonBlocking);
     S->flights.resize((size_t)S->max_inflight);
     for (auto &F : S->flights) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&F.stream, hipStreamNonBlocking);
@@ -816,7 +932,20 @@ int sl_server_stats(sl_server *h, sl_server_stats_t *st) {
     int64_t resident = 0;
     for (auto &kv : S->braids) resident += kv.second.state == kBraidResident;
     st->resident_braids = resident;
+    st->indexed_braids = S->n_indexed;
+    st->index_queries = S->n_index_queries;
     st->busy_seconds = S->any_submit ? std::chrono::duration<double>(S->t_last_done - S->t_first_submit).count() : 0;
+    return kOk;
+}
+
+int sl_server_set_index(sl_server *h, int64_t min_size) {
+    Server *S = reinterpret_cast<Server *>(h);
+    if (!S) {
+        set_error("sl_server_set_index: null handle");
+        return kErrArg;
+    }
+    std::lock_guard<std::mutex> g(S->mu);
+    S->index_min = min_size;
     return kOk;
 }
 

@@ -2,6 +2,8 @@
 micro-batches, admission by device memory, resident braids and follow-up
 queries.  Every result must equal the one-shot entry point's on the same
 input (bit-exact: the braid and the LCS values are integers)."""
+import time
+
 import numpy as np
 import pytest
 
@@ -25,11 +27,24 @@ def _pairs(g, count, median, lo, hi, sigma):
     return out
 
 
-def test_results_in_the_mix_equal_results_alone(gpu, oracle):
+@pytest.mark.parametrize("index_min", [0, 2000])
+def test_results_in_the_mix_equal_results_alone(gpu, oracle, index_min):
+    # index_min 0: every query scans its braid; 2000: braids of m+n >= 2000
+    # get a wavelet-matrix index and their queries take the index path
     g = _rng(101)
     pairs = _pairs(g, 96, 1500, 20, 30000, 4)
     srv = gpu.Server(pool_bytes=1 << 30, max_inflight=4, max_batch=12)
+    srv.set_index(index_min)
     ids = [srv.submit_compare(a, b) for a, b in pairs]   # a burst: micro-batches overlap
+    big = sum(len(a) + len(b) >= index_min for a, b in pairs)
+    if index_min > 0:
+        # index builds run after the results: wait until every large braid has one
+        for rid in ids:
+            srv.result(rid)
+        t0 = time.time()
+        while srv.stats()["indexed_braids"] < big and time.time() - t0 < 30:
+            time.sleep(0.001)
+        assert srv.stats()["indexed_braids"] == big > 0
     queries = []
     for k in range(80):
         t = int(g.integers(0, len(pairs)))
@@ -63,6 +78,11 @@ def test_results_in_the_mix_equal_results_alone(gpu, oracle):
     st = srv.stats()
     assert st["compares_done"] == len(pairs) and st["queries_done"] == len(queries)
     assert st["launches"] > len(pairs) // 12     # several micro-batches
+    if index_min > 0:
+        on_big = sum(len(pairs[t][0]) + len(pairs[t][1]) >= index_min for _, t, _, _, _ in queries)
+        assert st["indexed_braids"] == big and st["index_queries"] == on_big > 0
+    else:
+        assert st["indexed_braids"] == 0 and st["index_queries"] == 0
     srv.close()
 
 
