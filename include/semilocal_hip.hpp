@@ -24,6 +24,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <cmath>
 #include <vector>
 
 #include "semilocal_hip.h"
@@ -202,6 +203,37 @@ void hybrid_iterative(Perm &perm, const Input *a, int m, const Input *b, int n, 
     std::vector<int32_t> rc((size_t)(m + n));
     check(sl_hybrid_iterative(sa.ptr, m, sb.ptr, n, sa.bytes, small_m, small_n, rc.data()));
     detail::store(perm, rc);
+}
+
+// Block counts of semi_local::hybrid_iterative_wrapper (semi_local.h:
+// 742-788), so the same call combs the same blocks: rows in equal blocks of
+// at most 32000, columns split over threads_num and further while a block
+// would exceed 64000 - rows.
+inline void hybrid_wrapper_blocks(int m, int n, int threads_num, int *small_m, int *small_n) {
+    const int threads = threads_num > 0 ? threads_num : 1;
+    int sm = 1, rows = m;
+    if (m > 32000) {
+        const int parts = (int)std::ceil((double)m / 32000.0);
+        rows = (int)std::ceil((double)m / parts);
+        sm = (int)std::ceil((double)m / rows);
+    }
+    int sn = threads;
+    if (n > 32000) {
+        const int per_thread = (int)std::ceil((double)n / threads);
+        const int room = 64000 - rows;
+        if (per_thread >= room) sn = threads * (int)std::ceil((double)per_thread / room);
+    }
+    *small_m = sm;
+    *small_n = sn;
+}
+
+// semi_local::hybrid_iterative_wrapper<WithIf> -- semi_local.h:742-788.
+template <bool WithIf, class Perm, class Map>
+void hybrid_iterative_wrapper(Perm &perm, const unsigned short *a, int m, const unsigned short *b, int n, Map &map,
+                              int threads_num = 1) {
+    int small_m = 1, small_n = 1;
+    hybrid_wrapper_blocks(m, n, threads_num, &small_m, &small_n);
+    hybrid_iterative<unsigned short, WithIf>(perm, a, m, b, n, map, small_m, small_n, threads_num);
 }
 
 // distance_unit_monge_product::steady_ant::steady_ant_parallel_wrapper --

@@ -3,6 +3,8 @@
 ``python -m semilocal_amd.build`` compiles every ``csrc/*.hip`` file into
 ``semilocal_amd/libsemilocal_hip.so``.  The library is the product: host C++
 and the hand-written HIP kernels behind the C ABI in include/semilocal_hip.h.
+``build_cli()`` then links the command-line drivers (``cli/*.cpp``, host C++
+over the C ABI) into ``semilocal_amd/bin/``.
 """
 from __future__ import annotations
 
@@ -44,5 +46,25 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
     return LIB
 
 
+def build_cli(verbose: bool = False) -> list[str]:
+    """Links every ``cli/*.cpp`` driver against the library into ``bin/``
+    (rpath ``$ORIGIN/..``, so the drivers find the in-tree library)."""
+    out_dir = os.path.join(PKG, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    built = []
+    for src in sorted(glob.glob(os.path.join(PKG, "cli", "*.cpp"))):
+        exe = os.path.join(out_dir, os.path.splitext(os.path.basename(src))[0])
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-march=x86-64-v3", "-I", os.path.join(ROOT, "include"),
+               "-o", exe + ".tmp", src, "-L", PKG, "-lsemilocal_hip", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+        built.append(exe)
+    return built
+
+
 if __name__ == "__main__":
     build(verbose=True, extra=sys.argv[1:])
+    build_cli(verbose=True)

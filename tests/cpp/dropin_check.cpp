@@ -6,7 +6,8 @@
 // Every check calls a reference entry point and the same call site switched
 // to the engine, on the same seeded input, and compares the results:
 //   flat API:   semi_local::sticky_braid_mpi / sticky_braid_sequential /
-//               hybrid_iterative, steady_ant::steady_ant_parallel_wrapper,
+//               hybrid_iterative / hybrid_iterative_wrapper,
+//               steady_ant::steady_ant_parallel_wrapper,
 //               steady_ant::staggered_sticky_multiplication<RowGlue, false>,
 //               binary::llcs_2symbol_smart_combing<uint64_t / uint32_t>
 //   namespaced: semi_local::OpenMPIterativeCombing, OpenMPHybridIterative,
@@ -15,6 +16,7 @@
 // Prints one line per check and "dropin_check: N checks, M failures"; exits
 // non-zero on any failure.  Needs a GPU to run.
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -130,6 +132,20 @@ int main() {
         semilocal_hip::hybrid_iterative<int, false>(hip, a.data(), m, b.data(), n, map, shape.first, shape.second, 4);
         std::snprintf(name, sizeof name, "hybrid_iterative<int,false> 3000x2500 %dx%d blocks", shape.first,
                       shape.second);
+        report(name, same(ref, hip));
+    }
+
+    // ---- flat API: hybrid_iterative_wrapper (the block heuristic) ---------
+    // shapes reach every branch: one row block and threads_num column blocks;
+    // two row blocks with columns per thread below, and above, 64000 - rows
+    for (auto sh : {std::array<int, 3>{3000, 2500, 4}, {33000, 40000, 2}, {33000, 70000, 1}}) {
+        const int m = sh[0], n = sh[1], thr = sh[2];
+        auto a32 = symbols(g, m, 4), b32 = symbols(g, n, 4);
+        std::vector<unsigned short> a(a32.begin(), a32.end()), b(b32.begin(), b32.end());
+        Permutation ref(m + n, m + n), hip(m + n, m + n);
+        semi_local::hybrid_iterative_wrapper<false>(ref, a.data(), m, b.data(), n, map, thr);
+        semilocal_hip::hybrid_iterative_wrapper<false>(hip, a.data(), m, b.data(), n, map, thr);
+        std::snprintf(name, sizeof name, "hybrid_iterative_wrapper<false> %dx%d, %d threads", m, n, thr);
         report(name, same(ref, hip));
     }
 
