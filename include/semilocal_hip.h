@@ -328,6 +328,42 @@ int sl_server_release(sl_server *s, int64_t braid);
 int sl_server_stats(sl_server *s, sl_server_stats_t *stats);
 void sl_server_destroy(sl_server *s);
 
+/* ------------------------------------------------------------------------ */
+/* Approximate matching over resident braids.                                 */
+/* The pattern p is combed against overlapping chunks of the text (one ragged */
+/* batch launch; every substring of length <= max_window lies inside one      */
+/* chunk) and the chunk braids stay in HBM.  Scores are those of the          */
+/* reference's duplicate searches (match 0, mismatch -2, gap -1):             */
+/*   score(s, e) = 2 LCS(p, text[s, e)) - m - (e - s)   (-indel distance).    */
+/* ------------------------------------------------------------------------ */
+typedef struct sl_amatch sl_amatch;
+
+/* chunk = windows per chunk (0: a default from the text length and the CU
+ * count).  m, n >= 1; sym_bytes 1, 2 or 4. */
+int sl_amatch_create(sl_amatch **out, const void *p, int m, const void *text, int n, int sym_bytes,
+                     int max_window, int chunk);
+/* Chunks, windows per chunk, combed cells (m x chunk text lengths). */
+int sl_amatch_stats(const sl_amatch *am, int *chunks, int *chunk_windows, int64_t *cells);
+/* For every window [i, i+w), i = 0 .. n-w (n-w+1 entries per array): the
+ * window's own score, and its best substring [best_start, best_end) of length
+ * >= min_len: the highest score, then the longest, then the rightmost -- the
+ * window maximum of InteractiveDuplicateSearchSemiDense / SemiSparse
+ * (approximate_matching/ApproximateMatching.h:158-207, 373-413).  Needs
+ * 1 <= w <= min(n, max_window), 0 <= min_len <= w, w - min_len < 4096. */
+int sl_amatch_windows(sl_amatch *am, int w, int min_len, int32_t *window_score, int32_t *best_start,
+                      int32_t *best_end, int32_t *best_score);
+/* HIP-event times per run: re-combing every chunk, and the window kernel. */
+int sl_amatch_time(sl_amatch *am, int w, int min_len, int warmup, int iters, double *comb_ms,
+                   double *windows_ms);
+/* Replaces: InteractiveDuplicateSearchSemiDense::find (variant 0,            */
+/*   ApproximateMatching.h:339-432) and InteractiveDuplicateSearchSemiSparse::*/
+/*   find (variant 1, :252-336) -- (p, p_size, text, text_size, k, result).   */
+/* Needs 0 < k <= 1 and int(m / k) <= max_window.  *count receives the number */
+/* of intervals; sl_duplicate_results copies them (start, end, score).        */
+int sl_duplicate_search(sl_amatch *am, double k, int variant, int *count);
+int sl_duplicate_results(const sl_amatch *am, int32_t *start, int32_t *end, double *score);
+void sl_amatch_destroy(sl_amatch *am);
+
 #ifdef __cplusplus
 }
 #endif

@@ -210,3 +210,65 @@ class Reference:
         o = Oracle()
         ar, bw = o.pack_bits_reverse(a_bits), o.pack_bits(b_bits)
         return self.L.ref_llcs_2symbol_smart(ar, len(ar), bw, len(bw), len(a_bits), threads, int(formula_one))
+
+
+AMATCH_SO = os.path.join(HERE, "libamatch.so")
+REF_AMATCH_SO = os.path.join(HERE, "_ref", "libref_amatch.so")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def _dup_call(fn, variant, p, t, k):
+    p, t = _i32(p), _i32(t)
+    cap = max(16, len(t))
+    s, e, sc = np.empty(cap, np.int32), np.empty(cap, np.int32), np.empty(cap, np.float64)
+    cnt = fn(variant, p, len(p), t, len(t), C.c_double(k), s, e, sc, cap)
+    if cnt < 0:
+        raise ValueError("bad duplicate-search arguments")
+    if cnt > cap:
+        raise RuntimeError("more intervals than the buffer holds")
+    return [(int(s[i]), int(e[i]), float(sc[i])) for i in range(cnt)]
+
+
+class AMatchOracle:
+    """The restatement of the reference's duplicate searches (amatch.cpp):
+    window values by direct LCS DP, the sequential phases as the reference
+    runs them."""
+
+    def __init__(self, path: str = AMATCH_SO):
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        self.L = L
+        L.orc_am_windows.argtypes = [_i32p, C.c_int, _i32p, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p]
+        L.orc_dup_search.argtypes = [C.c_int, _i32p, C.c_int, _i32p, C.c_int, C.c_double, _i32p, _i32p, _f64p,
+                                     C.c_int]
+
+    def windows(self, p, t, w, min_len):
+        """(window score, best start, best end, best score) per window."""
+        p, t = _i32(p), _i32(t)
+        nw = len(t) - w + 1
+        out = [np.empty(nw, np.int32) for _ in range(4)]
+        if self.L.orc_am_windows(p, len(p), t, len(t), w, min_len, *out):
+            raise ValueError("bad window arguments")
+        return tuple(out)
+
+    def dup_search(self, p, t, k, variant):
+        return _dup_call(self.L.orc_dup_search, variant, p, t, k)
+
+
+class RefAMatch:
+    """The reference's own duplicate searches, compiled in place."""
+
+    def __init__(self, path: str = REF_AMATCH_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.L = C.CDLL(path)
+        self.L.ref_dup_search.argtypes = [C.c_int, _i32p, C.c_int, _i32p, C.c_int, C.c_double, _i32p, _i32p, _f64p,
+                                          C.c_int]
+
+    @staticmethod
+    def available(path: str = REF_AMATCH_SO) -> bool:
+        return os.path.exists(path)
+
+    def dup_search(self, p, t, k, variant):
+        return _dup_call(self.L.ref_dup_search, variant, p, t, k)

@@ -69,6 +69,14 @@ SIGNATURES = {
                              C.c_int),
     "sl_braid_index_stats": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int64)], C.c_int),
     "sl_braid_index_destroy": ([_vp], None),
+    "sl_amatch_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "sl_amatch_stats": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)], C.c_int),
+    "sl_amatch_windows": ([_vp, C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p], C.c_int),
+    "sl_amatch_time": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+                       C.c_int),
+    "sl_duplicate_search": ([_vp, C.c_double, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "sl_duplicate_results": ([_vp, _vp, _vp, _vp], C.c_int),
+    "sl_amatch_destroy": ([_vp], None),
     "sl_batch_destroy": ([_vp], None),
     "sl_llcs_bitparallel": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "sl_bitpar_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int], C.c_int),
@@ -251,6 +259,76 @@ class BraidIndex:
             self.close()
         except Exception:
             pass
+
+
+class AMatch:
+    """Approximate matching of a pattern against a text (sl_amatch_*): the
+    pattern is combed against overlapping chunks of the text in one batched
+    launch and the chunk braids stay in HBM; window queries and the
+    reference's duplicate searches run on them."""
+
+    SEMI_DENSE, SEMI_SPARSE = 0, 1
+
+    def __init__(self, pattern, text, max_window: int, chunk: int = 0):
+        p_arr, wp = as_symbols(pattern)
+        t_arr, wt = as_symbols(text)
+        if wp != wt:
+            p_arr, wp = np.ascontiguousarray(p_arr, dtype=np.int32), 4
+            t_arr, wt = np.ascontiguousarray(t_arr, dtype=np.int32), 4
+        self.m, self.n = len(p_arr), len(t_arr)
+        h = _vp()
+        check(lib().sl_amatch_create(C.byref(h), _ptr(p_arr) if self.m else None, self.m,
+                                     _ptr(t_arr) if self.n else None, self.n, wp, max_window, chunk))
+        self.h = h
+
+    def stats(self):
+        c, b, cells = C.c_int(), C.c_int(), C.c_int64()
+        check(lib().sl_amatch_stats(self.h, C.byref(c), C.byref(b), C.byref(cells)))
+        return {"chunks": c.value, "chunk_windows": b.value, "cells": cells.value}
+
+    def windows(self, w: int, min_len: int):
+        """Per window [i, i+w): its score and its best substring of length
+        >= min_len -- (window_score, best_start, best_end, best_score)."""
+        nw = max(self.n - w + 1, 1)
+        out = [np.empty(nw, np.int32) for _ in range(4)]
+        check(lib().sl_amatch_windows(self.h, w, min_len, *out))
+        return tuple(o[:self.n - w + 1] for o in out)
+
+    def time(self, w: int, min_len: int, warmup: int = 1, iters: int = 5):
+        comb, win = C.c_double(), C.c_double()
+        check(lib().sl_amatch_time(self.h, w, min_len, warmup, iters, C.byref(comb), C.byref(win)))
+        return comb.value, win.value
+
+    def duplicate_search(self, k: float, variant: int = 0):
+        """InteractiveDuplicateSearchSemiDense (0) / SemiSparse (1): list of
+        (start, end, score)."""
+        cnt = C.c_int()
+        check(lib().sl_duplicate_search(self.h, float(k), variant, C.byref(cnt)))
+        n = cnt.value
+        s, e, sc = np.empty(max(n, 1), np.int32), np.empty(max(n, 1), np.int32), np.empty(max(n, 1), np.float64)
+        check(lib().sl_duplicate_results(self.h, s.ctypes.data, e.ctypes.data, sc.ctypes.data))
+        return [(int(s[i]), int(e[i]), float(sc[i])) for i in range(n)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sl_amatch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def duplicate_search(pattern, text, k: float, variant: int = 0):
+    """One-shot duplicate search (the reference's
+    InteractiveDuplicateSearchSemiDense / SemiSparse ::find)."""
+    am = AMatch(pattern, text, int(len(pattern) / k))
+    try:
+        return am.duplicate_search(k, variant)
+    finally:
+        am.close()
 
 
 class Batch:

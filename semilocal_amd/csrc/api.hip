@@ -654,12 +654,18 @@ int batch_create_arena(Batch **out, int npairs, const void *a_cat, const int64_t
                        DeviceArena *arena, hipStream_t stream) {
     return batch_create(out, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes, 0, 0, arena, stream);
 }
+int batch_create_flags_internal(Batch **out, int npairs, const void *a_cat, const int64_t *a_off,
+                                const int32_t *a_len, const void *b_cat, const int64_t *b_off, const int32_t *b_len,
+                                int sym_bytes, int flags) {
+    return batch_create(out, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes, flags, 0);
+}
 int batch_launch_internal(Batch *bt) { return batch_run(bt); }
 int batch_launch_replayable_internal(Batch *bt) { return batch_run_replayable(bt); }
 int batch_wait_internal(Batch *bt) { return batch_sync(bt); }
 void batch_destroy_internal(Batch *bt) { batch_free(bt); }
 hipStream_t batch_stream_internal(const Batch *bt) { return bt->stream; }
 const int32_t *batch_rc_internal(const Batch *bt) { return bt->d_rc; }
+const int32_t *batch_cr_internal(const Batch *bt) { return bt->d_cr; }
 int64_t batch_perm_off_internal(const Batch *bt, int pair) { return bt->perm_off[(size_t)pair]; }
 int64_t batch_cells_internal(const Batch *bt) { return bt->cells; }
 

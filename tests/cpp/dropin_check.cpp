@@ -12,7 +12,8 @@
 //               binary::llcs_2symbol_smart_combing<uint64_t / uint32_t>
 //   namespaced: semi_local::OpenMPIterativeCombing, OpenMPHybridIterative,
 //               common::SequentialMemoryOptimizedStickBraidMultiplication,
-//               lcs::LCSBySemiLocal over either combing strategy.
+//               lcs::LCSBySemiLocal over either combing strategy,
+//               InteractiveDuplicateSearchSemiDense / SemiSparse::find.
 // Prints one line per check and "dropin_check: N checks, M failures"; exits
 // non-zero on any failure.  Needs a GPU to run.
 #include <algorithm>
@@ -21,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <tuple>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -30,8 +32,11 @@
 #include "prefix_lcs/bitwise/transposition_network_binary_alphabet.h"
 #include "include/LCS/LCS.h"
 #include "include/SemiLocal/SemiLocalLCS.h"
+#include "approximate_matching/utils.h"
+#include "approximate_matching/ApproximateMatching.h"
 
 #include "semilocal_hip_strategies.hpp"
+#include "semilocal_hip_amatch.hpp"
 
 static int checks = 0, failures = 0;
 
@@ -271,6 +276,28 @@ int main() {
         dev.multiply(p, q, hip);
         std::snprintf(name, sizeof name, "SequentialMemoryOptimizedStickBraidMultiplication vs HipStickyBraid n=%d", n);
         report(name, same(ref, hip));
+    }
+
+    // duplicate searches: pattern vs a text holding mutated copies of it
+    for (auto [m, n, k, sigma] : std::vector<std::tuple<int, int, double, int>>{
+             {12, 400, 0.8, 4}, {30, 2000, 0.9, 4}, {25, 1500, 0.6, 2}, {60, 5000, 0.95, 8}}) {
+        auto p = symbols(g, m, sigma), t = symbols(g, n, sigma);
+        for (int c = 0; c < 6; ++c) {
+            const int pos = (int)(g() % (uint64_t)(n - m));
+            for (int i = 0; i < m; ++i) t[(size_t)(pos + i)] = (g() % 8 == 0) ? (int)(g() % (uint64_t)sigma) : p[(size_t)i];
+        }
+        using Iv = approximate_matching::utils::Interval;
+        std::vector<Iv> ref_d, hip_d, ref_s, hip_s;
+        InteractiveDuplicateSearchSemiDense().find(p.data(), m, t.data(), n, k, ref_d);
+        semilocal_hip::InteractiveDuplicateSearchSemiDense().find(p.data(), m, t.data(), n, k, hip_d);
+        InteractiveDuplicateSearchSemiSparse().find(p.data(), m, t.data(), n, k, ref_s);
+        semilocal_hip::InteractiveDuplicateSearchSemiSparse().find(p.data(), m, t.data(), n, k, hip_s);
+        std::snprintf(name, sizeof name, "InteractiveDuplicateSearchSemiDense m=%d n=%d k=%.2f (%zu found)", m, n, k,
+                      ref_d.size());
+        report(name, ref_d == hip_d);
+        std::snprintf(name, sizeof name, "InteractiveDuplicateSearchSemiSparse m=%d n=%d k=%.2f (%zu found)", m, n, k,
+                      ref_s.size());
+        report(name, ref_s == hip_s);
     }
 
     std::printf("dropin_check: %d checks, %d failures\n", checks, failures);
