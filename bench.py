@@ -198,6 +198,82 @@ def bench_index(S, W):
             "parity": "index answers equal scan answers" if same else "MISMATCH"}
 
 
+def amatch_workload(n: int = 1_000_000, m: int = 1_000):
+    """Pattern of 1,000 symbols (alphabet 4) and a text of n symbols holding
+    n / 3,000 mutated copies of it (10% substitutions), seed 21."""
+    g = np.random.default_rng(21)
+    p = g.integers(0, 4, m).astype(np.uint8)
+    t = g.integers(0, 4, n).astype(np.uint8)
+    for pos in g.integers(0, n - m, n // 3000):
+        c = p.copy()
+        sub = g.random(m) < 0.10
+        c[sub] = g.integers(0, 4, int(np.count_nonzero(sub))).astype(np.uint8)
+        t[pos:pos + m] = c
+    return p, t
+
+
+def bench_amatch(S, k: float = 0.9):
+    """Approximate matching: InteractiveDuplicateSearchSemiDense's windows on a
+    1,000 x 1,000,000 pattern/text, k = 0.9 (w = 1,111, substrings of length
+    >= 900: 212 columns per window triangle)."""
+    p, t = amatch_workload()
+    m, n = len(p), len(t)
+    w, L = int(m / k), int(m * k)
+    t0 = time.perf_counter()
+    am = S.AMatch(p, t, w)
+    create_s = time.perf_counter() - t0
+    st = am.stats()
+    comb_ms, win_ms = am.time(w, L, warmup=1, iters=5)
+
+    def timed(variant, reps=5):
+        am.duplicate_search(k, variant)                 # warm: pinned staging, host sets
+        walls, splits = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            found = am.duplicate_search(k, variant)
+            walls.append((time.perf_counter() - t0) * 1e3)
+            splits.append(am.last_times())
+        i = int(np.argsort(walls)[reps // 2])          # the median call
+        return found, walls[i], splits[i]
+
+    res, dense_ms, dense_split = timed(0)
+    res_sparse, sparse_ms, sparse_split = timed(1)
+    am.close()
+    nw = n - w + 1
+    S_cols = w - L + 1
+    return {"workload": f"pattern {m} x text {n:,} (alphabet 4, {n // 3000} planted copies with 10% "
+                        f"substitutions), k = {k}: w = {w}, min length {L}",
+            "chunks": st["chunks"], "chunk_windows": st["chunk_windows"], "combed_cells": st["cells"],
+            "create_s_incl_upload": round(create_s, 3),
+            "comb_ms": round(comb_ms, 3), "comb_gcups": round(st["cells"] / (comb_ms * 1e-3) / 1e9, 1),
+            "windows_ms": round(win_ms, 3), "windows": nw,
+            "window_column_updates_per_s": round(nw * S_cols / (win_ms * 1e-3), 1),
+            "search_timing": "median of 5 warm calls, wall time of the C entry point incl. host phases",
+            "semi_dense_search_ms_incl_host": round(dense_ms, 2), "semi_dense_found": len(res),
+            "semi_dense_device_and_copy_ms": round(dense_split[0], 2), "semi_dense_host_phases_ms":
+                round(dense_split[1], 2),
+            "semi_sparse_search_ms_incl_host": round(sparse_ms, 2), "semi_sparse_found": len(res_sparse),
+            "semi_sparse_device_and_copy_ms": round(sparse_split[0], 2), "semi_sparse_host_phases_ms":
+                round(sparse_split[1], 2)}
+
+
+def cpu_baseline_amatch(k: float = 0.9, n_sample: int = 300_000):
+    """The reference's InteractiveDuplicateSearchSemiDense::find (compiled in
+    place, one thread) on the first n_sample symbols of the same text."""
+    from oracle.binding import AMatchOracle, RefAMatch
+
+    p, t = amatch_workload()
+    t = t[:n_sample]
+    eng, kind = (RefAMatch(), "reference") if RefAMatch.available() else (AMatchOracle(), "port")
+    t0 = time.perf_counter()
+    eng.dup_search(p, t, k, 0)
+    dt = time.perf_counter() - t0
+    nw = n_sample - int(len(p) / k) + 1
+    return {"value": round(nw / dt, 1), "unit": "windows/s", "cores": 1, "kind": kind,
+            "sample": f"text prefix of {n_sample:,} symbols ({nw:,} windows), {dt:.2f} s incl. its "
+                      f"sequential combing"}
+
+
 def bench_hybrid(S, W, golden):
     """Config 3: 8x8 blocks combed in one launch, merged by 6 levels of batched
     steady-ant multiplications."""
@@ -287,7 +363,7 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
     ap.add_argument("--no-extras", action="store_true", help="only the ragged batch (configs 1, 2, 3, 5 skipped)")
-    ap.add_argument("--only", choices=["plain", "binary", "hybrid", "multiply", "index", "serving"],
+    ap.add_argument("--only", choices=["plain", "binary", "hybrid", "multiply", "index", "serving", "amatch"],
                     help="run one extra section alone (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
@@ -311,6 +387,7 @@ def main():
               "hybrid": lambda: bench_hybrid(semilocal_amd, W, golden),
               "multiply": lambda: bench_multiply(semilocal_amd),
               "index": lambda: bench_index(semilocal_amd, W),
+              "amatch": lambda: bench_amatch(semilocal_amd),
               "serving": lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
                                                args.serve_pool_gb)}[args.only]
         print(json.dumps({"section": args.only, **fn()}), flush=True)
@@ -417,6 +494,7 @@ def main():
                         ("similar_pair_1e6_hybrid", lambda: bench_hybrid(semilocal_amd, W, golden)),
                         ("braid_multiply_2p20", lambda: bench_multiply(semilocal_amd)),
                         ("braid_index_queries", lambda: bench_index(semilocal_amd, W)),
+                        ("approximate_matching_1e6", lambda: bench_amatch(semilocal_amd)),
                         ("serving", lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
                                                           args.serve_pool_gb))):
             t0 = time.perf_counter()
@@ -426,6 +504,9 @@ def main():
         cb = cpu_baseline(pairs)
         log("cpu baseline:", cb)
         out["cpu_baseline"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in cb.items() if k != "seconds"}
+        if "approximate_matching_1e6" in out:
+            out["approximate_matching_1e6"]["cpu_reference"] = cpu_baseline_amatch()
+            log("approximate matching cpu reference:", out["approximate_matching_1e6"]["cpu_reference"])
     print(json.dumps(out), flush=True)
 
 

@@ -362,6 +362,9 @@ int sl_amatch_time(sl_amatch *am, int w, int min_len, int warmup, int iters, dou
 /* of intervals; sl_duplicate_results copies them (start, end, score).        */
 int sl_duplicate_search(sl_amatch *am, double k, int variant, int *count);
 int sl_duplicate_results(const sl_amatch *am, int32_t *start, int32_t *end, double *score);
+/* Wall time of the last sl_duplicate_search: window kernel + copy to host,
+ * then the sequential phases on the host. */
+int sl_amatch_last_times(const sl_amatch *am, double *device_ms, double *host_ms);
 void sl_amatch_destroy(sl_amatch *am);
 
 #ifdef __cplusplus

@@ -76,6 +76,7 @@ SIGNATURES = {
                        C.c_int),
     "sl_duplicate_search": ([_vp, C.c_double, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "sl_duplicate_results": ([_vp, _vp, _vp, _vp], C.c_int),
+    "sl_amatch_last_times": ([_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "sl_amatch_destroy": ([_vp], None),
     "sl_batch_destroy": ([_vp], None),
     "sl_llcs_bitparallel": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
@@ -308,6 +309,13 @@ class AMatch:
         s, e, sc = np.empty(max(n, 1), np.int32), np.empty(max(n, 1), np.int32), np.empty(max(n, 1), np.float64)
         check(lib().sl_duplicate_results(self.h, s.ctypes.data, e.ctypes.data, sc.ctypes.data))
         return [(int(s[i]), int(e[i]), float(sc[i])) for i in range(n)]
+
+    def last_times(self):
+        """(device ms, host ms) of the last duplicate_search: window kernel +
+        copy to host, then the sequential phases."""
+        d, h = C.c_double(), C.c_double()
+        check(lib().sl_amatch_last_times(self.h, C.byref(d), C.byref(h)))
+        return d.value, h.value
 
     def close(self):
         if getattr(self, "h", None):

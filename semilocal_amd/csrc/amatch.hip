@@ -29,10 +29,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
-#include <unordered_set>
 #include <vector>
 
 #include "common.h"
@@ -65,13 +65,31 @@ struct AmItem {
     int32_t gbase;      // text offset of the chunk (global index of its window 0)
 };
 
+// One step of a 64-bit max over DPP-moved lanes; rows outside ROW_MASK (and
+// lanes without a source) keep their own value.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_max_step(uint64_t v) {
+    const int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+    const uint32_t tlo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+    const uint32_t thi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+    const uint64_t t = ((uint64_t)thi << 32) | tlo;
+    return t > v ? t : v;
+}
+
+// Max over the 64 lanes, returned to every lane: quad swaps, half-row and
+// row mirrors leave each 16-lane row's max in all its lanes; row_bcast:15 and
+// row_bcast:31 carry it into rows 1 and 3, then 2 and 3, so lane 63 holds the
+// wave's max.  VALU moves only (no LDS round trips).
 __device__ inline uint64_t wave_max_u64(uint64_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint64_t u = __shfl_xor(v, o, 64);
-        v = u > v ? u : v;
-    }
-    return v;
+    v = dpp_max_step<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
+    v = dpp_max_step<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]
+    v = dpp_max_step<0x141, 0xf>(v);   // row_half_mirror
+    v = dpp_max_step<0x140, 0xf>(v);   // row_mirror
+    v = dpp_max_step<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v = dpp_max_step<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ inline int wave_sum_i32(int v) {
@@ -83,15 +101,27 @@ __device__ inline int wave_sum_i32(int v) {
 // Column slot q (0 <= q < S) holds the window column e with e = q (mod S):
 // the S columns of a window are consecutive, so the slots never collide and a
 // step (i -> i - 1) retires column i + w exactly where column i - 1 + L
-// enters.  Lane l holds slots l, l + 64, ..., in registers.
+// enters.  Lane l holds slots l, l + 64, ..., in registers: per slot its
+// column E, D = D(i, E) and best = ((score + bias) << len_bits) | (len - L),
+// the lexicographic max of (score, length) over the starts s in [i, E - L].
+// The reference keeps a start only if its score is higher, or equal with a
+// longer substring (DuplicateSearch::compare, ApproximateMatching.h:144-148);
+// across columns the window's best is the max of (score, length, end): its
+// loop visits ends from the right and keeps the first of equal (score,
+// length) (ApproximateMatching.h:392-405).
 //
-// Per slot: D = D(i, e) and best = ((score + bias) << len_bits) | (len - L),
-// the lexicographic max of (score, length) over the starts s in [i, e - L]:
-// the reference keeps a start only if its score is higher, or equal with a
-// longer substring (DuplicateSearch::compare, ApproximateMatching.h:144-148).
-// Across columns the window's best is the max of (score, length, end): its
-// loop visits ends from the right and keeps the first of equal (score, length)
-// (ApproximateMatching.h:392-405).
+// Two running diagonals live on the scalar unit:
+//   D(i, i+L) = D(i+1, i+1+L) + [P[i+m] < i+1+L] - [P^-1[i+L] >= i+m]   (entering column)
+//   D(i, i+w) = D(i+1, i+1+w) + [P[i+m] < i+1+w] - [P^-1[i+w] >= i+m]   (the window itself)
+constexpr int kNoColumn = 0x7fffffff;
+
+__device__ __forceinline__ int count_below(const int32_t *rc, int r0, int r1, int bound, int lane) {
+    int cnt = 0;
+#pragma unroll 8
+    for (int r = r0 + lane; r < r1; r += 64) cnt += rc[r] < bound ? 1 : 0;
+    return wave_sum_i32(cnt);
+}
+
 template <int SPT>
 __global__ __launch_bounds__(64) void am_windows(const int32_t *__restrict__ rc_all,
                                                  const int32_t *__restrict__ cr_all,
@@ -103,85 +133,109 @@ __global__ __launch_bounds__(64) void am_windows(const int32_t *__restrict__ rc_
     const int lane = threadIdx.x;
     const int32_t *rc = rc_all + it.perm_off;
     const int32_t *cr = cr_all + it.perm_off;
-    const int lo = it.lo, hi = it.hi;
+    const int lo = it.lo, hi = it.hi, nc = it.nc;
     // warm-up from i_start: the window hi - 1 needs its columns' starts from
     // e - L down to hi - 1, and its rightmost column (hi - 1 + w) enters at
     // i = hi - 1 + w - L.  i_start + L <= nc because hi - 1 <= nc - w.
     const int i_start = hi + S - 2;
-    // D(i_start, i_start + L) directly; then along the diagonal e = i + L:
-    //   D(i, i+L) = D(i+1, i+1+L) + [P[i+m] < i+1+L] - [P^-1[i+L] >= i+m]
-    int cnt = 0;
-    for (int r = i_start + m + lane; r < m + it.nc; r += 64) cnt += rc[r] < i_start + L ? 1 : 0;
-    int ddiag = wave_sum_i32(cnt);
+    int ddiag = count_below(rc, i_start + m, m + nc, i_start + L, lane);
+    int dwin = count_below(rc, hi - 1 + m, m + nc, hi - 1 + w, lane);
 
-    int D[SPT];
-    uint32_t best[SPT];
+    int E[SPT], D[SPT];
+    uint32_t B[SPT];
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
+        E[k] = kNoColumn;
         D[k] = 0;
-        best[k] = 0;
+        B[k] = 0;
     }
-    const uint32_t len_mask = (1u << len_bits) - 1u;
     int qe = (i_start + L) % S;   // slot of the column entering at step i
-    int pbuf = 0, cbuf = 0;
-    int r_score = 0, r_start = 0, r_end = 0, r_win = 0;
-    for (int i = i_start; i >= lo; --i) {
+    int pbuf = 0, cbuf = 0, wbuf = 0;
+
+    // warm-up: columns enter, no window is complete yet
+    for (int i = i_start; i >= hi; --i) {
         const int j = (i_start - i) & 63;
-        if (j == 0) {   // the next 64 steps' P[i+m] and P^-1[i+L], one per lane
+        if (j == 0) {
             // P[i_start + m] is never used (it feeds step i_start - 1 onwards)
             // and lies past the braid when L = 0 and i_start = nc
             const int src = i - lane;
-            pbuf = (src >= lo && src < it.nc) ? rc[src + m] : 0;
-            cbuf = src >= lo ? cr[src + L] : 0;
+            pbuf = (src >= hi && src < nc) ? rc[src + m] : 0;
+            cbuf = src >= hi ? cr[src + L] : 0;
         }
         const int pv = __builtin_amdgcn_readlane(pbuf, j);
         const int cv = __builtin_amdgcn_readlane(cbuf, j);
         if (i < i_start) ddiag += (pv < i + 1 + L ? 1 : 0) - (cv >= i + m ? 1 : 0);
-        const int win_slot = qe == 0 ? S - 1 : qe - 1;   // column i + w
-        const int entered = i_start - i;                  // columns with off <= entered hold data
-        uint64_t kmax = 0;
-        int ws = 0;
+        const int A = bias - i - m, Bc = -i - L;
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             const int q = lane + 64 * k;
-            if (q < S) {
-                int off = q - qe;
-                off += off < 0 ? S : 0;   // e - i - L
-                const int elen = L + off;
-                const int d = off == 0 ? ddiag : D[k] + (pv < i + elen ? 1 : 0);
+            if (k < SPT / 2 || q < S) {   // S > 32 SPT: groups below SPT / 2 are full
+                const bool enter = q == qe;
+                const bool live = E[k] != kNoColumn;
+                const int e = enter ? i + L : E[k];
+                const int d = enter ? ddiag : D[k] + (pv < e ? 1 : 0);
+                const uint32_t cand = ((uint32_t)(e - 2 * d + A) << len_bits) | (uint32_t)(e + Bc);
+                if (enter || live) {
+                    E[k] = e;
+                    D[k] = d;
+                    B[k] = enter ? cand : (cand > B[k] ? cand : B[k]);
+                }
+            }
+        }
+        qe = qe == 0 ? S - 1 : qe - 1;
+    }
+
+    // windows hi - 1 .. lo: every slot holds a column of the window
+    const uint32_t len_mask = (1u << len_bits) - 1u;
+    int r_score = 0, r_start = 0, r_end = 0, r_win = 0;
+    for (int i = hi - 1; i >= lo; --i) {
+        const int idx = hi - 1 - i;
+        const int j = idx & 63;
+        if (j == 0) {
+            const int src = i - lane;
+            pbuf = src >= lo ? rc[src + m] : 0;
+            cbuf = src >= lo ? cr[src + L] : 0;
+            wbuf = src >= lo ? cr[src + w] : 0;
+        }
+        const int pv = __builtin_amdgcn_readlane(pbuf, j);
+        const int cv = __builtin_amdgcn_readlane(cbuf, j);
+        const int cw = __builtin_amdgcn_readlane(wbuf, j);
+        if (i < i_start) ddiag += (pv < i + 1 + L ? 1 : 0) - (cv >= i + m ? 1 : 0);
+        if (i < hi - 1) dwin += (pv < i + 1 + w ? 1 : 0) - (cw >= i + m ? 1 : 0);
+        const int A = bias - i - m, Bc = -i - L;
+        uint64_t kmax = 0;
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int q = lane + 64 * k;
+            if (k < SPT / 2 || q < S) {   // S > 32 SPT: groups below SPT / 2 are full
+                const bool enter = q == qe;
+                const int e = enter ? i + L : E[k];
+                const int d = enter ? ddiag : D[k] + (pv < e ? 1 : 0);
+                const uint32_t cand = ((uint32_t)(e - 2 * d + A) << len_bits) | (uint32_t)(e + Bc);
+                const uint32_t b = enter ? cand : (cand > B[k] ? cand : B[k]);
+                E[k] = e;
                 D[k] = d;
-                const int score = elen - 2 * d - m;
-                const uint32_t cand = ((uint32_t)(score + bias) << len_bits) | (uint32_t)off;
-                uint32_t b = off == 0 ? cand : (cand > best[k] ? cand : best[k]);
-                if (off > entered) b = 0;
-                best[k] = b;
-                const uint64_t key = b ? (((uint64_t)b << 32) | (uint32_t)elen) : 0ull;
+                B[k] = b;
+                const uint64_t key = ((uint64_t)b << 32) | (uint32_t)(e - i);
                 kmax = key > kmax ? key : kmax;
-                if (q == win_slot) ws = score;
             }
         }
         kmax = wave_max_u64(kmax);
-        ws = __builtin_amdgcn_readlane(ws, win_slot & 63);
-        if (i < hi) {
-            const int idx = hi - 1 - i;
-            const int slot = idx & 63;
-            if (lane == slot) {
-                const uint32_t b = (uint32_t)(kmax >> 32);
-                const int e = i + (int)(uint32_t)kmax;
-                r_score = (int)(b >> len_bits) - bias;
-                r_end = e;
-                r_start = e - ((int)(b & len_mask) + L);
-                r_win = ws;
-            }
-            if (slot == 63 || i == lo) {
-                if (lane <= slot) {
-                    const int li = hi - 1 - (idx - slot + lane);
-                    const int64_t g = (int64_t)it.gbase + li;
-                    win_score[g] = r_win;
-                    best_score[g] = r_score;
-                    best_start[g] = it.gbase + r_start;
-                    best_end[g] = it.gbase + r_end;
-                }
+        if (lane == j) {
+            const uint32_t b = (uint32_t)(kmax >> 32);
+            const int e = i + (int)(uint32_t)kmax;
+            r_score = (int)(b >> len_bits) - bias;
+            r_end = e;
+            r_start = e - ((int)(b & len_mask) + L);
+            r_win = w - 2 * dwin - m;
+        }
+        if (j == 63 || i == lo) {
+            if (lane <= j) {
+                const int64_t g = (int64_t)it.gbase + (hi - 1 - (idx - j + lane));
+                win_score[g] = r_win;
+                best_score[g] = r_score;
+                best_start[g] = it.gbase + r_start;
+                best_end[g] = it.gbase + r_end;
             }
         }
         qe = qe == 0 ? S - 1 : qe - 1;
@@ -215,18 +269,21 @@ struct AMatch {
     am::AmItem *d_items = nullptr;
     size_t items_cap = 0;
     int32_t *d_out = nullptr;   // window score, best start, best end, best score (4 x cap)
+    int32_t *h_out = nullptr;   // pinned host copy of d_out
     int64_t out_cap = 0;
     std::vector<am::AmItem> h_items;   // work list of the last (w, L) launched
     int items_w = -1, items_L = -1;
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<AmInterval> results;
     int64_t cells = 0;
+    double last_device_ms = 0, last_host_ms = 0;   // last search: kernel + copy, then the host phases
 };
 
 static void amatch_free(AMatch *a) {
     if (!a) return;
     (void)hipFree(a->d_items);
     (void)hipFree(a->d_out);
+    if (a->h_out) (void)hipHostFree(a->h_out);
     for (auto &e : a->ev)
         if (e) (void)hipEventDestroy(e);
     if (a->batch) batch_destroy_internal(a->batch);
@@ -268,9 +325,12 @@ static int amatch_launch(AMatch *a, int w, int L) {
         }
         if ((int64_t)nw > a->out_cap) {
             (void)hipFree(a->d_out);
-            a->d_out = nullptr;
+            if (a->h_out) (void)hipHostFree(a->h_out);
+            a->d_out = a->h_out = nullptr;
             a->out_cap = 0;
             SL_HIP_TRY(hipMalloc((void **)&a->d_out, sizeof(int32_t) * 4 * (size_t)nw));
+            // cached pinned memory: the host reads it only after the stream has synced
+            SL_HIP_TRY(hipHostMalloc((void **)&a->h_out, sizeof(int32_t) * 4 * (size_t)nw, hipHostMallocNonCoherent));
             a->out_cap = nw;
         }
         SL_HIP_TRY(hipMemcpyAsync(a->d_items, a->h_items.data(), sizeof(am::AmItem) * nitems,
@@ -321,22 +381,16 @@ static int amatch_check_window(const AMatch *a, int w, int L, const char *who) {
     return kOk;
 }
 
-// The windows' arrays on the host: score, best start, best end, best score.
-static int amatch_windows_host(AMatch *a, int w, int L, std::vector<int32_t> &ws, std::vector<int32_t> &bs,
-                               std::vector<int32_t> &be, std::vector<int32_t> &bsc) {
+// Runs the window kernel and copies its four arrays (score, best start, best
+// end, best score; n - w + 1 entries each, out_cap apart) to the pinned host
+// buffer; returns it in *host.
+static int amatch_windows_host(AMatch *a, int w, int L, const int32_t **host) {
     int rc = amatch_launch(a, w, L);
     if (rc) return rc;
-    const size_t nw = (size_t)(a->n - w + 1);
-    ws.resize(nw);
-    bs.resize(nw);
-    be.resize(nw);
-    bsc.resize(nw);
-    const int64_t cap = a->out_cap;
-    SL_HIP_TRY(hipMemcpyAsync(ws.data(), a->d_out, 4 * nw, hipMemcpyDeviceToHost, a->stream));
-    SL_HIP_TRY(hipMemcpyAsync(bs.data(), a->d_out + cap, 4 * nw, hipMemcpyDeviceToHost, a->stream));
-    SL_HIP_TRY(hipMemcpyAsync(be.data(), a->d_out + 2 * cap, 4 * nw, hipMemcpyDeviceToHost, a->stream));
-    SL_HIP_TRY(hipMemcpyAsync(bsc.data(), a->d_out + 3 * cap, 4 * nw, hipMemcpyDeviceToHost, a->stream));
+    SL_HIP_TRY(hipMemcpyAsync(a->h_out, a->d_out, sizeof(int32_t) * 4 * (size_t)a->out_cap, hipMemcpyDeviceToHost,
+                              a->stream));
     SL_HIP_TRY(hipStreamSynchronize(a->stream));
+    *host = a->h_out;
     return kOk;
 }
 
@@ -344,6 +398,42 @@ static long long interval_key(int l, int r) {   // DuplicateSearch::buildKey (Ap
     long long key = l;
     return (key << 32) | r;
 }
+
+// Set of interval keys (both halves non-negative, so -1 marks an empty slot):
+// open addressing with linear probing, kept at most half full.
+class KeySet {
+public:
+    bool insert(long long key) {   // false if present
+        if (2 * (used_ + 1) > slots_.size()) grow();
+        return put(key);
+    }
+
+private:
+    std::vector<long long> slots_;
+    size_t used_ = 0;
+    size_t mask() const { return slots_.size() - 1; }
+    size_t home(long long key) const {
+        return (size_t)(((unsigned long long)key * 0x9E3779B97F4A7C15ull) >> 20) & mask();
+    }
+    bool put(long long key) {
+        for (size_t h = home(key);; h = (h + 1) & mask()) {
+            if (slots_[h] == key) return false;
+            if (slots_[h] == -1) {
+                slots_[h] = key;
+                ++used_;
+                return true;
+            }
+        }
+    }
+    void grow() {
+        std::vector<long long> old;
+        old.swap(slots_);
+        slots_.assign(old.empty() ? 1024 : 2 * old.size(), -1);
+        used_ = 0;
+        for (long long k : old)
+            if (k != -1) put(k);
+    }
+};
 
 // DuplicateSearch::phase3 (ApproximateMatching.h:212-228): sort by start, then
 // score descending (std::sort, so the input order decides among equal keys),
@@ -389,12 +479,16 @@ static int duplicate_search(AMatch *a, double k, int variant) {
     }
     int rc = amatch_check_window(a, w, L, "sl_duplicate_search");
     if (rc) return rc;
-    std::vector<int32_t> ws, bs, be, bsc;
-    rc = amatch_windows_host(a, w, L, ws, bs, be, bsc);
+    const int32_t *h = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = amatch_windows_host(a, w, L, &h);
     if (rc) return rc;
+    const auto t1 = std::chrono::steady_clock::now();
+    const int64_t cap = a->out_cap;
+    const int32_t *ws = h, *bs = h + cap, *be = h + 2 * cap, *bsc = h + 3 * cap;
     const int n = a->n;
     const int nw = n - w + 1;
-    std::unordered_set<long long> seen;
+    KeySet seen;
     std::vector<AmInterval> set_w2;
     if (variant == 0) {
         // InteractiveDuplicateSearchSemiDense::find (ApproximateMatching.h:339-432):
@@ -402,11 +496,9 @@ static int duplicate_search(AMatch *a, double k, int variant) {
         // own score reaches kdi contributes its best substring once; then the
         // list is reversed and passed to phase 3.
         for (int i = nw - 1; i >= 0; --i) {
-            const double window_dist = ws[(size_t)i];
-            if (window_dist >= kdi && seen.count(interval_key(bs[(size_t)i], be[(size_t)i])) == 0) {
-                set_w2.push_back(AmInterval{bs[(size_t)i], be[(size_t)i], (double)bsc[(size_t)i]});
-                seen.insert(interval_key(bs[(size_t)i], be[(size_t)i]));
-            }
+            const double window_dist = ws[i];
+            if (window_dist >= kdi && seen.insert(interval_key(bs[i], be[i])))
+                set_w2.push_back(AmInterval{bs[i], be[i], (double)bsc[i]});
         }
         std::reverse(set_w2.begin(), set_w2.end());
     } else {
@@ -417,7 +509,7 @@ static int duplicate_search(AMatch *a, double k, int variant) {
         // int before the halving.
         std::vector<int> set_w1;
         for (int end = w; end < n + 1;) {
-            const double dist = ws[(size_t)(end - w)];
+            const double dist = ws[end - w];
             if (dist >= kdi) set_w1.push_back(end - w);
             if (dist < kdi - 1) {
                 end += std::max(1, (std::abs((int)dist) - std::abs((int)kdi)) / 2);
@@ -426,15 +518,13 @@ static int duplicate_search(AMatch *a, double k, int variant) {
             }
         }
         // phase 2: each kept window's best substring, once
-        for (int i : set_w1) {
-            const long long key = interval_key(bs[(size_t)i], be[(size_t)i]);
-            if (seen.count(key) == 0) {
-                seen.insert(key);
-                set_w2.push_back(AmInterval{bs[(size_t)i], be[(size_t)i], (double)bsc[(size_t)i]});
-            }
-        }
+        for (int i : set_w1)
+            if (seen.insert(interval_key(bs[i], be[i]))) set_w2.push_back(AmInterval{bs[i], be[i], (double)bsc[i]});
     }
     phase3(set_w2, a->results);
+    const auto t2 = std::chrono::steady_clock::now();
+    a->last_device_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    a->last_host_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     return kOk;
 }
 
@@ -527,14 +617,15 @@ int sl_amatch_windows(sl_amatch *h, int w, int min_len, int32_t *window_score, i
     }
     int rc = amatch_check_window(a, w, min_len, "sl_amatch_windows");
     if (rc) return rc;
-    std::vector<int32_t> ws, bs, be, bsc;
-    rc = amatch_windows_host(a, w, min_len, ws, bs, be, bsc);
+    const int32_t *hbuf = nullptr;
+    rc = amatch_windows_host(a, w, min_len, &hbuf);
     if (rc) return rc;
-    const size_t bytes = 4 * ws.size();
-    std::memcpy(window_score, ws.data(), bytes);
-    std::memcpy(best_start, bs.data(), bytes);
-    std::memcpy(best_end, be.data(), bytes);
-    std::memcpy(best_score, bsc.data(), bytes);
+    const size_t bytes = sizeof(int32_t) * (size_t)(a->n - w + 1);
+    const int64_t cap = a->out_cap;
+    std::memcpy(window_score, hbuf, bytes);
+    std::memcpy(best_start, hbuf + cap, bytes);
+    std::memcpy(best_end, hbuf + 2 * cap, bytes);
+    std::memcpy(best_score, hbuf + 3 * cap, bytes);
     return kOk;
 }
 
@@ -596,6 +687,17 @@ int sl_duplicate_results(const sl_amatch *h, int32_t *start, int32_t *end, doubl
         end[i] = a->results[i].end;
         score[i] = a->results[i].score;
     }
+    return kOk;
+}
+
+int sl_amatch_last_times(const sl_amatch *h, double *device_ms, double *host_ms) {
+    const AMatch *a = reinterpret_cast<const AMatch *>(h);
+    if (!a) {
+        set_error("sl_amatch_last_times: null handle");
+        return kErrArg;
+    }
+    if (device_ms) *device_ms = a->last_device_ms;
+    if (host_ms) *host_ms = a->last_host_ms;
     return kOk;
 }
 
