@@ -329,6 +329,32 @@ int sl_server_stats(sl_server *s, sl_server_stats_t *stats);
 void sl_server_destroy(sl_server *s);
 
 /* ------------------------------------------------------------------------ */
+/* Score-only LLCS by Hyyro's bit-vector recurrence, batched on the GPU.      */
+/* Replaces: lllcs_hyyro::hyyro_magic / hyyro_magic_mpi_with_precalc          */
+/*   (prefix_lcs/bitwise/transposition_network_binary_alphabet.h:743-916) and */
+/*   the namespaced Hyyro<Word, Input>::compute (include/BitwiseLCS/           */
+/*   BitParalellLCS.h:337-433); any alphabet with at most 256 distinct symbols */
+/*   per pair, strings up to 2^20 symbols (the longer one is bit-packed).     */
+/* Inputs are laid out as for sl_batch_create; llcs[i] receives pair i's LLCS.*/
+/* ------------------------------------------------------------------------ */
+int sl_llcs_hyyro(const void *a, int m, const void *b, int n, int sym_bytes, int *llcs);
+int sl_llcs_hyyro_batch(int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                        const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes,
+                        int32_t *llcs);
+
+/* Resident form: symbols recoded and match-mask tables built once. */
+typedef struct sl_hyyro sl_hyyro;
+int sl_hyyro_create(sl_hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                    const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes);
+int sl_hyyro_run(sl_hyyro *h);                      /* asynchronous */
+int sl_hyyro_result(sl_hyyro *h, int32_t *llcs);    /* waits, then copies npairs scores */
+int sl_hyyro_time(sl_hyyro *h, int warmup, int iters, double *ms_per_run);
+/* Work: sum over pairs of rows x 64-bit words (after dropping rows whose
+ * symbol is absent from the packed string), and the mask-table words. */
+int sl_hyyro_stats(const sl_hyyro *h, int64_t *word_rows, int64_t *table_words);
+void sl_hyyro_destroy(sl_hyyro *h);
+
+/* ------------------------------------------------------------------------ */
 /* Approximate matching over resident braids.                                 */
 /* The pattern p is combed against overlapping chunks of the text (one ragged */
 /* batch launch; every substring of length <= max_window lies inside one      */

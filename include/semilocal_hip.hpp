@@ -25,6 +25,7 @@
 #include <string>
 #include <type_traits>
 #include <cmath>
+#include <unordered_map>
 #include <vector>
 
 #include "semilocal_hip.h"
@@ -291,5 +292,47 @@ int llcs_from_kernel(const Perm &kernel, int a_size, int b_size) {
     for (int s = 0; s < a_size; ++s) acc += kernel.get_col_by_row(s) >= b_size;
     return a_size - acc;
 }
+
+// Flat Hyyro entry points (prefix_lcs/bitwise/transposition_network_binary_
+// alphabet.h:743-916): the caller has built `lookup` with the reference's
+// lllcs_hyyro::preprocess (one bit table per symbol of a over the packed
+// string b).  The packed string is rebuilt from the tables (positions no
+// table marks take a symbol that matches nothing) and scored on the device
+// by sl_llcs_hyyro; same score.
+namespace lllcs_hyyro {
+
+template <class Input>
+int hyyro_magic(const int *str_a, int size_a_bits, int size_b_word, std::unordered_map<int, Input *> &lookup) {
+    const int W = (int)sizeof(Input) * 8;
+    const int64_t nb = (int64_t)size_b_word * W;
+    int sentinel = 0;
+    bool first = true;
+    for (auto &kv : lookup)
+        if (first || kv.first < sentinel) {
+            sentinel = kv.first;
+            first = false;
+        }
+    sentinel = first ? 0 : sentinel - 1;   // below every key: matches no symbol of a
+    std::vector<int32_t> b((size_t)nb, sentinel);
+    for (auto &kv : lookup) {
+        const Input *t = kv.second;
+        if (!t) continue;
+        for (int w = 0; w < size_b_word; ++w)
+            for (int j = 0; j < W; ++j)
+                if ((t[w] >> j) & Input(1)) b[(size_t)w * W + j] = kv.first;
+    }
+    int llcs = 0;
+    check(sl_llcs_hyyro(str_a, size_a_bits, b.data(), (int)nb, 4, &llcs));
+    return llcs;
+}
+
+template <class Input>
+int hyyro_magic_mpi_with_precalc(int *str_a, int size_a_bits, int size_b_word,
+                                 std::unordered_map<int, Input *> &lookup, int num_threads = 1) {
+    (void)num_threads;
+    return hyyro_magic<Input>(str_a, size_a_bits, size_b_word, lookup);
+}
+
+}  // namespace lllcs_hyyro
 
 }  // namespace semilocal_hip

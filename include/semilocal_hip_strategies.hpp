@@ -198,4 +198,20 @@ class HipLLCS : public lcs::LCSStrategy<Input> {
     }
 };
 
+// Hyyro's bit-vector LLCS on the device (sl_llcs_hyyro): replaces
+// bit_parallel::lcs::Hyyro<Word, Input> (BitParalellLCS.h:337-433), same
+// score.  Word is accepted for the signature (the device uses 64-bit words);
+// Input must be 1, 2 or 4 bytes; at most 256 distinct symbols in the longer
+// string, strings up to 2^20 symbols.
+template <class Word, class Input>
+class HipHyyro : public lcs::LCSStrategy<Input> {
+  public:
+    int compute(const Input *a, int sizeA, const Input *b, int sizeB) override {
+        static_assert(sizeof(Input) == 1 || sizeof(Input) == 2 || sizeof(Input) == 4, "1, 2 or 4-byte symbols");
+        int llcs = 0;
+        check(sl_llcs_hyyro(a, sizeA, b, sizeB, (int)sizeof(Input), &llcs));
+        return llcs;
+    }
+};
+
 }  // namespace semilocal_hip

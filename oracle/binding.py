@@ -174,6 +174,8 @@ class Reference:
         L.ref_steady_ant.argtypes = [_i32p, _i32p, C.c_int, _i32p, C.c_int]
         L.ref_llcs_2symbol_smart.argtypes = [_u64p, C.c_int, _u64p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ref_llcs_2symbol_smart.restype = C.c_int
+        L.ref_hyyro.argtypes = [_i32p, C.c_int, _i32p, C.c_int]
+        L.ref_hyyro.restype = C.c_int
 
     @staticmethod
     def available(path: str = REF_SO) -> bool:
@@ -205,6 +207,11 @@ class Reference:
         out = np.empty(len(p), np.int32)
         self.L.ref_steady_ant(p, q, len(p), out, nested)
         return out
+
+    def hyyro(self, a, b) -> int:
+        """lllcs_hyyro::preprocess + hyyro_magic<uint64_t> over (a, b)."""
+        a, b = _i32(a), _i32(b)
+        return self.L.ref_hyyro(a, len(a), b, len(b))
 
     def llcs_binary(self, a_bits, b_bits, threads: int = 1, formula_one: bool = False) -> int:
         o = Oracle()

@@ -91,4 +91,18 @@ int ref_llcs_2symbol_smart(const uint64_t *a_rev, int a_words, const uint64_t *b
         formula_one != 0);
 }
 
+// The reference's Hyyro LLCS (lllcs_hyyro::preprocess + hyyro_magic over
+// 64-bit words, transposition_network_binary_alphabet.h:743-820): the
+// alphabet of a, its bit tables over b, then the row recurrence.
+int ref_hyyro(const int32_t *a, int m, const int32_t *b, int n) {
+    if (m == 0 || n == 0) return 0;
+    std::unordered_set<int> alphabet(a, a + m);
+    std::unordered_map<int, uint64_t *> lookup;
+    lllcs_hyyro::preprocess<uint64_t>(b, n, alphabet, lookup);
+    const int words = (n + 63) / 64;
+    const int score = lllcs_hyyro::hyyro_magic<uint64_t>(a, m, words, lookup);
+    for (auto &kv : lookup) delete[] kv.second;
+    return score;
+}
+
 }  // extern "C"

@@ -69,6 +69,14 @@ SIGNATURES = {
                              C.c_int),
     "sl_braid_index_stats": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int64)], C.c_int),
     "sl_braid_index_destroy": ([_vp], None),
+    "sl_llcs_hyyro": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "sl_llcs_hyyro_batch": ([C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, _i32p], C.c_int),
+    "sl_hyyro_create": ([C.POINTER(_vp), C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int], C.c_int),
+    "sl_hyyro_run": ([_vp], C.c_int),
+    "sl_hyyro_result": ([_vp, _i32p], C.c_int),
+    "sl_hyyro_time": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "sl_hyyro_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "sl_hyyro_destroy": ([_vp], None),
     "sl_amatch_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "sl_amatch_stats": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)], C.c_int),
     "sl_amatch_windows": ([_vp, C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p], C.c_int),
@@ -253,6 +261,59 @@ class BraidIndex:
     def close(self):
         if getattr(self, "h", None):
             lib().sl_braid_index_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def llcs_hyyro(a, b) -> int:
+    """LLCS of one pair by Hyyro's bit-vector recurrence on the device."""
+    a_arr, wa = as_symbols(a)
+    b_arr, wb = as_symbols(b)
+    if wa != wb:
+        a_arr, wa = np.ascontiguousarray(a_arr, dtype=np.int32), 4
+        b_arr, wb = np.ascontiguousarray(b_arr, dtype=np.int32), 4
+    out = C.c_int()
+    check(lib().sl_llcs_hyyro(_ptr(a_arr) if len(a_arr) else None, len(a_arr), _ptr(b_arr) if len(b_arr) else None,
+                              len(b_arr), wa, C.byref(out)))
+    return out.value
+
+
+class HyyroBatch:
+    """Many pairs' LLCS by Hyyro's recurrence (sl_hyyro_*): symbols recoded
+    and match tables built once, then run / time."""
+
+    def __init__(self, pairs):
+        self.a_cat, self.a_off, self.a_len, self.b_cat, self.b_off, self.b_len, w = _concat(pairs)
+        self.npairs = len(pairs)
+        h = _vp()
+        check(lib().sl_hyyro_create(C.byref(h), self.npairs, _ptr(self.a_cat), self.a_off, self.a_len,
+                                    _ptr(self.b_cat), self.b_off, self.b_len, w))
+        self.h = h
+
+    def run(self) -> np.ndarray:
+        check(lib().sl_hyyro_run(self.h))
+        out = np.empty(max(self.npairs, 1), np.int32)
+        check(lib().sl_hyyro_result(self.h, out))
+        return out[:self.npairs]
+
+    def time(self, warmup: int = 1, iters: int = 5) -> float:
+        ms = C.c_double()
+        check(lib().sl_hyyro_time(self.h, warmup, iters, C.byref(ms)))
+        return ms.value
+
+    def stats(self):
+        wr, tw = C.c_int64(), C.c_int64()
+        check(lib().sl_hyyro_stats(self.h, C.byref(wr), C.byref(tw)))
+        return {"word_rows": wr.value, "table_words": tw.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sl_hyyro_destroy(self.h)
             self.h = None
 
     def __del__(self):

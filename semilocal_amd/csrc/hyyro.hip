@@ -1,0 +1,485 @@
+// hyyro.hip -- score-only LLCS by Hyyro's bit-vector recurrence, batched.
+//
+// For a pair (a, b) with b packed into K = ceil(n / 64) words and one match
+// mask per symbol (M_s bit j = [b_j == s]), every symbol x of a updates the
+// bit vector V (initially all ones):
+//
+//   p = M_x & V;   V = (V ^ p) | (V + p)        (one n-bit addition)
+//
+// and LLCS = the number of zero bits of V at the end (include/BitwiseLCS/
+// BitParalellLCS.h:337-433, prefix_lcs/bitwise/transposition_network_binary_
+// alphabet.h:743-916: hyyro_magic and its anti-diagonal form).
+//
+// The n-bit addition is the only coupling between words.  Lane l of a wave
+// holds R consecutive words; a row is two passes over them:
+//   1. with carry-in 0, each lane learns whether it generates a carry (g) and
+//      whether it is all ones after the add, so a carry would pass through (q);
+//   2. two ballots G, X = G | Q and one 64-bit add give every lane its carry-in
+//      (wave64 carry-lookahead: carry into lane l = bit l of (X + G + c) ^ X ^ G),
+//      and the lane redoes its words with it.
+// Pairs up to 65,536 columns run one wave each; longer ones (up to 2^20
+// columns) one 16-wave workgroup, whose waves fold their (carry-out if 0,
+// carry-out if 1) pairs through LDS once per row.  Masks come from a table
+// built on the device (symbols of b: at most 256 per pair), prefetched D rows
+// ahead; rows whose symbol does not occur in b are dropped on the host (their
+// p is 0 and they leave V unchanged).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "common.h"
+#include "semilocal_hip.h"
+
+namespace slhip {
+namespace hy {
+
+constexpr int kMaxSymbols = 256;
+constexpr int kBlockWaves = 16;
+constexpr int kMaxColumns = kBlockWaves * 64 * 64 * 16;   // 2^20
+
+struct HyPair {
+    int64_t rows_off;   // row symbols (int32 codes into the pair's table) in the row pool
+    int64_t tab_off;    // the pair's mask table (sigma x K words) in the table pool
+    int64_t cols_off;   // column symbols (uint8 codes) in the column pool
+    int32_t m, n, K, sigma;
+    int32_t out;        // caller index
+    int32_t pad_;
+};
+
+__global__ void hy_table(const HyPair *__restrict__ pairs, const uint8_t *__restrict__ cols,
+                         uint64_t *__restrict__ tab) {
+    const HyPair P = pairs[blockIdx.x];
+    const uint8_t *b = cols + P.cols_off;
+    const int64_t total = (int64_t)P.sigma * P.K;
+    for (int64_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int s = (int)(idx / P.K), w = (int)(idx % P.K);
+        const int j0 = 64 * w, j1 = min(P.n, j0 + 64);
+        uint64_t mask = 0;
+        for (int j = j0; j < j1; ++j) mask |= (uint64_t)(b[j] == s ? 1 : 0) << (j - j0);
+        tab[P.tab_off + idx] = mask;
+    }
+}
+
+// 64-bit add with carry in and out
+__device__ __forceinline__ uint64_t add_c(uint64_t x, uint64_t y, int &c) {
+    const uint64_t t = x + y;
+    const int o1 = t < x ? 1 : 0;
+    const uint64_t t2 = t + (uint64_t)c;
+    const int o2 = t2 < t ? 1 : 0;
+    c = o1 | o2;
+    return t2;
+}
+
+template <int R, int D, int NW>
+__global__ __launch_bounds__(64 * NW) void hy_rows(const HyPair *__restrict__ pairs, const int32_t *__restrict__ list,
+                                                   const int32_t *__restrict__ rows,
+                                                   const uint64_t *__restrict__ tab, int32_t *__restrict__ llcs) {
+    __shared__ uint32_t fold[2][NW];   // per row parity: wave's (carry-out if 0) | (carry-out if 1) << 1
+    __shared__ int zsum[NW];
+    const HyPair P = pairs[list[blockIdx.x]];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int w0 = (wave * 64 + lane) * R;   // first word of this lane
+    const uint64_t *T = tab + P.tab_off;
+    const int32_t *A = rows + P.rows_off;
+    const int K = P.K, m = P.m;
+    uint64_t V[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) V[r] = ~0ull;
+    // mask ring: masks of rows i .. i + D - 1
+    uint64_t Mq[D][R];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int s = d < m ? A[d] : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) Mq[d][r] = (d < m && w0 + r < K) ? T[(int64_t)s * K + w0 + r] : 0ull;
+    }
+    for (int i0 = 0; i0 < m; i0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int i = i0 + d;
+            if (i < m) {   // uniform
+                uint64_t p[R];
+                int c = 0, all_ones = 1;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    p[r] = Mq[d][r] & V[r];
+                    const uint64_t t = add_c(V[r], p[r], c);
+                    all_ones &= t == ~0ull ? 1 : 0;
+                }
+                const uint64_t G = __ballot(c);
+                const uint64_t X = G | __ballot(all_ones);
+                int cin = 0;
+                if (NW > 1) {
+                    // the wave's carry-out for carry-in 0 and 1, folded over
+                    // the lower waves of the block through LDS
+                    const uint64_t s0 = X + G;
+                    const int co0 = s0 < X ? 1 : 0;
+                    const int co1 = co0 | (s0 + 1 == 0 ? 1 : 0);
+                    if (lane == 0) fold[i & 1][wave] = (uint32_t)co0 | ((uint32_t)co1 << 1);
+                    __syncthreads();
+                    for (int j = 0; j < wave; ++j) {
+                        const uint32_t f = fold[i & 1][j];
+                        cin = cin ? (int)(f >> 1) : (int)(f & 1u);
+                    }
+                }
+                const uint64_t C = (X + G + (uint64_t)cin) ^ X ^ G;
+                c = (int)((C >> lane) & 1ull);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t t = add_c(V[r], p[r], c);
+                    V[r] = (V[r] ^ p[r]) | t;
+                }
+                // refill this ring slot with row i + D
+                const int nx = i + D;
+                if (nx < m) {
+                    const int s = A[nx];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) Mq[d][r] = w0 + r < K ? T[(int64_t)s * K + w0 + r] : 0ull;
+                }
+            }
+        }
+    }
+    // LLCS = zero bits of V (padding bits beyond n stay one)
+    int z = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) z += 64 - __popcll(V[r]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
+    if (NW == 1) {
+        if (lane == 0) llcs[P.out] = z;
+    } else {
+        if (lane == 0) zsum[wave] = z;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int j = 0; j < NW; ++j) tot += zsum[j];
+            llcs[P.out] = tot;
+        }
+    }
+}
+
+}  // namespace hy
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct Hyyro {
+    int npairs = 0;
+    hipStream_t stream = nullptr;
+    std::vector<hy::HyPair> pairs;
+    std::vector<std::vector<int32_t>> cls_list;   // launch class (R, waves) -> pair indices
+    hy::HyPair *d_pairs = nullptr;
+    int32_t *d_list = nullptr;          // all classes' lists, concatenated
+    std::vector<int64_t> list_off;
+    int32_t *d_rows = nullptr;
+    uint8_t *d_cols = nullptr;
+    uint64_t *d_tab = nullptr;
+    int32_t *d_llcs = nullptr;
+    int64_t tab_words = 0, word_rows = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+// class id: 0..4 = one wave with R = 1, 2, 4, 8, 16; 5 = 16 waves with R = 16
+int hy_class(int K) {
+    if (K <= 64) return 0;
+    if (K <= 128) return 1;
+    if (K <= 256) return 2;
+    if (K <= 512) return 3;
+    if (K <= 1024) return 4;
+    return 5;
+}
+constexpr int kClasses = 6;
+
+void hy_free(Hyyro *h) {
+    if (!h) return;
+    (void)hipFree(h->d_pairs);
+    (void)hipFree(h->d_list);
+    (void)hipFree(h->d_rows);
+    (void)hipFree(h->d_cols);
+    (void)hipFree(h->d_tab);
+    (void)hipFree(h->d_llcs);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+// b's symbols -> dense codes in order of first appearance; a's symbols ->
+// codes, dropping those absent from b.  `table` maps a symbol value to its
+// code (-1: none) for 1- and 2-byte symbols and is left all -1 on return.
+template <class Sym>
+int hy_encode(const Sym *a, int m, const Sym *b, int n, std::vector<int32_t> &rows, std::vector<uint8_t> &cols,
+              int *sigma, std::vector<int> &table) {
+    int count = 0;
+    std::vector<uint32_t> used;
+    auto finish = [&](int rc) {
+        for (uint32_t v : used) table[v] = -1;
+        return rc;
+    };
+    for (int j = 0; j < n; ++j) {
+        const uint32_t v = (uint32_t)b[j];
+        int c = table[v];
+        if (c < 0) {
+            if (count >= hy::kMaxSymbols) return finish(kErrUnsupported);
+            c = table[v] = count++;
+            used.push_back(v);
+        }
+        cols.push_back((uint8_t)c);
+    }
+    for (int i = 0; i < m; ++i) {
+        const int c = table[(uint32_t)a[i]];
+        if (c >= 0) rows.push_back(c);
+    }
+    *sigma = count;
+    return finish(kOk);
+}
+
+int hy_encode_u32(const uint32_t *a, int m, const uint32_t *b, int n, std::vector<int32_t> &rows,
+                  std::vector<uint8_t> &cols, int *sigma) {
+    std::unordered_map<uint32_t, int> code;
+    code.reserve(512);
+    for (int j = 0; j < n; ++j) {
+        auto it = code.find(b[j]);
+        int c;
+        if (it == code.end()) {
+            c = (int)code.size();
+            if (c >= hy::kMaxSymbols) return kErrUnsupported;
+            code.emplace(b[j], c);
+        } else {
+            c = it->second;
+        }
+        cols.push_back((uint8_t)c);
+    }
+    for (int i = 0; i < m; ++i) {
+        auto it = code.find(a[i]);
+        if (it != code.end()) rows.push_back(it->second);
+    }
+    *sigma = (int)code.size();
+    return kOk;
+}
+
+int hy_launch(Hyyro *h) {
+    for (int c = 0; c < kClasses; ++c) {
+        const unsigned cnt = (unsigned)h->cls_list[(size_t)c].size();
+        if (!cnt) continue;
+        const int32_t *lst = h->d_list + h->list_off[(size_t)c];
+#define SL_HY_LAUNCH(R, D, NW)                                                                              \
+    hipLaunchKernelGGL((hy::hy_rows<R, D, NW>), dim3(cnt), dim3(64 * NW), 0, h->stream, h->d_pairs, lst, h->d_rows, \
+                       h->d_tab, h->d_llcs)
+        switch (c) {
+            case 0: SL_HY_LAUNCH(1, 8, 1); break;
+            case 1: SL_HY_LAUNCH(2, 8, 1); break;
+            case 2: SL_HY_LAUNCH(4, 4, 1); break;
+            case 3: SL_HY_LAUNCH(8, 2, 1); break;
+            case 4: SL_HY_LAUNCH(16, 2, 1); break;
+            default: SL_HY_LAUNCH(16, 2, hy::kBlockWaves); break;
+        }
+#undef SL_HY_LAUNCH
+    }
+    SL_HIP_TRY(hipGetLastError());
+    return kOk;
+}
+
+int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+              const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes) {
+    *out = nullptr;
+    if (npairs < 0 || (npairs > 0 && (!a_cat || !a_off || !a_len || !b_cat || !b_off || !b_len)) ||
+        (sym_bytes != 1 && sym_bytes != 2 && sym_bytes != 4)) {
+        set_error("sl_hyyro_create: bad argument");
+        return kErrArg;
+    }
+    int rc = ensure_device();
+    if (rc) return rc;
+    Hyyro *h = new Hyyro();
+    h->npairs = npairs;
+    h->cls_list.assign(kClasses, {});
+    std::vector<int32_t> rows;
+    std::vector<uint8_t> cols;
+    std::vector<int> table(sym_bytes == 1 ? 256 : (sym_bytes == 2 ? 65536 : 0), -1);
+    for (int i = 0; i < npairs; ++i) {
+        int m = a_len[i], n = b_len[i];
+        if (m < 0 || n < 0) {
+            hy_free(h);
+            set_error("sl_hyyro_create: negative length");
+            return kErrArg;
+        }
+        const char *pa = (const char *)a_cat + (size_t)a_off[i] * sym_bytes;
+        const char *pb = (const char *)b_cat + (size_t)b_off[i] * sym_bytes;
+        if (m > n) {   // pack the longer string into the bit vector
+            std::swap(m, n);
+            std::swap(pa, pb);
+        }
+        if (n > hy::kMaxColumns) {
+            hy_free(h);
+            set_error("sl_hyyro_create: a string longer than 2^20 symbols (use the semi-local kernel)");
+            return kErrUnsupported;
+        }
+        hy::HyPair P{};
+        P.rows_off = (int64_t)rows.size();
+        P.cols_off = (int64_t)cols.size();
+        P.n = n;
+        P.K = (n + 63) / 64;
+        P.out = i;
+        int sigma = 0;
+        if (sym_bytes == 1)
+            rc = hy_encode((const uint8_t *)pa, m, (const uint8_t *)pb, n, rows, cols, &sigma, table);
+        else if (sym_bytes == 2)
+            rc = hy_encode((const uint16_t *)pa, m, (const uint16_t *)pb, n, rows, cols, &sigma, table);
+        else
+            rc = hy_encode_u32((const uint32_t *)pa, m, (const uint32_t *)pb, n, rows, cols, &sigma);
+        if (rc) {
+            hy_free(h);
+            set_error("sl_hyyro_create: more than 256 distinct symbols in one string (use the semi-local kernel)");
+            return rc;
+        }
+        P.m = (int)((int64_t)rows.size() - P.rows_off);
+        P.sigma = sigma;
+        P.tab_off = h->tab_words;
+        h->tab_words += (int64_t)sigma * P.K;
+        h->word_rows += (int64_t)P.m * P.K;
+        h->pairs.push_back(P);
+        h->cls_list[(size_t)hy_class(P.K)].push_back(i);
+    }
+    std::vector<int32_t> all_lists;
+    h->list_off.assign(kClasses, 0);
+    for (int c = 0; c < kClasses; ++c) {
+        h->list_off[(size_t)c] = (int64_t)all_lists.size();
+        all_lists.insert(all_lists.end(), h->cls_list[(size_t)c].begin(), h->cls_list[(size_t)c].end());
+    }
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    for (auto &ev : h->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    auto alloc = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 8));
+    };
+    alloc((void **)&h->d_pairs, sizeof(hy::HyPair) * h->pairs.size());
+    alloc((void **)&h->d_list, sizeof(int32_t) * all_lists.size());
+    alloc((void **)&h->d_rows, sizeof(int32_t) * rows.size());
+    alloc((void **)&h->d_cols, cols.size());
+    alloc((void **)&h->d_tab, sizeof(uint64_t) * (size_t)h->tab_words);
+    alloc((void **)&h->d_llcs, sizeof(int32_t) * (size_t)std::max(npairs, 1));
+    if (e == hipSuccess && !h->pairs.empty())
+        e = hipMemcpy(h->d_pairs, h->pairs.data(), sizeof(hy::HyPair) * h->pairs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !all_lists.empty())
+        e = hipMemcpy(h->d_list, all_lists.data(), sizeof(int32_t) * all_lists.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !rows.empty())
+        e = hipMemcpy(h->d_rows, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !cols.empty()) e = hipMemcpy(h->d_cols, cols.data(), cols.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && npairs > 0) {
+        hipLaunchKernelGGL(hy::hy_table, dim3((unsigned)npairs), dim3(256), 0, h->stream, h->d_pairs, h->d_cols,
+                           h->d_tab);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        set_error(std::string("sl_hyyro_create: ") + hipGetErrorString(e));
+        hy_free(h);
+        return kErrHip;
+    }
+    *out = h;
+    return kOk;
+}
+
+}  // namespace
+}  // namespace slhip
+
+using namespace slhip;
+
+extern "C" {
+
+int sl_hyyro_create(sl_hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                    const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes) {
+    if (!out) {
+        set_error("sl_hyyro_create: null handle pointer");
+        return kErrArg;
+    }
+    Hyyro *h = nullptr;
+    const int rc = hy_create(&h, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes);
+    *out = reinterpret_cast<sl_hyyro *>(h);
+    return rc;
+}
+
+int sl_hyyro_run(sl_hyyro *hh) {
+    Hyyro *h = reinterpret_cast<Hyyro *>(hh);
+    if (!h) {
+        set_error("sl_hyyro_run: null handle");
+        return kErrArg;
+    }
+    return hy_launch(h);
+}
+
+int sl_hyyro_result(sl_hyyro *hh, int32_t *llcs) {
+    Hyyro *h = reinterpret_cast<Hyyro *>(hh);
+    if (!h || (h->npairs > 0 && !llcs)) {
+        set_error("sl_hyyro_result: null argument");
+        return kErrArg;
+    }
+    if (h->npairs == 0) return kOk;
+    SL_HIP_TRY(hipMemcpyAsync(llcs, h->d_llcs, sizeof(int32_t) * (size_t)h->npairs, hipMemcpyDeviceToHost,
+                              h->stream));
+    SL_HIP_TRY(hipStreamSynchronize(h->stream));
+    return kOk;
+}
+
+int sl_hyyro_time(sl_hyyro *hh, int warmup, int iters, double *ms_per_run) {
+    Hyyro *h = reinterpret_cast<Hyyro *>(hh);
+    if (!h || iters < 1 || warmup < 0 || !ms_per_run) {
+        set_error("sl_hyyro_time: bad argument");
+        return kErrArg;
+    }
+    int rc = kOk;
+    for (int r = 0; r < warmup && !rc; ++r) rc = hy_launch(h);
+    if (rc) return rc;
+    SL_HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+    for (int r = 0; r < iters && !rc; ++r) rc = hy_launch(h);
+    SL_HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+    if (rc) return rc;
+    SL_HIP_TRY(hipEventSynchronize(h->ev[1]));
+    float ms = 0;
+    SL_HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+    *ms_per_run = ms / iters;
+    return kOk;
+}
+
+int sl_hyyro_stats(const sl_hyyro *hh, int64_t *word_rows, int64_t *table_words) {
+    const Hyyro *h = reinterpret_cast<const Hyyro *>(hh);
+    if (!h) {
+        set_error("sl_hyyro_stats: null handle");
+        return kErrArg;
+    }
+    if (word_rows) *word_rows = h->word_rows;
+    if (table_words) *table_words = h->tab_words;
+    return kOk;
+}
+
+void sl_hyyro_destroy(sl_hyyro *hh) { hy_free(reinterpret_cast<Hyyro *>(hh)); }
+
+int sl_llcs_hyyro_batch(int npairs, const void *a_cat, const int64_t *a_off, const int32_t *a_len,
+                        const void *b_cat, const int64_t *b_off, const int32_t *b_len, int sym_bytes, int32_t *llcs) {
+    sl_hyyro *h = nullptr;
+    int rc = sl_hyyro_create(&h, npairs, a_cat, a_off, a_len, b_cat, b_off, b_len, sym_bytes);
+    if (!rc) rc = sl_hyyro_run(h);
+    if (!rc) rc = sl_hyyro_result(h, llcs);
+    sl_hyyro_destroy(h);
+    return rc;
+}
+
+int sl_llcs_hyyro(const void *a, int m, const void *b, int n, int sym_bytes, int *llcs) {
+    if (!llcs || m < 0 || n < 0 || (m > 0 && !a) || (n > 0 && !b)) {
+        set_error("sl_llcs_hyyro: bad argument");
+        return kErrArg;
+    }
+    static const uint32_t dummy = 0;
+    const int64_t off = 0;
+    int32_t out = 0;
+    const int rc = sl_llcs_hyyro_batch(1, m ? a : &dummy, &off, &m, n ? b : &dummy, &off, &n, sym_bytes, &out);
+    if (!rc) *llcs = out;
+    return rc;
+}
+
+}  // extern "C"

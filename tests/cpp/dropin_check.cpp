@@ -13,7 +13,8 @@
 //   namespaced: semi_local::OpenMPIterativeCombing, OpenMPHybridIterative,
 //               common::SequentialMemoryOptimizedStickBraidMultiplication,
 //               lcs::LCSBySemiLocal over either combing strategy,
-//               InteractiveDuplicateSearchSemiDense / SemiSparse::find.
+//               InteractiveDuplicateSearchSemiDense / SemiSparse::find,
+//               bit_parallel::lcs::Hyyro, lllcs_hyyro::hyyro_magic(_mpi_with_precalc).
 // Prints one line per check and "dropin_check: N checks, M failures"; exits
 // non-zero on any failure.  Needs a GPU to run.
 #include <algorithm>
@@ -21,6 +22,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <optional>
 #include <random>
 #include <tuple>
 #include <unordered_map>
@@ -34,6 +36,7 @@
 #include "include/SemiLocal/SemiLocalLCS.h"
 #include "approximate_matching/utils.h"
 #include "approximate_matching/ApproximateMatching.h"
+#include "include/BitwiseLCS/BitParalellLCS.h"
 
 #include "semilocal_hip_strategies.hpp"
 #include "semilocal_hip_amatch.hpp"
@@ -298,6 +301,28 @@ int main() {
         std::snprintf(name, sizeof name, "InteractiveDuplicateSearchSemiSparse m=%d n=%d k=%.2f (%zu found)", m, n, k,
                       ref_s.size());
         report(name, ref_s == hip_s);
+    }
+
+    // Hyyro's bit-vector LLCS: namespaced strategy and the flat entry points
+    for (auto [m, n, sigma] : std::vector<std::tuple<int, int, int>>{
+             {1, 1, 2}, {63, 65, 4}, {700, 3000, 4}, {5000, 70000, 20}, {2000, 130000, 4}}) {
+        auto a = symbols(g, m, sigma), b = symbols(g, n, sigma);
+        bit_parallel::lcs::Hyyro<uint64_t, int> cpu;
+        semilocal_hip::HipHyyro<uint64_t, int> dev;
+        const int want = cpu.compute(a.data(), m, b.data(), n);
+        std::snprintf(name, sizeof name, "bit_parallel::lcs::Hyyro vs HipHyyro m=%d n=%d sigma=%d", m, n, sigma);
+        report(name, dev.compute(a.data(), m, b.data(), n) == want);
+        std::unordered_set<int> alphabet(a.begin(), a.end());
+        std::unordered_map<int, uint32_t *> lookup;
+        lllcs_hyyro::preprocess<uint32_t>(b.data(), n, alphabet, lookup);
+        const int words = (n + 31) / 32;
+        const int ref_flat = lllcs_hyyro::hyyro_magic<uint32_t>(a.data(), m, words, lookup);
+        const int dev_flat = semilocal_hip::lllcs_hyyro::hyyro_magic<uint32_t>(a.data(), m, words, lookup);
+        const int dev_mpi =
+            semilocal_hip::lllcs_hyyro::hyyro_magic_mpi_with_precalc<uint32_t>(a.data(), m, words, lookup, 4);
+        std::snprintf(name, sizeof name, "lllcs_hyyro::hyyro_magic<uint32_t> m=%d n=%d", m, n);
+        report(name, ref_flat == want && dev_flat == want && dev_mpi == want);
+        for (auto &kv : lookup) delete[] kv.second;
     }
 
     std::printf("dropin_check: %d checks, %d failures\n", checks, failures);

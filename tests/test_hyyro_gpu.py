@@ -1,0 +1,71 @@
+"""GPU parity of the Hyyro bit-vector LLCS (sl_llcs_hyyro, sl_hyyro_*):
+scores equal plain LCS dynamic programming (oracle) across the launch classes
+-- one wave with 1..16 words per lane (packed string up to 65,536 symbols)
+and 16-wave blocks (up to 2^20) -- at word, lane and wave boundaries, for 1-,
+2- and 4-byte symbols; at full size they equal the semi-local kernel's LLCS.
+Bar: exact integers."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# packed-string lengths straddling the class boundaries: 64 words per lane-set,
+# 4096 / 8192 / 16384 / 32768 / 65536 columns per wave, then 16 waves
+BOUNDARY_N = [1, 63, 64, 65, 4095, 4096, 4097, 8192, 8193, 16385, 32769, 65536, 65537, 131071]
+
+
+@pytest.mark.parametrize("n", BOUNDARY_N)
+def test_single_pairs_at_class_boundaries_equal_dp(gpu, oracle, n):
+    g = np.random.default_rng(n)
+    m = int(min(700, max(1, 4_000_000 // n)))
+    a = g.integers(0, 4, m).astype(np.uint8)
+    b = g.integers(0, 4, n).astype(np.uint8)
+    print(f"hyyro m={m} n={n}", flush=True)
+    assert gpu.llcs_hyyro(a, b) == oracle.llcs_dp(a, b)
+    assert gpu.llcs_hyyro(b, a) == oracle.llcs_dp(a, b)     # the longer string is the packed one
+
+
+def test_batch_of_ragged_pairs_equals_dp(gpu, oracle):
+    g = np.random.default_rng(3)
+    pairs = []
+    for _ in range(300):
+        m, n = int(g.integers(0, 900)), int(g.integers(0, 5000))
+        sigma = int(g.choice([2, 4, 20, 200]))
+        pairs.append((g.integers(0, sigma, m).astype(np.uint8), g.integers(0, sigma, n).astype(np.uint8)))
+    hb = gpu.HyyroBatch(pairs)
+    got = hb.run()
+    assert got.tolist() == [oracle.llcs_dp(a, b) for a, b in pairs]
+    again = hb.run()                                          # re-runs reset their state
+    assert np.array_equal(got, again)
+    assert hb.time(warmup=0, iters=2) > 0 and hb.stats()["word_rows"] > 0
+    hb.close()
+
+
+@pytest.mark.parametrize("dtype,sigma", [(np.uint16, 256), (np.int32, 37), (np.int32, 2)])
+def test_wide_symbols_equal_dp(gpu, oracle, dtype, sigma):
+    g = np.random.default_rng(sigma)
+    vals = g.choice(np.arange(-100_000, 100_000), sigma, replace=False) if dtype == np.int32 else np.arange(sigma)
+    a = vals[g.integers(0, sigma, 1500)].astype(dtype)
+    b = vals[g.integers(0, sigma, 9000)].astype(dtype)
+    assert gpu.llcs_hyyro(a, b) == oracle.llcs_dp(a.astype(np.int64).astype(np.int32), b.astype(np.int32))
+
+
+def test_large_pairs_equal_the_semi_local_kernel(gpu):
+    g = np.random.default_rng(9)
+    for m, n in [(3000, 1 << 20), (20_000, 300_000)]:
+        a = g.integers(0, 4, m).astype(np.uint8)
+        b = a[g.integers(0, m, n)] if n < m else g.integers(0, 4, n).astype(np.uint8)
+        print(f"hyyro large m={m} n={n}", flush=True)
+        perm = gpu.semi_local_comb(a, b)
+        assert gpu.llcs_hyyro(a, b) == gpu.llcs_from_kernel(perm, m, n)
+
+
+def test_unsupported_inputs_are_rejected(gpu):
+    g = np.random.default_rng(1)
+    a = np.arange(300, dtype=np.int32)
+    with pytest.raises(gpu.SemiLocalError):
+        gpu.llcs_hyyro(a, a)                                  # 300 distinct symbols
+    with pytest.raises(gpu.SemiLocalError):
+        gpu.llcs_hyyro(g.integers(0, 4, 10).astype(np.uint8), np.zeros((1 << 20) + 1, np.uint8))
+    assert gpu.llcs_hyyro(np.zeros(0, np.uint8), np.ones(5, np.uint8)) == 0
+    assert gpu.llcs_hyyro(np.array([1, 2, 3], np.uint8), np.array([4, 5], np.uint8)) == 0
