@@ -274,6 +274,75 @@ def cpu_baseline_amatch(k: float = 0.9, n_sample: int = 300_000):
                       f"sequential combing"}
 
 
+def bench_hyyro(S, W, pairs=None):
+    """Score-only LLCS by Hyyro's bit-vector recurrence: the ragged batch's
+    4096 pairs (LLCS of every pair), and one 20,000 x 1,000,000 pair."""
+    pairs = pairs if pairs is not None else W.ragged_batch()
+    t0 = time.perf_counter()
+    hb = S.HyyroBatch(pairs)
+    create_s = time.perf_counter() - t0
+    scores = hb.run()
+    ms = hb.time(warmup=1, iters=5)
+    st = hb.stats()
+    hb.close()
+    cells = int(sum(len(a) * len(b) for a, b in pairs))
+    g = np.random.default_rng(17)
+    a = g.integers(0, 4, 20_000).astype(np.uint8)
+    b = g.integers(0, 4, 1_000_000).astype(np.uint8)
+    one = S.HyyroBatch([(a, b)])
+    one_ms = one.time(warmup=1, iters=3)
+    one_st = one.stats()
+    one.close()
+    # short pairs: 4096 x (2000 x 2000), alphabet 20 -- Hyyro's LLCS against
+    # the semi-local batch's full kernels (default tile shape) on the same pairs
+    short = [(g.integers(0, 20, 2000).astype(np.uint8), g.integers(0, 20, 2000).astype(np.uint8))
+             for _ in range(4096)]
+    hs = S.HyyroBatch(short)
+    short_ms = hs.time(warmup=1, iters=5)
+    short_llcs = hs.run()
+    hs.close()
+    sb = S.Batch(short)
+    sb_ms, _ = sb.time(per_pair=False, warmup=1, iters=5)
+    sb_llcs = [S.llcs_from_kernel(r, 2000, 2000) for r in sb.run().results()]
+    sb.close()
+    short_cells = 4096 * 2000 * 2000
+    return {"workload": "LLCS of the ragged batch's 4096 pairs (lengths log-normal, alphabet 20); one 20,000 x "
+                        "1,000,000 pair (alphabet 4); 4096 pairs of 2000 x 2000 (alphabet 20)",
+            "batch_create_s_incl_recode_upload_tables": round(create_s, 3),
+            "batch_ms": round(ms, 3), "batch_cells": cells, "batch_gcups": round(cells / (ms * 1e-3) / 1e9, 1),
+            "batch_word_rows": st["word_rows"],
+            "batch_word_rows_per_s": round(st["word_rows"] / (ms * 1e-3), 1),
+            "batch_llcs_sum": int(scores.sum()),
+            "long_pair_ms": round(one_ms, 2), "long_pair_gcups": round(2e10 / (one_ms * 1e-3) / 1e9, 1),
+            "long_pair_word_rows": one_st["word_rows"],
+            "short_pairs_4096x2000x2000_ms": round(short_ms, 3),
+            "short_pairs_gcups": round(short_cells / (short_ms * 1e-3) / 1e9, 1),
+            "short_pairs_semi_local_batch_ms": round(sb_ms, 3),
+            "short_pairs_llcs_equal_semi_local": bool(np.array_equal(short_llcs, np.array(sb_llcs)))}
+
+
+def cpu_baseline_hyyro(pairs, target_s: float = 5.0):
+    """The reference's lllcs_hyyro (preprocess + hyyro_magic<uint64_t>,
+    compiled in place, one thread) on a seeded sample of the same pairs;
+    the oracle's DP when the reference build is absent."""
+    from oracle.binding import Oracle, Reference
+
+    eng, kind = (Reference(), "reference") if Reference.available() else (Oracle(), "port")
+    fn = eng.hyyro if kind == "reference" else eng.llcs_dp
+    order = np.random.default_rng(98).permutation(len(pairs))
+    cells, done, t0 = 0, 0, time.perf_counter()
+    for i in order:
+        a, b = pairs[i]
+        fn(a.astype(np.int32), b.astype(np.int32))
+        cells += len(a) * len(b)
+        done += 1
+        if time.perf_counter() - t0 > target_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(cells / dt / 1e9, 2), "unit": "GCUPS", "cores": 1, "kind": kind,
+            "sample": f"{done} of the {len(pairs)} pairs (seeded random order), {cells:.3e} cells, {dt:.1f} s"}
+
+
 def bench_hybrid(S, W, golden):
     """Config 3: 8x8 blocks combed in one launch, merged by 6 levels of batched
     steady-ant multiplications."""
@@ -363,7 +432,8 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
     ap.add_argument("--no-extras", action="store_true", help="only the ragged batch (configs 1, 2, 3, 5 skipped)")
-    ap.add_argument("--only", choices=["plain", "binary", "hybrid", "multiply", "index", "serving", "amatch"],
+    ap.add_argument("--only", choices=["plain", "binary", "hybrid", "multiply", "index", "serving", "amatch",
+                                       "hyyro"],
                     help="run one extra section alone (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
@@ -388,6 +458,7 @@ def main():
               "multiply": lambda: bench_multiply(semilocal_amd),
               "index": lambda: bench_index(semilocal_amd, W),
               "amatch": lambda: bench_amatch(semilocal_amd),
+              "hyyro": lambda: bench_hyyro(semilocal_amd, W),
               "serving": lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
                                                args.serve_pool_gb)}[args.only]
         print(json.dumps({"section": args.only, **fn()}), flush=True)
@@ -495,6 +566,7 @@ def main():
                         ("braid_multiply_2p20", lambda: bench_multiply(semilocal_amd)),
                         ("braid_index_queries", lambda: bench_index(semilocal_amd, W)),
                         ("approximate_matching_1e6", lambda: bench_amatch(semilocal_amd)),
+                        ("hyyro_llcs", lambda: bench_hyyro(semilocal_amd, W, pairs)),
                         ("serving", lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
                                                           args.serve_pool_gb))):
             t0 = time.perf_counter()
@@ -504,6 +576,9 @@ def main():
         cb = cpu_baseline(pairs)
         log("cpu baseline:", cb)
         out["cpu_baseline"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in cb.items() if k != "seconds"}
+        if "hyyro_llcs" in out:
+            out["hyyro_llcs"]["cpu_reference"] = cpu_baseline_hyyro(pairs)
+            log("hyyro cpu reference:", out["hyyro_llcs"]["cpu_reference"])
         if "approximate_matching_1e6" in out:
             out["approximate_matching_1e6"]["cpu_reference"] = cpu_baseline_amatch()
             log("approximate matching cpu reference:", out["approximate_matching_1e6"]["cpu_reference"])

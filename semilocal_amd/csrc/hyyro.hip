@@ -10,19 +10,22 @@
 // BitParalellLCS.h:337-433, prefix_lcs/bitwise/transposition_network_binary_
 // alphabet.h:743-916: hyyro_magic and its anti-diagonal form).
 //
-// The n-bit addition is the only coupling between words.  Lane l of a wave
-// holds R consecutive words; a row is two passes over them:
-//   1. with carry-in 0, each lane learns whether it generates a carry (g) and
-//      whether it is all ones after the add, so a carry would pass through (q);
-//   2. two ballots G, X = G | Q and one 64-bit add give every lane its carry-in
-//      (wave64 carry-lookahead: carry into lane l = bit l of (X + G + c) ^ X ^ G),
-//      and the lane redoes its words with it.
-// Pairs up to 65,536 columns run one wave each; longer ones (up to 2^20
-// columns) one 16-wave workgroup, whose waves fold their (carry-out if 0,
-// carry-out if 1) pairs through LDS once per row.  Masks come from a table
-// built on the device (symbols of b: at most 256 per pair), prefetched D rows
-// ahead; rows whose symbol does not occur in b are dropped on the host (their
-// p is 0 and they leave V unchanged).
+// The n-bit addition is the only coupling between words; two kernels handle
+// it.
+//  * hy_rows (pairs up to 64 words: one wave): lane l holds word l and a row
+//    is two passes -- with carry-in 0 each lane learns whether it generates
+//    a carry (g) and whether it is all ones after the add (q); two ballots
+//    G, X = G | Q and one 64-bit add give every lane its carry-in (wave64
+//    carry-lookahead: carry into lane l = bit l of (X + G) ^ X ^ G), and the
+//    lane redoes its word with it.
+//  * hy_sys (wider pairs, up to 2^20 columns): systolic -- lane L holds R
+//    words and processes row t - L at step t, taking its carry from lane
+//    L - 1 of the previous step by one DPP move; carries cross waves through
+//    LDS once per 64 steps (the waves run one 64-step group apart).
+// Masks come from a table built on the device (symbols of the packed string:
+// at most 256 per pair) and are prefetched D rows ahead; rows whose symbol
+// does not occur in the packed string are dropped on the host (their p is 0
+// and they leave V unchanged).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,8 +40,7 @@ namespace slhip {
 namespace hy {
 
 constexpr int kMaxSymbols = 256;
-constexpr int kBlockWaves = 16;
-constexpr int kMaxColumns = kBlockWaves * 64 * 64 * 16;   // 2^20
+constexpr int kMaxColumns = 16 * 64 * 64 * 16;   // 16 waves x 64 lanes x 16 words: 2^20
 
 struct HyPair {
     int64_t rows_off;   // row symbols (int32 codes into the pair's table) in the row pool
@@ -76,25 +78,31 @@ __device__ __forceinline__ uint64_t add_c(uint64_t x, uint64_t y, int &c) {
 template <int R, int D, int NW>
 __global__ __launch_bounds__(64 * NW) void hy_rows(const HyPair *__restrict__ pairs, const int32_t *__restrict__ list,
                                                    const int32_t *__restrict__ rows,
-                                                   const uint64_t *__restrict__ tab, int32_t *__restrict__ llcs) {
-    __shared__ uint32_t fold[2][NW];   // per row parity: wave's (carry-out if 0) | (carry-out if 1) << 1
-    __shared__ int zsum[NW];
+                                                   const uint64_t *__restrict__ tab, const uint64_t *__restrict__ zero,
+                                                   int32_t *__restrict__ llcs) {
+    static_assert(64 % D == 0, "the row ring must divide the 64-row symbol chunks");
+    static_assert(NW == 1, "one wave per pair: the carry-in of lane 0 is 0");
     const HyPair P = pairs[list[blockIdx.x]];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int w0 = (wave * 64 + lane) * R;   // first word of this lane
+    const int lane = threadIdx.x & 63;
+    const int w0 = lane * R;   // first word of this lane
     const uint64_t *T = tab + P.tab_off;
     const int32_t *A = rows + P.rows_off;
     const int K = P.K, m = P.m;
     uint64_t V[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) V[r] = ~0ull;
+    // row symbols, 64 per lane-parallel load: rows [c, c + 64) and [c + 64, c + 128)
+    // loads are unconditional (see hy_sys): symbols of rows past m are read
+    // from a clamped index and never used
+    int scur = A[lane < m ? lane : 0];
+    int snxt = A[64 + lane < m ? 64 + lane : 0];
     // mask ring: masks of rows i .. i + D - 1
     uint64_t Mq[D][R];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const int s = d < m ? A[d] : 0;
+        const int sd = __builtin_amdgcn_readlane(scur, d);
 #pragma unroll
-        for (int r = 0; r < R; ++r) Mq[d][r] = (d < m && w0 + r < K) ? T[(int64_t)s * K + w0 + r] : 0ull;
+        for (int r = 0; r < R; ++r) Mq[d][r] = *(d < m && w0 + r < K ? T + (int64_t)sd * K + w0 + r : zero);
     }
     for (int i0 = 0; i0 < m; i0 += D) {
 #pragma unroll
@@ -111,35 +119,30 @@ __global__ __launch_bounds__(64 * NW) void hy_rows(const HyPair *__restrict__ pa
                 }
                 const uint64_t G = __ballot(c);
                 const uint64_t X = G | __ballot(all_ones);
-                int cin = 0;
-                if (NW > 1) {
-                    // the wave's carry-out for carry-in 0 and 1, folded over
-                    // the lower waves of the block through LDS
-                    const uint64_t s0 = X + G;
-                    const int co0 = s0 < X ? 1 : 0;
-                    const int co1 = co0 | (s0 + 1 == 0 ? 1 : 0);
-                    if (lane == 0) fold[i & 1][wave] = (uint32_t)co0 | ((uint32_t)co1 << 1);
-                    __syncthreads();
-                    for (int j = 0; j < wave; ++j) {
-                        const uint32_t f = fold[i & 1][j];
-                        cin = cin ? (int)(f >> 1) : (int)(f & 1u);
-                    }
-                }
-                const uint64_t C = (X + G + (uint64_t)cin) ^ X ^ G;
+                const uint64_t C = (X + G) ^ X ^ G;
                 c = (int)((C >> lane) & 1ull);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const uint64_t t = add_c(V[r], p[r], c);
                     V[r] = (V[r] ^ p[r]) | t;
                 }
-                // refill this ring slot with row i + D
+                // refill this ring slot with row i + D (its symbol from the
+                // lane buffers: (i + D) - c lies in [D, 64 + D) where c is the
+                // 64-aligned chunk holding i)
                 const int nx = i + D;
                 if (nx < m) {
-                    const int s = A[nx];
+                    const int off = nx & 63;
+                    const int sx = (nx >> 6) == (i >> 6) ? __builtin_amdgcn_readlane(scur, off)
+                                                         : __builtin_amdgcn_readlane(snxt, off);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) Mq[d][r] = w0 + r < K ? T[(int64_t)s * K + w0 + r] : 0ull;
+                    for (int r = 0; r < R; ++r) Mq[d][r] = *(w0 + r < K ? T + (int64_t)sx * K + w0 + r : zero);
                 }
             }
+        }
+        if (((i0 + D) & 63) == 0) {   // next chunk of 64 rows: shift the symbol buffers
+            scur = snxt;
+            const int nr = i0 + D + 64 + lane;
+            snxt = A[nr < m ? nr : 0];
         }
     }
     // LLCS = zero bits of V (padding bits beyond n stay one)
@@ -148,16 +151,119 @@ __global__ __launch_bounds__(64 * NW) void hy_rows(const HyPair *__restrict__ pa
     for (int r = 0; r < R; ++r) z += 64 - __popcll(V[r]);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
-    if (NW == 1) {
-        if (lane == 0) llcs[P.out] = z;
-    } else {
-        if (lane == 0) zsum[wave] = z;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int tot = 0;
-            for (int j = 0; j < NW; ++j) tot += zsum[j];
-            llcs[P.out] = tot;
+    if (lane == 0) llcs[P.out] = z;
+}
+
+// Systolic form for wide pairs.  Lane L (0 .. 64 NW - 1) owns words
+// [L R, L R + R) and processes row i = t - L at step t, so its carry-in is
+// the carry-out of lane L - 1 at step t - 1 (same row): one DPP wave_shr:1
+// per step inside a wave.  Across waves the carries of lane 63 are collected
+// per group of 64 steps into a 64-bit word in LDS; wave w runs the steps
+// [64 (g - w), 64 (g - w) + 64) in group g (one group behind wave w - 1), so
+// everything it reads was written in groups g - 1 and g - 2 and one barrier
+// per group suffices.  Masks of the row a lane meets at step t + D are
+// gathered at step t into ring slot t mod D (uniform across lanes), their
+// symbols D steps earlier still.
+constexpr int kDppWaveShr1 = 0x138;
+
+template <int R, int D, int NW>
+__global__ __launch_bounds__(64 * NW) void hy_sys(const HyPair *__restrict__ pairs, const int32_t *__restrict__ list,
+                                                  const int32_t *__restrict__ rows, const uint64_t *__restrict__ tab,
+                                                  const uint64_t *__restrict__ zero, int32_t *__restrict__ llcs) {
+    static_assert(64 % D == 0, "the ring must divide a group of 64 steps");
+    __shared__ uint64_t hand[3][NW];   // lane 63's carry-outs per group (slot g mod 3)
+    __shared__ int zsum[NW];
+    const HyPair P = pairs[list[blockIdx.x]];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int L = wave * 64 + lane;
+    const int w0 = L * R;
+    const uint64_t *T = tab + P.tab_off;
+    const int32_t *A = rows + P.rows_off;
+    const int K = P.K, m = P.m;
+    uint64_t V[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) V[r] = ~0ull;
+    uint64_t Mq[D][R];
+    int Sq[D];
+    // step t of this wave is 64 (g - wave) + j; prime the rings for its first
+    // 2D steps (t = -64 wave + 0 .. 2D - 1): rows below 0 have no masks
+    const int t_first = -64 * wave;
+    // Every load below is unconditional and its value is used as loaded:
+    // masks of rows outside [0, m) or words past K come from `zero` (a word
+    // holding 0 after the table pool), symbols of such rows from a clamped
+    // index (they are never used).  Guarded loads would become branches, and
+    // masking a loaded value makes the compiler wait for the load at once;
+    // either empties the prefetch rings every step.
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int i = t_first + d - L;
+        const bool ok = i >= 0 && i < m;
+        const int s = A[ok ? i : 0];
+#pragma unroll
+        for (int r = 0; r < R; ++r) Mq[d][r] = *(ok && w0 + r < K ? T + (int64_t)s * K + w0 + r : zero);
+        const int i2 = i + D;
+        Sq[d] = A[i2 >= 0 && i2 < m ? i2 : 0];
+    }
+    const int ngroups = (NW - 1) + (m + 64 * NW - 1 + 63) / 64 + 1;
+    int cprev = 0;   // this lane's carry-out at the previous step
+    for (int g = 0; g < ngroups; ++g) {
+        const int t0 = 64 * (g - wave);
+        uint64_t in = 0;
+        if (wave > 0) {
+            const uint64_t prev1 = g >= 1 ? hand[(g - 1) % 3][wave - 1] : 0ull;
+            const uint64_t prev2 = g >= 2 ? hand[(g - 2) % 3][wave - 1] : 0ull;
+            in = (prev1 << 1) | (prev2 >> 63);
         }
+        uint64_t outbits = 0;
+        for (int j0 = 0; j0 < 64; j0 += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int j = j0 + d;
+                const int t = t0 + j;
+                const int i = t - L;
+                const int from_left = __builtin_amdgcn_update_dpp(0, cprev, kDppWaveShr1, 0xf, 0xf, false);
+                int c = lane == 0 ? (int)((in >> j) & 1ull) : from_left;
+                if (i >= 0 && i < m) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const uint64_t p = Mq[d][r] & V[r];
+                        const uint64_t t1 = V[r] + p;
+                        const int o1 = t1 < V[r] ? 1 : 0;
+                        const uint64_t t2 = t1 + (uint64_t)c;
+                        const int o2 = t2 < t1 ? 1 : 0;
+                        V[r] = (V[r] ^ p) | t2;
+                        c = o1 | o2;
+                    }
+                } else {
+                    c = 0;
+                }
+                cprev = c;
+                outbits |= (uint64_t)(__builtin_amdgcn_readlane(c, 63) & 1) << j;
+                // refill: masks for step t + D (symbol already in Sq[d]),
+                // symbol for step t + 2D
+                const int in_ = i + D;
+                const bool ok = in_ >= 0 && in_ < m;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    Mq[d][r] = *(ok && w0 + r < K ? T + (int64_t)Sq[d] * K + w0 + r : zero);
+                const int i2 = i + 2 * D;
+                Sq[d] = A[i2 >= 0 && i2 < m ? i2 : 0];
+            }
+        }
+        if (lane == 0) hand[g % 3][wave] = outbits;
+        __syncthreads();
+    }
+    int z = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) z += 64 - __popcll(V[r]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
+    if (lane == 0) zsum[wave] = z;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int j = 0; j < NW; ++j) tot += zsum[j];
+        llcs[P.out] = tot;
     }
 }
 
@@ -184,16 +290,20 @@ struct Hyyro {
 
 namespace {
 
-// class id: 0..4 = one wave with R = 1, 2, 4, 8, 16; 5 = 16 waves with R = 16
+// Launch classes by packed width K (words).  Up to 64 words: one wave of
+// the lookahead kernel (hy_rows), the faster form for short pairs.  Wider:
+// the systolic kernel (hy_sys) on 2 or 4 waves with one word per lane up to
+// 256 words, then 4 waves with 2 .. 16 words per lane, then 8 and 16 waves
+// with 16.  Wider lanes amortise the per-step fixed cost (the carry move, the
+// row's symbol and mask gathers) over more words; four waves (one per SIMD)
+// keep one pair's steps issue-light on its compute unit.
 int hy_class(int K) {
-    if (K <= 64) return 0;
-    if (K <= 128) return 1;
-    if (K <= 256) return 2;
-    if (K <= 512) return 3;
-    if (K <= 1024) return 4;
-    return 5;
+    static const int limits[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384};
+    int c = 0;
+    while (K > limits[c]) ++c;
+    return c;
 }
-constexpr int kClasses = 6;
+constexpr int kClasses = 9;
 
 void hy_free(Hyyro *h) {
     if (!h) return;
@@ -268,16 +378,19 @@ int hy_launch(Hyyro *h) {
         const unsigned cnt = (unsigned)h->cls_list[(size_t)c].size();
         if (!cnt) continue;
         const int32_t *lst = h->d_list + h->list_off[(size_t)c];
-#define SL_HY_LAUNCH(R, D, NW)                                                                              \
-    hipLaunchKernelGGL((hy::hy_rows<R, D, NW>), dim3(cnt), dim3(64 * NW), 0, h->stream, h->d_pairs, lst, h->d_rows, \
-                       h->d_tab, h->d_llcs)
+#define SL_HY_LAUNCH(KERNEL, R, D, NW)                                                                       \
+    hipLaunchKernelGGL((hy::KERNEL<R, D, NW>), dim3(cnt), dim3(64 * NW), 0, h->stream, h->d_pairs, lst, h->d_rows, \
+                       h->d_tab, h->d_tab + h->tab_words, h->d_llcs)
         switch (c) {
-            case 0: SL_HY_LAUNCH(1, 8, 1); break;
-            case 1: SL_HY_LAUNCH(2, 8, 1); break;
-            case 2: SL_HY_LAUNCH(4, 4, 1); break;
-            case 3: SL_HY_LAUNCH(8, 2, 1); break;
-            case 4: SL_HY_LAUNCH(16, 2, 1); break;
-            default: SL_HY_LAUNCH(16, 2, hy::kBlockWaves); break;
+            case 0: SL_HY_LAUNCH(hy_rows, 1, 16, 1); break;   // one wave: the lookahead form is faster
+            case 1: SL_HY_LAUNCH(hy_sys, 1, 32, 2); break;
+            case 2: SL_HY_LAUNCH(hy_sys, 1, 32, 4); break;
+            case 3: SL_HY_LAUNCH(hy_sys, 2, 16, 4); break;
+            case 4: SL_HY_LAUNCH(hy_sys, 4, 8, 4); break;
+            case 5: SL_HY_LAUNCH(hy_sys, 8, 4, 4); break;
+            case 6: SL_HY_LAUNCH(hy_sys, 16, 2, 4); break;
+            case 7: SL_HY_LAUNCH(hy_sys, 16, 2, 8); break;
+            default: SL_HY_LAUNCH(hy_sys, 16, 2, 16); break;
         }
 #undef SL_HY_LAUNCH
     }
@@ -359,9 +472,9 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     };
     alloc((void **)&h->d_pairs, sizeof(hy::HyPair) * h->pairs.size());
     alloc((void **)&h->d_list, sizeof(int32_t) * all_lists.size());
-    alloc((void **)&h->d_rows, sizeof(int32_t) * rows.size());
+    alloc((void **)&h->d_rows, sizeof(int32_t) * (rows.size() + 1));   // + a spare: clamped loads read index 0
     alloc((void **)&h->d_cols, cols.size());
-    alloc((void **)&h->d_tab, sizeof(uint64_t) * (size_t)h->tab_words);
+    alloc((void **)&h->d_tab, sizeof(uint64_t) * (size_t)(h->tab_words + 1));   // + the zero word
     alloc((void **)&h->d_llcs, sizeof(int32_t) * (size_t)std::max(npairs, 1));
     if (e == hipSuccess && !h->pairs.empty())
         e = hipMemcpy(h->d_pairs, h->pairs.data(), sizeof(hy::HyPair) * h->pairs.size(), hipMemcpyHostToDevice);
@@ -370,6 +483,8 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     if (e == hipSuccess && !rows.empty())
         e = hipMemcpy(h->d_rows, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !cols.empty()) e = hipMemcpy(h->d_cols, cols.data(), cols.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)   // the zero word after the table pool (masks of rows and words outside a pair)
+        e = hipMemsetAsync(h->d_tab + h->tab_words, 0, sizeof(uint64_t), h->stream);
     if (e == hipSuccess && npairs > 0) {
         hipLaunchKernelGGL(hy::hy_table, dim3((unsigned)npairs), dim3(256), 0, h->stream, h->d_pairs, h->d_cols,
                            h->d_tab);

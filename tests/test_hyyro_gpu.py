@@ -1,17 +1,18 @@
 """GPU parity of the Hyyro bit-vector LLCS (sl_llcs_hyyro, sl_hyyro_*):
-scores equal plain LCS dynamic programming (oracle) across the launch classes
--- one wave with 1..16 words per lane (packed string up to 65,536 symbols)
-and 16-wave blocks (up to 2^20) -- at word, lane and wave boundaries, for 1-,
-2- and 4-byte symbols; at full size they equal the semi-local kernel's LLCS.
-Bar: exact integers."""
+scores equal plain LCS dynamic programming (oracle) across the launch
+classes -- the one-wave lookahead kernel up to 64 words, the systolic kernel
+on 2 and 4 waves with 1..16 words per lane and on 8 and 16 waves beyond --
+at word, lane and wave boundaries, for 1-, 2- and 4-byte symbols; at full
+size they equal the semi-local kernel's LLCS.  Bar: exact integers."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-# packed-string lengths straddling the class boundaries: 64 words per lane-set,
-# 4096 / 8192 / 16384 / 32768 / 65536 columns per wave, then 16 waves
-BOUNDARY_N = [1, 63, 64, 65, 4095, 4096, 4097, 8192, 8193, 16385, 32769, 65536, 65537, 131071]
+# packed-string lengths straddling the launch-class boundaries (64, 128, 256,
+# 512, 1,024, 2,048, 4,096, 8,192 words)
+BOUNDARY_N = [1, 63, 64, 65, 4095, 4096, 4097, 8192, 8193, 16384, 16385, 32769, 65536, 65537, 131071, 131073,
+              262144, 262145, 524288, 524289]
 
 
 @pytest.mark.parametrize("n", BOUNDARY_N)
