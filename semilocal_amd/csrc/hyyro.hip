@@ -286,6 +286,11 @@ struct Hyyro {
     int32_t *d_llcs = nullptr;
     int64_t tab_words = 0, word_rows = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // launch classes are independent: each runs on a stream of its own
+    // (default priority), forked from and joined back into `stream`
+    std::vector<hipStream_t> cls_stream;
+    std::vector<hipEvent_t> cls_done;
+    hipEvent_t fork = nullptr;
 };
 
 namespace {
@@ -315,6 +320,11 @@ void hy_free(Hyyro *h) {
     (void)hipFree(h->d_llcs);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : h->cls_done)
+        if (e) (void)hipEventDestroy(e);
+    if (h->fork) (void)hipEventDestroy(h->fork);
+    for (auto &cs : h->cls_stream)
+        if (cs) (void)hipStreamDestroy(cs);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -374,12 +384,16 @@ int hy_encode_u32(const uint32_t *a, int m, const uint32_t *b, int n, std::vecto
 }
 
 int hy_launch(Hyyro *h) {
-    for (int c = 0; c < kClasses; ++c) {
+    SL_HIP_TRY(hipEventRecord(h->fork, h->stream));
+    // widest class first: it holds the longest single pairs
+    for (int c = kClasses - 1; c >= 0; --c) {
         const unsigned cnt = (unsigned)h->cls_list[(size_t)c].size();
         if (!cnt) continue;
         const int32_t *lst = h->d_list + h->list_off[(size_t)c];
+        hipStream_t cs = h->cls_stream[(size_t)c];
+        SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
 #define SL_HY_LAUNCH(KERNEL, R, D, NW)                                                                       \
-    hipLaunchKernelGGL((hy::KERNEL<R, D, NW>), dim3(cnt), dim3(64 * NW), 0, h->stream, h->d_pairs, lst, h->d_rows, \
+    hipLaunchKernelGGL((hy::KERNEL<R, D, NW>), dim3(cnt), dim3(64 * NW), 0, cs, h->d_pairs, lst, h->d_rows, \
                        h->d_tab, h->d_tab + h->tab_words, h->d_llcs)
         switch (c) {
             case 0: SL_HY_LAUNCH(hy_rows, 1, 16, 1); break;   // one wave: the lookahead form is faster
@@ -393,8 +407,10 @@ int hy_launch(Hyyro *h) {
             default: SL_HY_LAUNCH(hy_sys, 16, 2, 16); break;
         }
 #undef SL_HY_LAUNCH
+        SL_HIP_TRY(hipGetLastError());
+        SL_HIP_TRY(hipEventRecord(h->cls_done[(size_t)c], cs));
+        SL_HIP_TRY(hipStreamWaitEvent(h->stream, h->cls_done[(size_t)c], 0));
     }
-    SL_HIP_TRY(hipGetLastError());
     return kOk;
 }
 
@@ -467,6 +483,14 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     for (auto &ev : h->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
+    h->cls_stream.assign(kClasses, nullptr);
+    h->cls_done.assign(kClasses, nullptr);
+    for (int c = 0; c < kClasses && e == hipSuccess; ++c) {
+        if (h->cls_list[(size_t)c].empty()) continue;
+        e = hipStreamCreateWithFlags(&h->cls_stream[(size_t)c], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->cls_done[(size_t)c], hipEventDisableTiming);
+    }
     auto alloc = [&](void **p, size_t bytes) {
         if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 8));
     };
