@@ -37,6 +37,9 @@
 #include "semilocal_hip.h"
 
 namespace slhip {
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t stream);
+
 namespace hy {
 
 constexpr int kMaxSymbols = 256;
@@ -267,6 +270,179 @@ __global__ __launch_bounds__(64 * NW) void hy_sys(const HyPair *__restrict__ pai
     }
 }
 
+// Segmented systolic form for pairs wider than 256 words: the packed string
+// is cut into segments of 256 R words, each a 4-wave workgroup running the
+// systolic scheme of hy_sys with its global wave index Wg = 4 seg + wave (so
+// lanes, rows and the one-group skew continue across segments).  The carry
+// into a segment's lane 0 is the previous segment's lane-63 output: its last
+// wave publishes, per 64-step group, the 64 carry bits as two tagged 64-bit
+// entries (high word: (epoch << 16) | group, low word: 32 carry bits),
+// written and read with agent-scope atomics like the combing kernel's strand
+// hand-off, so a reader that sees the tag has the bits.  Workgroups are
+// persistent and take (pair, segment) tickets in order: a segment waits only
+// for the segment before it, whose ticket a running workgroup already holds.
+// The wait is bounded; running out sets ctl->error and the run reports a
+// timeout.
+struct SegItem {
+    int32_t pair, seg;
+};
+
+constexpr int kSegWaves = 4;
+
+__device__ __forceinline__ uint64_t seg_ld(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void seg_st(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int R, int D>
+__global__ __launch_bounds__(64 * kSegWaves) void hy_seg(const HyPair *__restrict__ pairs,
+                                                         const SegItem *__restrict__ items, uint32_t nitems,
+                                                         const int64_t *__restrict__ xoff,
+                                                         const int32_t *__restrict__ rows,
+                                                         const uint64_t *__restrict__ tab,
+                                                         const uint64_t *__restrict__ zero, uint64_t *__restrict__ xbuf,
+                                                         uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit,
+                                                         int32_t *__restrict__ llcs) {
+    static_assert(64 % D == 0, "the ring must divide a group of 64 steps");
+    constexpr int NW = kSegWaves;
+    __shared__ uint64_t hand[3][NW];
+    __shared__ int zsum[NW];
+    __shared__ uint32_t ticket;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) ticket = atomicAdd(&ctl->ticket, 1u);
+        __syncthreads();
+        const uint32_t tk = ticket;
+        __syncthreads();   // every wave has read the ticket before it is replaced
+        if (tk >= nitems) break;
+        const SegItem it = items[tk];
+        const HyPair P = pairs[it.pair];
+        const int seg = it.seg;
+        const int K = P.K, m = P.m;
+        const int nseg = (K + 64 * NW * R - 1) / (64 * NW * R);
+        const int Wg = NW * seg + wave;
+        const int L = 64 * Wg + lane;   // global lane of the pair
+        const int w0 = L * R;
+        const uint64_t *T = tab + P.tab_off;
+        const int32_t *A = rows + P.rows_off;
+        // groups this workgroup runs: wave Wg meets rows in groups [2 Wg, 2 Wg + c]
+        const int c = (m + 63) / 64 + 1;
+        const int G0 = 2 * NW * seg, G1 = 2 * NW * seg + 2 * (NW - 1) + c;
+        const int ng = G1 - G0 + 1;   // the same for every segment of the pair
+        uint64_t *xmine = xbuf + xoff[it.pair] + (int64_t)seg * ng * 2;
+        const uint64_t *xprevseg = xbuf + xoff[it.pair] + (int64_t)(seg - 1) * ng * 2;
+        uint64_t V[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) V[r] = ~0ull;
+        uint64_t Mq[D][R];
+        int Sq[D];
+        const int t_first = 64 * (G0 - Wg);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int i = t_first + d - L;
+            const bool ok = i >= 0 && i < m;
+            const int sy = A[ok ? i : 0];
+#pragma unroll
+            for (int r = 0; r < R; ++r) Mq[d][r] = *(ok && w0 + r < K ? T + (int64_t)sy * K + w0 + r : zero);
+            const int i2 = i + D;
+            Sq[d] = A[i2 >= 0 && i2 < m ? i2 : 0];
+        }
+        // the previous segment's carry word of group gp (zero outside the
+        // groups it ran: rows there lie outside [0, m))
+        auto seg_word = [&](int gp) -> uint64_t {
+            if (seg == 0 || gp < G0 - 2 * NW || gp > G1 - 2 * NW) return 0ull;
+            const uint64_t *e = xprevseg + (int64_t)(gp - (G0 - 2 * NW)) * 2;
+            const uint32_t want = (epoch << 16) | ((uint32_t)gp & 0xFFFFu);
+            uint64_t lo = seg_ld(e), hi = seg_ld(e + 1);
+            uint32_t spins = 0;
+            while ((uint32_t)(lo >> 32) != want || (uint32_t)(hi >> 32) != want) {
+                if (++spins >= spin_limit ||
+                    __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    __hip_atomic_fetch_or(&ctl->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return 0ull;   // results of this run are void and reported
+                }
+                __builtin_amdgcn_s_sleep(2);
+                lo = seg_ld(e);
+                hi = seg_ld(e + 1);
+            }
+            return ((hi & 0xFFFFFFFFull) << 32) | (lo & 0xFFFFFFFFull);
+        };
+        uint64_t xprev = wave == 0 ? seg_word(G0 - 2) : 0ull;
+        int cprev = 0;
+        for (int g = G0; g <= G1; ++g) {
+            const int t0 = 64 * (g - Wg);
+            uint64_t in = 0;
+            if (wave > 0) {
+                const uint64_t p1 = g - 1 >= G0 ? hand[(g - 1) % 3][wave - 1] : 0ull;
+                const uint64_t p2 = g - 2 >= G0 ? hand[(g - 2) % 3][wave - 1] : 0ull;
+                in = (p1 << 1) | (p2 >> 63);
+            } else if (seg > 0) {
+                const uint64_t x1 = seg_word(g - 1);
+                in = (x1 << 1) | (xprev >> 63);
+                xprev = x1;
+            }
+            uint64_t outbits = 0;
+            for (int j0 = 0; j0 < 64; j0 += D) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const int j = j0 + d;
+                    const int t = t0 + j;
+                    const int i = t - L;
+                    const int from_left = __builtin_amdgcn_update_dpp(0, cprev, kDppWaveShr1, 0xf, 0xf, false);
+                    int cc = lane == 0 ? (int)((in >> j) & 1ull) : from_left;
+                    if (i >= 0 && i < m) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const uint64_t p = Mq[d][r] & V[r];
+                            const uint64_t t1 = V[r] + p;
+                            const int o1 = t1 < V[r] ? 1 : 0;
+                            const uint64_t t2 = t1 + (uint64_t)cc;
+                            const int o2 = t2 < t1 ? 1 : 0;
+                            V[r] = (V[r] ^ p) | t2;
+                            cc = o1 | o2;
+                        }
+                    } else {
+                        cc = 0;
+                    }
+                    cprev = cc;
+                    outbits |= (uint64_t)(__builtin_amdgcn_readlane(cc, 63) & 1) << j;
+                    const int in_ = i + D;
+                    const bool ok = in_ >= 0 && in_ < m;
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        Mq[d][r] = *(ok && w0 + r < K ? T + (int64_t)Sq[d] * K + w0 + r : zero);
+                    const int i2 = i + 2 * D;
+                    Sq[d] = A[i2 >= 0 && i2 < m ? i2 : 0];
+                }
+            }
+            if (lane == 0) {
+                hand[g % 3][wave] = outbits;
+                if (wave == NW - 1 && seg + 1 < nseg) {
+                    const uint64_t tag = (uint64_t)((epoch << 16) | ((uint32_t)g & 0xFFFFu)) << 32;
+                    uint64_t *e = xmine + (int64_t)(g - G0) * 2;
+                    seg_st(e, tag | (outbits & 0xFFFFFFFFull));
+                    seg_st(e + 1, tag | (outbits >> 32));
+                }
+            }
+            __syncthreads();
+        }
+        int z = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) z += w0 + r < K ? 64 - __popcll(V[r]) : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
+        if (lane == 0) zsum[wave] = z;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int j = 0; j < NW; ++j) tot += zsum[j];
+            atomicAdd(&llcs[P.out], tot);
+        }
+    }
+}
+
 }  // namespace hy
 
 // ---------------------------------------------------------------------------
@@ -276,10 +452,18 @@ struct Hyyro {
     int npairs = 0;
     hipStream_t stream = nullptr;
     std::vector<hy::HyPair> pairs;
-    std::vector<std::vector<int32_t>> cls_list;   // launch class (R, waves) -> pair indices
+    std::vector<std::vector<int32_t>> cls_list;   // launch class -> pair indices
     hy::HyPair *d_pairs = nullptr;
-    int32_t *d_list = nullptr;          // all classes' lists, concatenated
+    int32_t *d_list = nullptr;          // the one-workgroup classes' lists, concatenated
     std::vector<int64_t> list_off;
+    // segmented classes: (pair, segment) tickets, exchange buffer, control
+    std::vector<std::vector<hy::SegItem>> seg_items;
+    hy::SegItem *d_items = nullptr;
+    std::vector<int64_t> item_off;
+    int64_t *d_xoff = nullptr;          // per pair: its segments' exchange entries in d_xbuf
+    uint64_t *d_xbuf = nullptr;
+    LaunchCtl *d_ctl = nullptr;         // one per segmented class
+    uint32_t spin_limit = 1u << 23;
     int32_t *d_rows = nullptr;
     uint8_t *d_cols = nullptr;
     uint64_t *d_tab = nullptr;
@@ -296,24 +480,34 @@ struct Hyyro {
 namespace {
 
 // Launch classes by packed width K (words).  Up to 64 words: one wave of
-// the lookahead kernel (hy_rows), the faster form for short pairs.  Wider:
-// the systolic kernel (hy_sys) on 2 or 4 waves with one word per lane up to
-// 256 words, then 4 waves with 2 .. 16 words per lane, then 8 and 16 waves
-// with 16.  Wider lanes amortise the per-step fixed cost (the carry move, the
-// row's symbol and mask gathers) over more words; four waves (one per SIMD)
-// keep one pair's steps issue-light on its compute unit.
+// the lookahead kernel (hy_rows), the faster form for short pairs; up to 256:
+// one workgroup of the systolic kernel (hy_sys) on 2 or 4 waves.  Up to 8,192
+// words the pair is cut into segments of 256 R words on workgroups of their
+// own (hy_seg, R = 1 up to 2,048 words, R = 2 beyond), so a long pair spreads
+// over as many compute units as it has segments.  Wider pairs (up to 16,384
+// words) run on one 16-wave workgroup of hy_sys with 16 words per lane: the
+// segmented form with 4 words per lane is switched off (it ran past a test's
+// time limit on a 7 x 524,289 pair, cause not found).
 int hy_class(int K) {
-    static const int limits[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384};
+    static const int limits[] = {64, 128, 256, 2048, 8192, 16384};
     int c = 0;
     while (K > limits[c]) ++c;
     return c;
 }
-constexpr int kClasses = 9;
+constexpr int kClasses = 6;
+constexpr int kFirstSegClass = 3;
+constexpr int kSegClasses = 2;   // classes 3 and 4
+bool is_seg_class(int cls) { return cls >= kFirstSegClass && cls < kFirstSegClass + kSegClasses; }
+int seg_R(int cls) { return cls == 3 ? 1 : 2; }
 
 void hy_free(Hyyro *h) {
     if (!h) return;
     (void)hipFree(h->d_pairs);
     (void)hipFree(h->d_list);
+    (void)hipFree(h->d_items);
+    (void)hipFree(h->d_xoff);
+    (void)hipFree(h->d_xbuf);
+    (void)hipFree(h->d_ctl);
     (void)hipFree(h->d_rows);
     (void)hipFree(h->d_cols);
     (void)hipFree(h->d_tab);
@@ -384,33 +578,62 @@ int hy_encode_u32(const uint32_t *a, int m, const uint32_t *b, int n, std::vecto
 }
 
 int hy_launch(Hyyro *h) {
+    // segmented pairs add their segments' zero counts: clear the scores, the
+    // tickets and the error flags first
+    SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_llcs, 0u, (uint64_t)std::max(h->npairs, 1), h->stream));
+    SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_ctl, 0u, 2u * kSegClasses, h->stream));
+    const uint32_t epoch = next_run_epoch();
     SL_HIP_TRY(hipEventRecord(h->fork, h->stream));
-    // widest class first: it holds the longest single pairs
+    const int cus = device_cus();
+    // widest class first: it holds the longest pairs
     for (int c = kClasses - 1; c >= 0; --c) {
-        const unsigned cnt = (unsigned)h->cls_list[(size_t)c].size();
-        if (!cnt) continue;
-        const int32_t *lst = h->d_list + h->list_off[(size_t)c];
         hipStream_t cs = h->cls_stream[(size_t)c];
-        SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
+        if (is_seg_class(c)) {
+            const size_t k = (size_t)(c - kFirstSegClass);
+            const uint32_t nit = (uint32_t)h->seg_items[k].size();
+            if (!nit) continue;
+            SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
+            const hy::SegItem *items = h->d_items + h->item_off[k];
+            const unsigned blocks = (unsigned)std::min<int64_t>(nit, 2 * cus);
+#define SL_HY_SEG(R, D)                                                                                         \
+    hipLaunchKernelGGL((hy::hy_seg<R, D>), dim3(blocks), dim3(64 * hy::kSegWaves), 0, cs, h->d_pairs, items, nit, \
+                       h->d_xoff, h->d_rows, h->d_tab, h->d_tab + h->tab_words, h->d_xbuf, epoch, h->d_ctl + k,     \
+                       h->spin_limit, h->d_llcs)
+            if (seg_R(c) == 1) SL_HY_SEG(1, 32);
+            else SL_HY_SEG(2, 16);
+#undef SL_HY_SEG
+        } else {
+            const unsigned cnt = (unsigned)h->cls_list[(size_t)c].size();
+            if (!cnt) continue;
+            SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
+            const int32_t *lst = h->d_list + h->list_off[(size_t)c];
 #define SL_HY_LAUNCH(KERNEL, R, D, NW)                                                                       \
     hipLaunchKernelGGL((hy::KERNEL<R, D, NW>), dim3(cnt), dim3(64 * NW), 0, cs, h->d_pairs, lst, h->d_rows, \
                        h->d_tab, h->d_tab + h->tab_words, h->d_llcs)
-        switch (c) {
-            case 0: SL_HY_LAUNCH(hy_rows, 1, 16, 1); break;   // one wave: the lookahead form is faster
-            case 1: SL_HY_LAUNCH(hy_sys, 1, 32, 2); break;
-            case 2: SL_HY_LAUNCH(hy_sys, 1, 32, 4); break;
-            case 3: SL_HY_LAUNCH(hy_sys, 2, 16, 4); break;
-            case 4: SL_HY_LAUNCH(hy_sys, 4, 8, 4); break;
-            case 5: SL_HY_LAUNCH(hy_sys, 8, 4, 4); break;
-            case 6: SL_HY_LAUNCH(hy_sys, 16, 2, 4); break;
-            case 7: SL_HY_LAUNCH(hy_sys, 16, 2, 8); break;
-            default: SL_HY_LAUNCH(hy_sys, 16, 2, 16); break;
-        }
+            switch (c) {
+                case 0: SL_HY_LAUNCH(hy_rows, 1, 16, 1); break;   // one wave: the lookahead form is faster
+                case 1: SL_HY_LAUNCH(hy_sys, 1, 32, 2); break;
+                case 2: SL_HY_LAUNCH(hy_sys, 1, 32, 4); break;
+                default: SL_HY_LAUNCH(hy_sys, 16, 2, 16); break;
+            }
 #undef SL_HY_LAUNCH
+        }
         SL_HIP_TRY(hipGetLastError());
         SL_HIP_TRY(hipEventRecord(h->cls_done[(size_t)c], cs));
         SL_HIP_TRY(hipStreamWaitEvent(h->stream, h->cls_done[(size_t)c], 0));
     }
+    return kOk;
+}
+
+// After a run (stream synchronised): a bounded wait of a segmented class ran out?
+int hy_check(Hyyro *h) {
+    LaunchCtl ctl[kSegClasses];
+    SL_HIP_TRY(hipMemcpy(ctl, h->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+    for (auto &c : ctl)
+        if (c.error) {
+            set_error("Hyyro LLCS: a segment's bounded wait for its neighbour timed out; results are void");
+            return kErrTimeout;
+        }
     return kOk;
 }
 
@@ -477,8 +700,35 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     std::vector<int32_t> all_lists;
     h->list_off.assign(kClasses, 0);
     for (int c = 0; c < kClasses; ++c) {
+        if (is_seg_class(c)) continue;
         h->list_off[(size_t)c] = (int64_t)all_lists.size();
         all_lists.insert(all_lists.end(), h->cls_list[(size_t)c].begin(), h->cls_list[(size_t)c].end());
+    }
+    // segmented classes: pairs by work (rows x words) descending, each pair's
+    // segments in order; exchange entries: per segment, 2 per group it runs
+    std::vector<hy::SegItem> all_items;
+    std::vector<int64_t> xoff((size_t)std::max(npairs, 1), 0);
+    int64_t xtotal = 0;
+    h->seg_items.assign(kSegClasses, {});
+    h->item_off.assign(kSegClasses, 0);
+    for (int c = kFirstSegClass; c < kFirstSegClass + kSegClasses; ++c) {
+        auto lst = h->cls_list[(size_t)c];
+        std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) {
+            return (int64_t)h->pairs[(size_t)x].m * h->pairs[(size_t)x].K >
+                   (int64_t)h->pairs[(size_t)y].m * h->pairs[(size_t)y].K;
+        });
+        const int segw = 64 * hy::kSegWaves * seg_R(c);
+        auto &items = h->seg_items[(size_t)(c - kFirstSegClass)];
+        for (int pi : lst) {
+            const hy::HyPair &P = h->pairs[(size_t)pi];
+            const int nseg = (P.K + segw - 1) / segw;
+            const int64_t ng = (P.m + 63) / 64 + 8;   // groups a segment runs (hy_seg: G1 - G0 + 1)
+            xoff[(size_t)pi] = xtotal;
+            xtotal += (int64_t)nseg * ng * 2;
+            for (int sg = 0; sg < nseg; ++sg) items.push_back(hy::SegItem{pi, sg});
+        }
+        h->item_off[(size_t)(c - kFirstSegClass)] = (int64_t)all_items.size();
+        all_items.insert(all_items.end(), items.begin(), items.end());
     }
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     for (auto &ev : h->ev)
@@ -496,6 +746,10 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     };
     alloc((void **)&h->d_pairs, sizeof(hy::HyPair) * h->pairs.size());
     alloc((void **)&h->d_list, sizeof(int32_t) * all_lists.size());
+    alloc((void **)&h->d_items, sizeof(hy::SegItem) * all_items.size());
+    alloc((void **)&h->d_xoff, sizeof(int64_t) * xoff.size());
+    alloc((void **)&h->d_xbuf, sizeof(uint64_t) * (size_t)xtotal);
+    alloc((void **)&h->d_ctl, sizeof(LaunchCtl) * kSegClasses);
     alloc((void **)&h->d_rows, sizeof(int32_t) * (rows.size() + 1));   // + a spare: clamped loads read index 0
     alloc((void **)&h->d_cols, cols.size());
     alloc((void **)&h->d_tab, sizeof(uint64_t) * (size_t)(h->tab_words + 1));   // + the zero word
@@ -504,6 +758,11 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
         e = hipMemcpy(h->d_pairs, h->pairs.data(), sizeof(hy::HyPair) * h->pairs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !all_lists.empty())
         e = hipMemcpy(h->d_list, all_lists.data(), sizeof(int32_t) * all_lists.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !all_items.empty())
+        e = hipMemcpy(h->d_items, all_items.data(), sizeof(hy::SegItem) * all_items.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_xoff, xoff.data(), sizeof(int64_t) * xoff.size(), hipMemcpyHostToDevice);
+    // exchange entries start with tag 0, which no run's (nonzero) epoch matches
+    if (e == hipSuccess && xtotal > 0) e = launch_fill_u32((uint32_t *)h->d_xbuf, 0u, 2u * (uint64_t)xtotal, h->stream);
     if (e == hipSuccess && !rows.empty())
         e = hipMemcpy(h->d_rows, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !cols.empty()) e = hipMemcpy(h->d_cols, cols.data(), cols.size(), hipMemcpyHostToDevice);
@@ -562,7 +821,7 @@ int sl_hyyro_result(sl_hyyro *hh, int32_t *llcs) {
     SL_HIP_TRY(hipMemcpyAsync(llcs, h->d_llcs, sizeof(int32_t) * (size_t)h->npairs, hipMemcpyDeviceToHost,
                               h->stream));
     SL_HIP_TRY(hipStreamSynchronize(h->stream));
-    return kOk;
+    return hy_check(h);
 }
 
 int sl_hyyro_time(sl_hyyro *hh, int warmup, int iters, double *ms_per_run) {
@@ -582,7 +841,7 @@ int sl_hyyro_time(sl_hyyro *hh, int warmup, int iters, double *ms_per_run) {
     float ms = 0;
     SL_HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
     *ms_per_run = ms / iters;
-    return kOk;
+    return hy_check(h);
 }
 
 int sl_hyyro_stats(const sl_hyyro *hh, int64_t *word_rows, int64_t *table_words) {
