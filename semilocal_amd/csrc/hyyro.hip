@@ -463,7 +463,9 @@ struct Hyyro {
     int64_t *d_xoff = nullptr;          // per pair: its segments' exchange entries in d_xbuf
     uint64_t *d_xbuf = nullptr;
     LaunchCtl *d_ctl = nullptr;         // one per segmented class
-    uint32_t spin_limit = 1u << 23;
+    // a segment waits a few groups (well under a millisecond) for its
+    // neighbour; 2^20 polls is over a second of margin and fails fast
+    uint32_t spin_limit = 1u << 20;
     int32_t *d_rows = nullptr;
     uint8_t *d_cols = nullptr;
     uint64_t *d_tab = nullptr;
