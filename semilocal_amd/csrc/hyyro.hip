@@ -483,15 +483,15 @@ namespace {
 
 // Launch classes by packed width K (words).  Up to 64 words: one wave of
 // the lookahead kernel (hy_rows), the faster form for short pairs; up to 256:
-// one workgroup of the systolic kernel (hy_sys) on 2 or 4 waves.  Up to 8,192
-// words the pair is cut into segments of 256 R words on workgroups of their
-// own (hy_seg, R = 1 up to 2,048 words, R = 2 beyond), so a long pair spreads
-// over as many compute units as it has segments.  Wider pairs (up to 16,384
-// words) run on one 16-wave workgroup of hy_sys with 16 words per lane: the
-// segmented form with 4 words per lane is switched off (it ran past a test's
-// time limit on a 7 x 524,289 pair, cause not found).
+// one workgroup of the systolic kernel (hy_sys) on 2 or 4 waves.  Wider pairs
+// (up to 16,384 words) are cut into segments of 256 R words on workgroups of
+// their own (hy_seg, R = 1 up to 2,048 words, R = 2 beyond), so a long pair
+// spreads over as many compute units as it has segments.  (The segmented form
+// with 4 words per lane is not used: it ran past a test's time limit on a
+// 7 x 524,289 pair, cause not found.)  Class 5 (one 16-wave workgroup of
+// hy_sys with 16 words per lane) holds no pairs with these limits.
 int hy_class(int K) {
-    static const int limits[] = {64, 128, 256, 2048, 8192, 16384};
+    static const int limits[] = {64, 128, 256, 2048, 16384, 16384};
     int c = 0;
     while (K > limits[c]) ++c;
     return c;
