@@ -35,6 +35,7 @@ def test_combing_drivers_print_the_oracle_hash(tmp_path, oracle, driver, threads
 
 
 @pytest.mark.parametrize("driver,m,n,sigma", [("prefix_lcs_hip", 3000, 2500, 4), ("prefix_lcs_hip", 500, 64, 20),
+                                              ("prefix_lcs_hip", 400, 900, 300),     # > 256 symbols: combing path
                                               ("prefix_lcs_hip_bit_parallel", 1024, 2048, 2),
                                               ("prefix_lcs_hip_bit_parallel", 4096, 640, 2),
                                               ("prefix_lcs_hip_bit_parallel", 1000, 777, 2)])
@@ -46,7 +47,7 @@ def test_lcs_drivers_print_the_dp_score(tmp_path, oracle, driver, m, n, sigma):
     print(f"driver {driver} m {m} n {n}", flush=True)
     r = harness.run_combing(driver, 2, fa, fb, timeout=120)
     assert (r.a_size, r.b_size) == (m, n)
-    assert r.value == oracle.llcs_dp(a.astype(np.uint8), b.astype(np.uint8))
+    assert r.value == oracle.llcs_dp(a, b)
 
 
 @pytest.mark.parametrize("n,seed", [(1, 3), (5000, 42), (1 << 16, 7)])
