@@ -40,7 +40,6 @@ struct DeviceState {
 };
 static DeviceState g_dev;
 static std::mutex g_dev_mu;
-std::mutex g_oneshot_mu;
 
 int ensure_device(int device) {
     std::lock_guard<std::mutex> lk(g_dev_mu);
