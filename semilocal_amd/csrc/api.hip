@@ -906,36 +906,46 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
         set_error("sl_batch_autotune: bad argument");
         return kErrArg;
     }
+    struct Shape {
+        int sched, r, w;
+        double ms;
+    };
+    auto time_shape = [&](Shape &sh, int warmup, int runs) -> int {
+        bt->schedule = sh.sched;
+        int rc = sl_batch_set_tiles(h, sh.r, sh.w);
+        if (!rc) rc = sl_batch_time(h, 0, warmup, runs, &sh.ms, nullptr);
+        if (rc)
+            set_error("sl_batch_autotune: rows_per_lane " + std::to_string(sh.r) + ", waves_per_cu " +
+                      std::to_string(sh.w) + ", order " + std::to_string(sh.sched) + ": " + last_error());
+        return rc;
+    };
+    // every shape `iters` cold runs (each run is a whole launch), then the
+    // four fastest again after a warm-up run with more runs, as a caller's
+    // steady state runs them: shapes a few percent apart trade places under
+    // the first pass's noise and cold launches
     const int wpc_cand[] = {4, 8, 12, 16, 20, 24, 32};
-    double best = 1e30;
-    int best_r = bt->rows_per_lane, best_w = (int)bt->waves_per_cu, best_s = bt->schedule;
-    for (int sched = 0; sched <= kMaxSchedule; ++sched) {
-        bt->schedule = sched;
-        for (int r : kTileHeights) {
+    std::vector<Shape> shapes;
+    for (int sched = 0; sched <= kMaxSchedule; ++sched)
+        for (int r : kTileHeights)
             for (int w : wpc_cand) {
-                int rc = sl_batch_set_tiles(h, r, w);
+                Shape sh{sched, r, w, 0.0};
+                const int rc = time_shape(sh, 0, iters);
                 if (rc) return rc;
-                double ms = 0.0;
-                rc = sl_batch_time(h, 0, 0, iters, &ms, nullptr);   // each run is a whole launch: no warm-up
-                if (rc) {
-                    set_error("sl_batch_autotune: rows_per_lane " + std::to_string(r) + ", waves_per_cu " +
-                              std::to_string(w) + ", order " + std::to_string(sched) + ": " + last_error());
-                    return rc;
-                }
-                if (ms < best) {
-                    best = ms;
-                    best_r = r;
-                    best_w = w;
-                    best_s = sched;
-                }
+                shapes.push_back(sh);
             }
-        }
+    std::sort(shapes.begin(), shapes.end(), [](const Shape &x, const Shape &y) { return x.ms < y.ms; });
+    const size_t finalists = std::min<size_t>(4, shapes.size());
+    for (size_t i = 0; i < finalists; ++i) {
+        const int rc = time_shape(shapes[i], 1, std::max(10, 2 * iters + 1));
+        if (rc) return rc;
     }
-    bt->schedule = best_s;
-    int rc = sl_batch_set_tiles(h, best_r, best_w);
+    const Shape best = *std::min_element(shapes.begin(), shapes.begin() + (std::ptrdiff_t)finalists,
+                                         [](const Shape &x, const Shape &y) { return x.ms < y.ms; });
+    bt->schedule = best.sched;
+    int rc = sl_batch_set_tiles(h, best.r, best.w);
     if (rc) return rc;
-    if (rows_per_lane) *rows_per_lane = best_r;
-    if (waves_per_cu) *waves_per_cu = best_w;
+    if (rows_per_lane) *rows_per_lane = best.r;
+    if (waves_per_cu) *waves_per_cu = best.w;
     return kOk;
 }
 
