@@ -183,14 +183,21 @@ class HipStickyBraid : public common::BraidMultiplicationStrategy {
     }
 };
 
-// LLCS through the device kernel: an lcs::LCSStrategy<Input> (LCS.h:9-13),
-// usable where the reference takes lcs::LCSBySemiLocal over a combing
-// strategy or a bit_parallel::lcs::BitSemiLocalSticky (BitParalellLCS.h:
-// 868-1401); any alphabet.
+// LLCS on the device: an lcs::LCSStrategy<Input> (LCS.h:9-13), usable where
+// the reference takes lcs::LCSBySemiLocal over a combing strategy or a
+// bit_parallel::lcs::BitSemiLocalSticky (BitParalellLCS.h:868-1401); any
+// alphabet.  Scored by Hyyro's bit-vector recurrence (sl_llcs_hyyro) when the
+// longer string has at most 256 distinct symbols and 2^20 symbols, otherwise
+// read off the semi-local kernel (m - #{s < m : P[s] >= n}); the same value.
 template <class Input>
 class HipLLCS : public lcs::LCSStrategy<Input> {
   public:
     int compute(const Input *a, int sizeA, const Input *b, int sizeB) override {
+        detail::Symbols<Input> sa(a, sizeA), sb(b, sizeB);
+        int llcs = 0;
+        const int st = sl_llcs_hyyro(sa.ptr, sizeA, sb.ptr, sizeB, sa.bytes, &llcs);
+        if (st == SL_OK) return llcs;
+        if (st != SL_ERR_UNSUPPORTED) check(st);
         const auto rc = detail::comb(a, sizeA, b, sizeB);
         int acc = 0;
         for (int s = 0; s < sizeA; ++s) acc += rc[(size_t)s] >= sizeB;
