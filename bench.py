@@ -417,6 +417,19 @@ def bench_plain(S, W, golden):
             "parity": "bit-exact" if W.sha256(res) == golden["plain_pair_10k"]["perm_sha256"] else "MISMATCH"}
 
 
+
+def copy_probe(mb):
+    """One plain device copy of `mb` MiB each way: the scale for the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE byte counters, profiled in the same
+    process as the kernel they calibrate."""
+    from semilocal_amd._lib import probe_copy
+
+    nbytes = mb << 20
+    pms_copy = probe_copy(nbytes)
+    log(f"copy probe: {nbytes} bytes read, {nbytes} bytes written in {pms_copy:.3f} ms "
+        f"({2 * nbytes / pms_copy / 1e6:.0f} GB/s)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -462,6 +475,8 @@ def main():
               "serving": lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
                                                args.serve_pool_gb)}[args.only]
         print(json.dumps({"section": args.only, **fn()}), flush=True)
+        if args.copy_probe_mb:
+            copy_probe(args.copy_probe_mb)
         return
     t0 = time.perf_counter()
     pairs = W.ragged_batch()
@@ -510,13 +525,7 @@ def main():
     else:
         pms = float("nan")
     if args.copy_probe_mb:
-        # a plain device copy of known size: the scale for the PMC byte counters
-        from semilocal_amd._lib import probe_copy
-
-        nbytes = args.copy_probe_mb << 20
-        pms_copy = probe_copy(nbytes)
-        log(f"copy probe: {nbytes} bytes read, {nbytes} bytes written in {pms_copy:.3f} ms "
-            f"({2 * nbytes / pms_copy / 1e6:.0f} GB/s)")
+        copy_probe(args.copy_probe_mb)
 
     gcups = cells / (ms * 1e-3) / 1e9
     gcups_pp = cells / (pms * 1e-3) / 1e9
