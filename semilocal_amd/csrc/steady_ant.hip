@@ -28,8 +28,10 @@
 // kSegSteps evaluations.  Where it crosses anti-diagonal t (t = steps taken)
 // is the first c on that diagonal with delta(r(c)+1, c) >= 0, r(c) = s-1-t+c,
 // found by binary search (the predicate is monotone along a diagonal) from a
-// per-subproblem table of block counts plus two short scans.  Each segment is
-// then walked by one lane from LDS.  The product is U with the good points
+// per-subproblem table of block counts plus two short scans, down to a range
+// of about a thousand candidates, which one pass of prefix sums over the
+// diagonal's own rows and columns settles.  Each segment is then walked by
+// one lane from LDS.  The product is U with the good points
 // written over it, which is how the reference merges r_hi into r_lo and adds
 // the good points.
 //
@@ -437,6 +439,50 @@ __device__ __forceinline__ int wave_delta_lds(const uint16_t *U16, int s, int r,
     return __builtin_amdgcn_readfirstlane(wave_sum(acc));
 }
 
+// Crossing of anti-diagonal t0 by the walk of an LDS subproblem of size s:
+// the first c in [a, b] = [max(0, t0-s+1), min(t0, s)] with
+// D(c) = delta(c + s - t0, c) >= 0, and delta(r, c) at r = c + s - t0 - 1.
+// D(c+1) - D(c) reads only row R(c)'s and column c's points (the identity
+// above wave_window_search), and b - a <= s <= kTile, so one pass over
+// positions c = a + 64 e + lane (e < 16) finds it: per row of 64 positions
+// the terms (0, 1 or 2) go into two ballots, a lane's prefix is two masked
+// bit counts, and the uniform running total carries D across rows.  D is
+// monotone, so the pass stops at the first row holding a non-negative D.
+__device__ __forceinline__ int2 wave_cross_lds(const uint16_t *U16, const uint16_t *Ui16, int s, int t0) {
+    const int lane = threadIdx.x & 63;
+    const int off = s - t0;
+    const int a = max(0, t0 - s + 1), b = min(t0, s);
+    int carry = wave_delta_lds(U16, s, a + off, a);   // D(a), then D at the current row's first position
+    int c_ans = a;
+    for (int e = 0; e < kTile / 64; ++e) {
+        const int c0 = a + 64 * e;
+        if (c0 >= b) break;   // uniform
+        const int c = c0 + lane, R = c + off;
+        const bool valid = c < b;
+        const uint32_t uu = U16[valid ? R : 0], vv = Ui16[valid ? c : 0];
+        const int col = (int)(uu & 0x7FFFu), row = (int)(vv & 0x7FFFu);
+        const int tu = (uu & kHi16) ? (col <= c) : (col >= c);
+        const int tv = (vv & kHi16) ? (row < R) : (row > R);
+        const int d = valid ? tu + tv : 0;
+        const uint64_t m1 = __ballot(d & 1), m2 = __ballot(d & 2);
+        const int pre = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u)) +
+                        2 * (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m2 >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m2, 0u));
+        const uint64_t neg = __ballot(valid && carry + pre < 0);   // a prefix of the row's lanes
+        c_ans += __popcll(neg);
+        if (neg != __ballot(valid)) {   // the crossing is in this row
+            carry += __popcll(m1 & neg) + 2 * __popcll(m2 & neg);
+            break;
+        }
+        carry += __popcll(m1) + 2 * __popcll(m2);
+    }
+    // carry = D(c_ans); delta one row above subtracts row r's term
+    const int r = c_ans + off - 1;
+    const uint32_t ur = U16[r];
+    const int dr = carry - (((int)(ur & 0x7FFFu) >= c_ans) ^ (int)((ur & kHi16) != 0));
+    return make_int2(c_ans, dr);
+}
+
 __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restrict__ P0, const int32_t *__restrict__ Qi0,
                                                      const int32_t *__restrict__ tile_prob,
                                                      const SaProb *__restrict__ probs, int S0,
@@ -521,10 +567,10 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
                 }
             }
         } else {
-            // segments of kSmallSeg steps.  Their start points are found by
-            // binary search on anti-diagonals, one wave per segment (delta from
-            // an LDS scan); then every segment is walked by its own lane, so
-            // the walks run side by side on the vector ALU.
+            // segments of kSmallSeg steps.  Their start points are found on
+            // anti-diagonals, one wave per segment (wave_cross_lds); then
+            // every segment is walked by its own lane, so the walks run side
+            // by side on the vector ALU.
             const int nsps = max(1, (2 * S) / kSmallSeg);
             const int nsub = (len + S - 1) / S;
             const int nseg = nsub * nsps;   // <= kSmallMaxSeg
@@ -535,15 +581,8 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
                 const int t0 = k * kSmallSeg;
                 int4 st = make_int4(-1, 0, 0, 0);
                 if (s > h && t0 <= t_end) {   // uniform over the wave
-                    const uint16_t *U16 = sU + s0;
-                    int a = max(0, t0 - s + 1), b = min(t0, s);
-                    while (a < b) {
-                        const int mid = (a + b) >> 1;
-                        if (wave_delta_lds(U16, s, s - t0 + mid, mid) >= 0) b = mid;   // delta(r(mid)+1, mid)
-                        else a = mid + 1;
-                    }
-                    const int c = a, r = s - 1 - t0 + c;
-                    st = make_int4(r, c, wave_delta_lds(U16, s, r, c),
+                    const int2 cd = wave_cross_lds(sU + s0, sUi + s0, s, t0);
+                    st = make_int4(s - 1 - t0 + cd.x, cd.x, cd.y,
                                    nsps == 1 ? t_end + 1 : min(kSmallSeg, t_end - t0 + 1));
                 }
                 if (lane == 0) sStart[w] = st;
@@ -737,9 +776,104 @@ __device__ int wave_delta(const uint32_t *__restrict__ U, const uint32_t *__rest
     return __builtin_amdgcn_readfirstlane(base + wave_sum(acc));
 }
 
+// Along an anti-diagonal, with R(c) = c + off, D(c) = delta(R(c), c) moves by
+//     D(c+1) - D(c) = [row R(c)'s point: hi with col <= c, or lo with col >= c]
+//                   + [column c's point: hi with row < R(c), or lo with row > R(c)]
+// (the cells that enter {row < R, col < c} and leave {row >= R, col >= c}),
+// so D over a short range is D(a) plus a prefix sum of terms that read only
+// U[R(c)] and Ui[c], both consecutive in c.  kWinSpan positions per pass: 16
+// per lane, lane L taking [w0 + 16 L, w0 + 16 L + 16) with w0 = a rounded down
+// to a multiple of 4.
+constexpr int kWinSpan = 1020;   // b - a bound: [a, b) lies inside [w0, w0 + 1024)
+
+// Lane terms of one window.  Ui is read as aligned 16-byte groups from w0;
+// U from R(w0) rounded down to a multiple of 4 (M = R(w0) mod 4, so entry
+// e + M is row R(w0 + 16 L + e)), one group more when M > 0.  A group is read
+// only when it holds a position below b, and for U when it starts at row >= 0
+// (rows below 0 belong to positions below a).  Such groups end at most 3
+// entries past row or column s - 1, inside the arrays' slack.  Positions
+// outside [a, b) get term 0.
+template <int M>
+__device__ __forceinline__ int window_terms(const uint32_t *__restrict__ Ux, const uint32_t *__restrict__ Uix,
+                                            int w0, int off, int a, int b, int (&dl)[16]) {
+    constexpr int GU = M > 0 ? 5 : 4;
+    const int lane = threadIdx.x & 63;
+    const int p0 = w0 + lane * 16;
+    const int rb = w0 + off - M + lane * 16;   // first U row read by this lane (a multiple of 4)
+    uint4 u[GU], v[4];
+#pragma unroll
+    for (int g = 0; g < GU; ++g) {
+        const int gr = rb + 4 * g;
+        u[g] = (gr >= 0 && gr - off < b) ? *reinterpret_cast<const uint4 *>(Ux + gr) : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int gc = p0 + 4 * g;
+        v[g] = gc < b ? *reinterpret_cast<const uint4 *>(Uix + gc) : make_uint4(0u, 0u, 0u, 0u);
+    }
+    uint32_t ue[4 * GU], ve[16];
+#pragma unroll
+    for (int g = 0; g < GU; ++g) {
+        ue[4 * g] = u[g].x;
+        ue[4 * g + 1] = u[g].y;
+        ue[4 * g + 2] = u[g].z;
+        ue[4 * g + 3] = u[g].w;
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        ve[4 * g] = v[g].x;
+        ve[4 * g + 1] = v[g].y;
+        ve[4 * g + 2] = v[g].z;
+        ve[4 * g + 3] = v[g].w;
+    }
+    int sum = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int c = p0 + e, R = c + off;
+        const uint32_t uu = ue[e + M], vv = ve[e];
+        const int col = (int)(uu & kValMask), row = (int)(vv & kValMask);
+        const int tu = (uu >> 31) ? (col <= c) : (col >= c);
+        const int tv = (vv >> 31) ? (row < R) : (row > R);
+        dl[e] = (c >= a && c < b) ? tu + tv : 0;
+        sum += dl[e];
+    }
+    return sum;
+}
+
+// First c in [a, b] with D(c) >= 0 (b when none below b), given D(a) = Da and
+// b - a <= kWinSpan; returns (c, D(c)).  Uniform across the wave.
+__device__ int2 wave_window_search(const uint32_t *__restrict__ Ux, const uint32_t *__restrict__ Uix, int off,
+                                   int a, int b, int Da) {
+    const int w0 = a & ~3;
+    int dl[16];
+    int sum;
+    switch ((w0 + off) & 3) {
+    case 0: sum = window_terms<0>(Ux, Uix, w0, off, a, b, dl); break;
+    case 1: sum = window_terms<1>(Ux, Uix, w0, off, a, b, dl); break;
+    case 2: sum = window_terms<2>(Ux, Uix, w0, off, a, b, dl); break;
+    default: sum = window_terms<3>(Ux, Uix, w0, off, a, b, dl); break;
+    }
+    const int p0 = w0 + (int)(threadIdx.x & 63) * 16;
+    int run = Da + wave_incl_scan(sum) - sum;   // D(p0) for p0 >= a
+    int neg = 0, dneg = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int c = p0 + e;
+        const int n = (c >= a && c < b && run < 0) ? 1 : 0;   // D(c) < 0 exactly on [a, answer)
+        neg += n;
+        dneg += n * dl[e];
+        run += dl[e];
+    }
+    const int tot = __builtin_amdgcn_readfirstlane(wave_sum((dneg << 16) | neg));   // neg <= 1024, dneg <= 2048
+    return make_int2(a + (tot & 0xFFFF), Da + (tot >> 16));
+}
+
 // Start point of each walk segment: one wave per entry (subproblem, k); walk
 // segment k covers steps [k kSegSteps, (k+1) kSegSteps).  Output
 // (r, c, delta(r, c), t0), or r = -1 when the walk ends before step t0.
+// Binary search on D(c) = delta(c + s - t0, c) until at most kWinSpan
+// candidates remain, then one window pass; delta(r, c) one row above is
+// D(c) less row r's term.
 __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
                            const int32_t *__restrict__ cntHi, const int32_t *__restrict__ cntLo,
                            const SaSeg *__restrict__ segs, const int2 *__restrict__ walks, int nwalks,
@@ -753,15 +887,19 @@ __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__res
     const int t0 = job.y * kSegSteps;
     int4 out = make_int4(-1, -1, 0, 0);
     if (t0 <= t_end) {
+        const int off = s - t0;
         int a = max(0, t0 - s + 1), b = min(t0, s);
-        while (a < b) {
+        while (b - a > kWinSpan) {
             const int mid = (a + b) >> 1;
-            const int rr = s - 1 - t0 + mid;
-            if (wave_delta(U, Ui, cntHi, cntLo, sg, rr + 1, mid) >= 0) b = mid;
+            if (wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0) b = mid;
             else a = mid + 1;
         }
-        const int c = a, r = s - 1 - t0 + c;
-        out = make_int4(r, c, wave_delta(U, Ui, cntHi, cntLo, sg, r, c), t0);
+        const int Da = wave_delta(U, Ui, cntHi, cntLo, sg, a + off, a);
+        const int2 cd = wave_window_search(U + sg.x0, Ui + sg.x0, off, a, b, Da);
+        const int c = cd.x, r = c + off - 1;
+        const uint32_t ur = U[sg.x0 + r];
+        const int dr = cd.y - (((int)(ur & kValMask) >= c) ^ (int)(ur >> 31));
+        out = make_int4(r, c, dr, t0);
     }
     if ((threadIdx.x & 63) == 0) seg_start[w] = out;
 }
