@@ -277,6 +277,14 @@ __device__ __forceinline__ void ant_walk(int r, int c, int d, int s, int steps, 
 // columns c .. c+K-1, and reach at most 19 entries past the subproblem's end
 // (its tile padding, the next subproblem or the arrays' slack; such entries
 // are never used).  A step indexes the window by its row and column.
+//
+// Two changes keep a step's dependent chain short.  A walk that moves right
+// next needs column c+1, one that moves up row r-2; every other value is the
+// same as the step before.  So both candidates are read from LDS one step
+// ahead, while the current step is computed, and the move picks one: no LDS
+// read waits on the step's direction.  And every lane takes K steps per
+// window, with a lane that has stopped (or used its steps) frozen by a mask,
+// so the step loop has a uniform trip count and no per-lane exit.
 template <class PutF>
 __device__ __forceinline__ void ant_walk_window4(int r, int c, int d, int s, int steps, const uint32_t *U,
                                                  const uint32_t *Ui, PutF put, uint32_t *lw) {
@@ -284,7 +292,8 @@ __device__ __forceinline__ void ant_walk_window4(int r, int c, int d, int s, int
     uint32_t ur = U[r];
     uint32_t left = c > 0 ? Ui[c - 1] : 0u;
     int t = 0;
-    while (t < steps) {
+    bool alive = steps > 0;
+    while (__ballot(alive) != 0) {   // uniform
         const int ar = r - K > 0 ? (r - K) & ~3 : 0;
         const int ac = c & ~3;
         uint4 wr[G], wc[G];
@@ -304,19 +313,40 @@ __device__ __forceinline__ void ant_walk_window4(int r, int c, int d, int s, int
             lw[(4 * G + 4 * g + 2) * 64] = wc[g].z;
             lw[(4 * G + 4 * g + 3) * 64] = wc[g].w;
         }
-        const int t_stop = t + K < steps ? t + K : steps;
-        bool done = false;
-        for (; t < t_stop; ++t) {
-            const int ir = r - 1 - ar, ic = c - ac;          // in [0, 4G) for up to K steps
-            const uint32_t up = lw[(ir >= 0 ? ir : 0) * 64];   // row r - 1 (unused at r = 0)
-            const uint32_t nxt = lw[(4 * G + ic) * 64];       // column c
-            bool right;
-            if (!ant_step(r, c, d, s, ur, left, up, nxt, right, put)) {
-                done = true;
-                break;
-            }
+        // slots: row r-1 at ir (read as slot 0 when r = 0, where it is never
+        // used), column c at 4G + ic; within K steps ic + 1 < 4G
+        int ir = r - 1 - ar, ic = c - ac;
+        uint32_t up = lw[(ir > 0 ? ir : 0) * 64];
+        uint32_t nxt = lw[(4 * G + ic) * 64];
+#pragma unroll 2
+        for (int k = 0; k < K; ++k) {
+            const uint32_t up2 = lw[(ir > 1 ? ir - 1 : 0) * 64];   // row r-2
+            const uint32_t nxt2 = lw[(4 * G + ic + 1) * 64];      // column c+1
+            const int wcol = ((int)(left & kValMask) >= r) ^ (int)(left >> 31);
+            const int wrow = ((int)(ur & kValMask) >= c) ^ (int)(ur >> 31);
+            if (alive && c > 0 && d - wcol < 0 && d + wrow > 0) put(r, c - 1);
+            const int rt = (int)((uint32_t)d >> 31);   // 1: move right (delta < 0), 0: up
+            const int stop = (rt & (c >= s)) | ((rt ^ 1) & (r == 0));
+            alive = alive && !stop;
+            const int live = -(int)alive;                  // all ones when this lane moves
+            const uint32_t rm = 0u - (uint32_t)rt;
+            const uint32_t v = (nxt & rm) | (up & ~rm);    // the point the move passes
+            const int thr = c + ((r - c) & (int)rm);
+            const int dd = ((int)(v & kValMask) >= thr) ^ (int)(v >> 31);
+            const int up1 = rt ^ 1;
+            d += ((dd ^ -up1) + up1) & live;
+            const uint32_t mr = rm & (uint32_t)live, mu = ~rm & (uint32_t)live;   // moved right / up
+            left = (v & mr) | (left & ~mr);
+            ur = (v & mu) | (ur & ~mu);
+            nxt = (nxt2 & mr) | (nxt & ~mr);
+            up = (up2 & mu) | (up & ~mu);
+            r -= up1 & live;
+            c += rt & live;
+            ir -= up1 & live;
+            ic += rt & live;
+            ++t;
+            alive = alive && t < steps;
         }
-        if (done) break;
     }
 }
 
@@ -415,7 +445,7 @@ __device__ __forceinline__ void ant_walk_box(int r, int c, int d, int s, int ste
 // inside its tile, from the split down to size 1 and back up, in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kSmallLevels = 10;   // log2(kTile)
-constexpr int kSmallSeg = 64;      // walk steps per segment inside the small solver
+constexpr int kSmallSeg = 32;      // walk steps per segment inside the small solver
 constexpr int kSmallBoxS = 16;     // from this subproblem size on, walks are cut into searched segments
 constexpr int kSmallMaxSeg = 64;   // segments per level and tile: max(kTile / kSmallBoxS, 2 kTile / kSmallSeg)
 constexpr uint32_t kHi16 = 0x8000u;
