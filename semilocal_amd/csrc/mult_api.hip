@@ -290,7 +290,7 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
                            p->d_cntHi, p->d_cntLo);
         hipLaunchKernelGGL(table_cols, dim3((unsigned)ceil_div(L.ncol, 4)), dim3(256), 0, s, segs,
                            p->d_cols + L.col0, L.ncol, p->d_cntHi, p->d_cntLo);
-        hipLaunchKernelGGL(seg_search, dim3((unsigned)ceil_div(L.nwalk, 4)), dim3(256), 0, s, p->d_U, p->d_Ui,
+        hipLaunchKernelGGL(seg_search, dim3((unsigned)ceil_div(L.nwalk, 4 * kSearchRun)), dim3(256), 0, s, p->d_U, p->d_Ui,
                            p->d_cntHi, p->d_cntLo, segs, walks, L.nwalk, p->d_cend, p->d_start);
         hipLaunchKernelGGL(seg_walk, dim3((unsigned)(8 * ceil_div(ceil_div(L.nwalk, kWalkBlock), 8))), dim3(kWalkBlock), 0, s,
                            p->d_U, p->d_Ui, segs,

@@ -811,37 +811,41 @@ __device__ int wave_delta(const uint32_t *__restrict__ U, const uint32_t *__rest
 //                   + [column c's point: hi with row < R(c), or lo with row > R(c)]
 // (the cells that enter {row < R, col < c} and leave {row >= R, col >= c}),
 // so D over a short range is D(a) plus a prefix sum of terms that read only
-// U[R(c)] and Ui[c], both consecutive in c.  kWinSpan positions per pass: 16
-// per lane, lane L taking [w0 + 16 L, w0 + 16 L + 16) with w0 = a rounded down
-// to a multiple of 4.
-constexpr int kWinSpan = 1020;   // b - a bound: [a, b) lies inside [w0, w0 + 1024)
+// U[R(c)] and Ui[c], both consecutive in c.  A window pass takes PER
+// positions per lane, lane L taking [w0 + PER L, w0 + PER (L + 1)) with w0 = a
+// rounded down to a multiple of 4, so it settles b - a <= 64 PER - 4.
+constexpr int kWinPer = 16;                   // positions per lane after a binary search
+constexpr int kWinSpan = 64 * kWinPer - 4;    // b - a bound for that pass
+constexpr int kIncPer = 8;                    // positions per lane for a run's next segment
+static_assert(64 * kIncPer - 4 >= kSegSteps, "a run's next crossing lies within kSegSteps columns");
+static_assert(kSegSteps % 64 == 0, "row sums take kSegSteps / 64 rows per lane");
 
 // Lane terms of one window.  Ui is read as aligned 16-byte groups from w0;
 // U from R(w0) rounded down to a multiple of 4 (M = R(w0) mod 4, so entry
-// e + M is row R(w0 + 16 L + e)), one group more when M > 0.  A group is read
-// only when it holds a position below b, and for U when it starts at row >= 0
-// (rows below 0 belong to positions below a).  Such groups end at most 3
-// entries past row or column s - 1, inside the arrays' slack.  Positions
-// outside [a, b) get term 0.
-template <int M>
+// e + M is row R(w0 + PER L + e)), one group more when M > 0.  A group is
+// read only when it holds a position below b, and for U when it starts at
+// row >= 0 (rows below 0 belong to positions below a).  Such groups end at
+// most 3 entries past row or column s - 1, inside the arrays' slack.
+// Positions outside [a, b) get term 0.
+template <int M, int PER>
 __device__ __forceinline__ int window_terms(const uint32_t *__restrict__ Ux, const uint32_t *__restrict__ Uix,
-                                            int w0, int off, int a, int b, int (&dl)[16]) {
-    constexpr int GU = M > 0 ? 5 : 4;
+                                            int w0, int off, int a, int b, int (&dl)[PER]) {
+    constexpr int GV = PER / 4, GU = M > 0 ? GV + 1 : GV;
     const int lane = threadIdx.x & 63;
-    const int p0 = w0 + lane * 16;
-    const int rb = w0 + off - M + lane * 16;   // first U row read by this lane (a multiple of 4)
-    uint4 u[GU], v[4];
+    const int p0 = w0 + lane * PER;
+    const int rb = w0 + off - M + lane * PER;   // first U row read by this lane (a multiple of 4)
+    uint4 u[GU], v[GV];
 #pragma unroll
     for (int g = 0; g < GU; ++g) {
         const int gr = rb + 4 * g;
         u[g] = (gr >= 0 && gr - off < b) ? *reinterpret_cast<const uint4 *>(Ux + gr) : make_uint4(0u, 0u, 0u, 0u);
     }
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < GV; ++g) {
         const int gc = p0 + 4 * g;
         v[g] = gc < b ? *reinterpret_cast<const uint4 *>(Uix + gc) : make_uint4(0u, 0u, 0u, 0u);
     }
-    uint32_t ue[4 * GU], ve[16];
+    uint32_t ue[4 * GU], ve[PER];
 #pragma unroll
     for (int g = 0; g < GU; ++g) {
         ue[4 * g] = u[g].x;
@@ -850,7 +854,7 @@ __device__ __forceinline__ int window_terms(const uint32_t *__restrict__ Ux, con
         ue[4 * g + 3] = u[g].w;
     }
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < GV; ++g) {
         ve[4 * g] = v[g].x;
         ve[4 * g + 1] = v[g].y;
         ve[4 * g + 2] = v[g].z;
@@ -858,7 +862,7 @@ __device__ __forceinline__ int window_terms(const uint32_t *__restrict__ Ux, con
     }
     int sum = 0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < PER; ++e) {
         const int c = p0 + e, R = c + off;
         const uint32_t uu = ue[e + M], vv = ve[e];
         const int col = (int)(uu & kValMask), row = (int)(vv & kValMask);
@@ -871,23 +875,24 @@ __device__ __forceinline__ int window_terms(const uint32_t *__restrict__ Ux, con
 }
 
 // First c in [a, b] with D(c) >= 0 (b when none below b), given D(a) = Da and
-// b - a <= kWinSpan; returns (c, D(c)).  Uniform across the wave.
+// b - a <= 64 PER - 4; returns (c, D(c)).  Uniform across the wave.
+template <int PER>
 __device__ int2 wave_window_search(const uint32_t *__restrict__ Ux, const uint32_t *__restrict__ Uix, int off,
                                    int a, int b, int Da) {
     const int w0 = a & ~3;
-    int dl[16];
+    int dl[PER];
     int sum;
     switch ((w0 + off) & 3) {
-    case 0: sum = window_terms<0>(Ux, Uix, w0, off, a, b, dl); break;
-    case 1: sum = window_terms<1>(Ux, Uix, w0, off, a, b, dl); break;
-    case 2: sum = window_terms<2>(Ux, Uix, w0, off, a, b, dl); break;
-    default: sum = window_terms<3>(Ux, Uix, w0, off, a, b, dl); break;
+    case 0: sum = window_terms<0, PER>(Ux, Uix, w0, off, a, b, dl); break;
+    case 1: sum = window_terms<1, PER>(Ux, Uix, w0, off, a, b, dl); break;
+    case 2: sum = window_terms<2, PER>(Ux, Uix, w0, off, a, b, dl); break;
+    default: sum = window_terms<3, PER>(Ux, Uix, w0, off, a, b, dl); break;
     }
-    const int p0 = w0 + (int)(threadIdx.x & 63) * 16;
+    const int p0 = w0 + (int)(threadIdx.x & 63) * PER;
     int run = Da + wave_incl_scan(sum) - sum;   // D(p0) for p0 >= a
     int neg = 0, dneg = 0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < PER; ++e) {
         const int c = p0 + e;
         const int n = (c >= a && c < b && run < 0) ? 1 : 0;   // D(c) < 0 exactly on [a, answer)
         neg += n;
@@ -898,40 +903,73 @@ __device__ int2 wave_window_search(const uint32_t *__restrict__ Ux, const uint32
     return make_int2(a + (tot & 0xFFFF), Da + (tot >> 16));
 }
 
-// Start point of each walk segment: one wave per entry (subproblem, k); walk
-// segment k covers steps [k kSegSteps, (k+1) kSegSteps).  Output
-// (r, c, delta(r, c), t0), or r = -1 when the walk ends before step t0.
-// Binary search on D(c) = delta(c + s - t0, c) until at most kWinSpan
-// candidates remain, then one window pass; delta(r, c) one row above is
-// D(c) less row r's term.
+// Row r's term of delta: delta(r+1, c) - delta(r, c) = [hi with col < c, or lo
+// with col >= c].
+__device__ __forceinline__ int row_term(uint32_t u, int c) {
+    return ((int)(u & kValMask) >= c) ^ (int)(u >> 31);
+}
+
+// Start points of the walk segments: one wave per run of kSearchRun entries
+// (subproblem, k) of the list; walk segment k covers steps [k kSegSteps,
+// (k+1) kSegSteps).  Output (r, c, delta(r, c), t0), or r = -1 when the walk
+// ends before step t0.
+// A run's first entry (or one that does not continue its predecessor's walk)
+// binary-searches D(c) = delta(c + s - t0, c) until at most kWinSpan
+// candidates remain, then takes one window pass.  The next segment of the
+// same walk crosses diagonal t0 + kSegSteps within kSegSteps columns to the
+// right of this crossing c (a step moves one row or one column), and
+// D there at c is this D(c) less the row terms of the kSegSteps rows between
+// the two diagonals: one short sum instead of a search.  delta(r, c) one row
+// above the crossing is D(c) less row r's term.
 __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
                            const int32_t *__restrict__ cntHi, const int32_t *__restrict__ cntLo,
                            const SaSeg *__restrict__ segs, const int2 *__restrict__ walks, int nwalks,
                            const int32_t *__restrict__ c_end, int4 *__restrict__ seg_start) {
-    const int w = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
-    if (w >= nwalks) return;
-    const int2 job = walks[w];
-    const SaSeg sg = segs[job.x];
-    const int s = sg.s;
-    const int t_end = s - 1 + c_end[job.x];
-    const int t0 = job.y * kSegSteps;
-    int4 out = make_int4(-1, -1, 0, 0);
-    if (t0 <= t_end) {
-        const int off = s - t0;
-        int a = max(0, t0 - s + 1), b = min(t0, s);
-        while (b - a > kWinSpan) {
-            const int mid = (a + b) >> 1;
-            if (wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0) b = mid;
-            else a = mid + 1;
+    const int wv = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    int prev_seg = -1, prev_k = -1, pc = 0, pD = 0;   // last crossing of this run: c and D(c)
+    for (int j = 0; j < kSearchRun; ++j) {
+        const int w = wv * kSearchRun + j;
+        if (w >= nwalks) break;   // uniform
+        const int2 job = walks[w];
+        const SaSeg sg = segs[job.x];
+        const int s = sg.s;
+        const int t_end = s - 1 + c_end[job.x];
+        const int t0 = job.y * kSegSteps;
+        int4 out = make_int4(-1, -1, 0, 0);
+        if (t0 <= t_end) {
+            const int off = s - t0;
+            const int lo = max(0, t0 - s + 1), hi = min(t0, s);
+            const uint32_t *Ux = U + sg.x0;
+            int2 cd;
+            if (job.x == prev_seg && job.y == prev_k + 1 && pc >= lo) {
+                // rows [pc + off, pc + off + kSegSteps) lie in [1, s - 1]
+                int acc = 0;
+#pragma unroll
+                for (int k = 0; k < kSegSteps / 64; ++k) acc += row_term(Ux[pc + off + lane + 64 * k], pc);
+                const int Da = pD - __builtin_amdgcn_readfirstlane(wave_sum(acc));
+                cd = wave_window_search<kIncPer>(Ux, Ui + sg.x0, off, pc, min(pc + kSegSteps, hi), Da);
+            } else {
+                int a = lo, b = hi;
+                while (b - a > kWinSpan) {
+                    const int mid = (a + b) >> 1;
+                    if (wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0) b = mid;
+                    else a = mid + 1;
+                }
+                const int Da = wave_delta(U, Ui, cntHi, cntLo, sg, a + off, a);
+                cd = wave_window_search<kWinPer>(Ux, Ui + sg.x0, off, a, b, Da);
+            }
+            const int c = cd.x, r = c + off - 1;
+            out = make_int4(r, c, cd.y - row_term(Ux[r], c), t0);
+            prev_seg = job.x;
+            prev_k = job.y;
+            pc = c;
+            pD = cd.y;
+        } else {
+            prev_seg = -1;
         }
-        const int Da = wave_delta(U, Ui, cntHi, cntLo, sg, a + off, a);
-        const int2 cd = wave_window_search(U + sg.x0, Ui + sg.x0, off, a, b, Da);
-        const int c = cd.x, r = c + off - 1;
-        const uint32_t ur = U[sg.x0 + r];
-        const int dr = cd.y - (((int)(ur & kValMask) >= c) ^ (int)(ur >> 31));
-        out = make_int4(r, c, dr, t0);
+        if (lane == 0) seg_start[w] = out;
     }
-    if ((threadIdx.x & 63) == 0) seg_start[w] = out;
 }
 
 // Walks: one lane per segment, reading the level's U / U^-1 a window of
