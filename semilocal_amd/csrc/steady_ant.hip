@@ -446,6 +446,7 @@ __device__ __forceinline__ void ant_walk_box(int r, int c, int d, int s, int ste
 // ---------------------------------------------------------------------------
 constexpr int kSmallLevels = 10;   // log2(kTile)
 constexpr int kSmallSeg = 32;      // walk steps per segment inside the small solver
+static_assert(kSmallSeg <= 64, "a run's row sum takes one row per lane");
 constexpr int kSmallBoxS = 16;     // from this subproblem size on, walks are cut into searched segments
 constexpr int kSmallMaxSeg = 64;   // segments per level and tile: max(kTile / kSmallBoxS, 2 kTile / kSmallSeg)
 constexpr uint32_t kHi16 = 0x8000u;
@@ -469,20 +470,19 @@ __device__ __forceinline__ int wave_delta_lds(const uint16_t *U16, int s, int r,
     return __builtin_amdgcn_readfirstlane(wave_sum(acc));
 }
 
-// Crossing of anti-diagonal t0 by the walk of an LDS subproblem of size s:
-// the first c in [a, b] = [max(0, t0-s+1), min(t0, s)] with
-// D(c) = delta(c + s - t0, c) >= 0, and delta(r, c) at r = c + s - t0 - 1.
-// D(c+1) - D(c) reads only row R(c)'s and column c's points (the identity
-// above wave_window_search), and b - a <= s <= kTile, so one pass over
-// positions c = a + 64 e + lane (e < 16) finds it: per row of 64 positions
-// the terms (0, 1 or 2) go into two ballots, a lane's prefix is two masked
-// bit counts, and the uniform running total carries D across rows.  D is
-// monotone, so the pass stops at the first row holding a non-negative D.
-__device__ __forceinline__ int2 wave_cross_lds(const uint16_t *U16, const uint16_t *Ui16, int s, int t0) {
+// Crossing of an anti-diagonal by the walk of an LDS subproblem: with
+// R(c) = c + off, the first c in [a, b] with D(c) = delta(R(c), c) >= 0
+// (b when none below b), given D(a) = Da and b - a <= kTile; returns
+// (c, D(c)).  D(c+1) - D(c) reads only row R(c)'s and column c's points (the
+// identity above wave_window_search), so one pass over positions
+// c = a + 64 e + lane finds it: per row of 64 positions the terms (0, 1 or
+// 2) go into two ballots, a lane's prefix is two masked bit counts, and the
+// uniform running total carries D across rows.  D is monotone, so the pass
+// stops at the first row holding a non-negative D.
+__device__ __forceinline__ int2 wave_window_lds(const uint16_t *U16, const uint16_t *Ui16, int off, int a, int b,
+                                                int Da) {
     const int lane = threadIdx.x & 63;
-    const int off = s - t0;
-    const int a = max(0, t0 - s + 1), b = min(t0, s);
-    int carry = wave_delta_lds(U16, s, a + off, a);   // D(a), then D at the current row's first position
+    int carry = Da;   // D at the current row's first position
     int c_ans = a;
     for (int e = 0; e < kTile / 64; ++e) {
         const int c0 = a + 64 * e;
@@ -506,11 +506,12 @@ __device__ __forceinline__ int2 wave_cross_lds(const uint16_t *U16, const uint16
         }
         carry += __popcll(m1) + 2 * __popcll(m2);
     }
-    // carry = D(c_ans); delta one row above subtracts row r's term
-    const int r = c_ans + off - 1;
-    const uint32_t ur = U16[r];
-    const int dr = carry - (((int)(ur & 0x7FFFu) >= c_ans) ^ (int)((ur & kHi16) != 0));
-    return make_int2(c_ans, dr);
+    return make_int2(c_ans, carry);
+}
+
+// Row r's term of delta in LDS form: delta(r+1, c) - delta(r, c).
+__device__ __forceinline__ int row_term16(uint32_t u, int c) {
+    return ((int)(u & 0x7FFFu) >= c) ^ (int)((u & kHi16) != 0);
 }
 
 __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restrict__ P0, const int32_t *__restrict__ Qi0,
@@ -598,22 +599,44 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
             }
         } else {
             // segments of kSmallSeg steps.  Their start points are found on
-            // anti-diagonals, one wave per segment (wave_cross_lds); then
-            // every segment is walked by its own lane, so the walks run side
-            // by side on the vector ALU.
+            // anti-diagonals, each wave taking a run of consecutive segments;
+            // then every segment is walked by its own lane, so the walks run
+            // side by side on the vector ALU.  A run's first segment (or one
+            // that starts another subproblem) takes D at the diagonal's lower
+            // end by an LDS scan and one window pass over the diagonal; the
+            // next segment of the same walk crosses within kSmallSeg columns
+            // of this crossing c, and D there at c is this D(c) less the row
+            // terms of the kSmallSeg rows between the two diagonals.
             const int nsps = max(1, (2 * S) / kSmallSeg);
             const int nsub = (len + S - 1) / S;
             const int nseg = nsub * nsps;   // <= kSmallMaxSeg
-            for (int w = wid; w < nseg; w += kTile / 64) {
+            const int per = (nseg + kTile / 64 - 1) / (kTile / 64);
+            int prev = -1, pc = 0, pD = 0;   // last segment of this run with a crossing: index, c, D(c)
+            for (int w = wid * per; w < min(nseg, (wid + 1) * per); ++w) {
                 const int sub = w / nsps, k = w - sub * nsps;
                 const int s0 = sub * S, s = min(S, len - s0);
                 const int t_end = s - 1 + sCend[sub];
                 const int t0 = k * kSmallSeg;
                 int4 st = make_int4(-1, 0, 0, 0);
                 if (s > h && t0 <= t_end) {   // uniform over the wave
-                    const int2 cd = wave_cross_lds(sU + s0, sUi + s0, s, t0);
-                    st = make_int4(s - 1 - t0 + cd.x, cd.x, cd.y,
+                    const uint16_t *U16 = sU + s0, *Ui16 = sUi + s0;
+                    const int off = s - t0;
+                    const int lo = max(0, t0 - s + 1), hi = min(t0, s);
+                    int2 cd;
+                    if (prev == w - 1 && k > 0 && pc >= lo) {
+                        // rows [pc + off, pc + off + kSmallSeg) lie in [1, s - 1]
+                        const int acc = lane < kSmallSeg ? row_term16(U16[pc + off + lane], pc) : 0;
+                        const int Da = pD - __builtin_amdgcn_readfirstlane(wave_sum(acc));
+                        cd = wave_window_lds(U16, Ui16, off, pc, min(pc + kSmallSeg, hi), Da);
+                    } else {
+                        cd = wave_window_lds(U16, Ui16, off, lo, hi, wave_delta_lds(U16, s, lo + off, lo));
+                    }
+                    const int r = cd.x + off - 1;
+                    st = make_int4(r, cd.x, cd.y - row_term16(U16[r], cd.x),
                                    nsps == 1 ? t_end + 1 : min(kSmallSeg, t_end - t0 + 1));
+                    prev = w;
+                    pc = cd.x;
+                    pD = cd.y;
                 }
                 if (lane == 0) sStart[w] = st;
             }
