@@ -8,7 +8,7 @@ namespace sa {
 
 constexpr int kTile = 1024;      // elements per block of the split / small-solver kernels
 constexpr int kBlk = 1024;       // block size of the per-subproblem count tables
-constexpr int kSegSteps = 256;   // walk steps per parallel walk segment (large levels)
+constexpr int kSegSteps = 128;   // walk steps per parallel walk segment (large levels)
 constexpr int kSearchRun = 4;    // consecutive walk segments whose starts one wave finds (large levels)
 constexpr int kWalkBlock = 64;   // lanes (segments) per block of the large-level walk kernel
 constexpr int kWalkWindow = 16;  // rows and columns a large-level walk reads ahead per round trip
