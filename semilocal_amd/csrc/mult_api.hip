@@ -239,7 +239,7 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
     PLAN_TRY(dalloc(&p->d_Ui, nt + kWalkSlack));
     PLAN_TRY(dalloc(&p->d_cntP, (size_t)p->ntiles));
     PLAN_TRY(dalloc(&p->d_cntQ, (size_t)p->ntiles));
-    PLAN_TRY(dalloc(&p->d_cend, (size_t)p->max_seg));
+    PLAN_TRY(dalloc(&p->d_cend, std::max<size_t>(segs.size(), 1)));   // one entry per segment of every level
     PLAN_TRY(dalloc(&p->d_cntHi, (size_t)p->max_tab));
     PLAN_TRY(dalloc(&p->d_cntLo, (size_t)p->max_tab));
     PLAN_TRY(dalloc(&p->d_start, (size_t)p->max_walk));
@@ -276,14 +276,18 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
     hipLaunchKernelGGL(solve_small, dim3(nt), dim3(kTile), 0, s, p->d_P[cur], p->d_Qi[cur], p->d_tile_prob,
                        p->d_probs, p->S0, p->d_R[0]);
     int rc = 0;
+    if (!p->lv.empty()) {   // every level's c_end entries, cleared by one launch
+        const Level &L0 = p->lv.back();
+        SL_HIP_TRY(fill(p->d_cend, 0, (int64_t)(L0.seg0 + L0.nseg), s));
+    }
     for (size_t k = p->lv.size(); k-- > 0;) {
         const Level &L = p->lv[k];
         const SaSeg *segs = p->d_segs + L.seg0;
         const int2 *walks = p->d_walks + L.walk0;
-        SL_HIP_TRY(fill(p->d_cend, 0, L.nseg, s));
+        int32_t *cend = p->d_cend + L.seg0;
         hipLaunchKernelGGL(merge_map, dim3(eb), dim3(256), 0, s, p->d_R[rc], p->d_PR + k * ntot,
                            p->d_QC + k * ntot, p->d_tile_prob, p->d_probs, p->d_seg_first + L.sf0, L.S, ntot,
-                           p->d_U, p->d_Ui, p->d_R[rc ^ 1], p->d_cend);
+                           p->d_U, p->d_Ui, p->d_R[rc ^ 1], cend);
         rc ^= 1;
         if (L.nwalk == 0) continue;
         hipLaunchKernelGGL(table_rows, dim3((unsigned)L.nrow), dim3(256), 0, s, p->d_U, segs, p->d_rows + L.row0,
@@ -291,10 +295,10 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
         hipLaunchKernelGGL(table_cols, dim3((unsigned)ceil_div(L.ncol, 4)), dim3(256), 0, s, segs,
                            p->d_cols + L.col0, L.ncol, p->d_cntHi, p->d_cntLo);
         hipLaunchKernelGGL(seg_search, dim3((unsigned)ceil_div(L.nwalk, 4 * kSearchRun)), dim3(256), 0, s, p->d_U, p->d_Ui,
-                           p->d_cntHi, p->d_cntLo, segs, walks, L.nwalk, p->d_cend, p->d_start);
+                           p->d_cntHi, p->d_cntLo, segs, walks, L.nwalk, cend, p->d_start);
         hipLaunchKernelGGL(seg_walk, dim3((unsigned)(8 * ceil_div(ceil_div(L.nwalk, kWalkBlock), 8))), dim3(kWalkBlock), 0, s,
                            p->d_U, p->d_Ui, segs,
-                           walks, L.nwalk, p->d_cend, p->d_start, p->d_R[rc]);
+                           walks, L.nwalk, cend, p->d_start, p->d_R[rc]);
     }
     p->result = rc;
     SL_HIP_TRY(hipGetLastError());

@@ -45,24 +45,28 @@ namespace sa {
 constexpr uint32_t kHiBit = 0x80000000u;
 constexpr uint32_t kValMask = 0x7FFFFFFFu;
 
-__device__ __forceinline__ int block_excl_scan(int v, int *tmp /* blockDim/64 + 1 ints */, int &total) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int x = v;
+// Inclusive scan across the 64 lanes of a wave.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off);
-        if (lane >= off) x += y;
+        const int y = __shfl_up(v, off);
+        if (lane >= off) v += y;
     }
+    return v;
+}
+
+__device__ __forceinline__ int block_excl_scan(int v, int *tmp /* blockDim/64 + 1 ints */, int &total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int x = wave_incl_scan(v);
     if (lane == 63) tmp[wid] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-            const int t = tmp[w];
-            tmp[w] = acc;
-            acc += t;
-        }
-        tmp[blockDim.x >> 6] = acc;
+    if (wid == 0) {   // the wave totals (at most 64) scanned by wave 0
+        const int nw = (int)(blockDim.x >> 6);
+        const int t = lane < nw ? tmp[lane] : 0;
+        const int inc = wave_incl_scan(t);
+        if (lane < nw) tmp[lane] = inc - t;
+        if (lane == nw - 1) tmp[nw] = inc;
     }
     __syncthreads();
     total = tmp[blockDim.x >> 6];
@@ -74,17 +78,6 @@ __device__ __forceinline__ int block_excl_scan(int v, int *tmp /* blockDim/64 + 
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
-// Inclusive scan across the 64 lanes of a wave.
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(v, off);
-        if (lane >= off) v += y;
-    }
     return v;
 }
 
@@ -137,12 +130,11 @@ __global__ __launch_bounds__(kTile) void split_count(const int32_t *__restrict__
         fp = P[x] < h;
         fq = Qi[x] < h;
     }
-    int tp, tq;
-    (void)block_excl_scan(fp, tmp, tp);
-    (void)block_excl_scan(fq, tmp, tq);
+    int tpq;   // both counts in one scan: each is at most kTile < 2^16
+    (void)block_excl_scan(fp | (fq << 16), tmp, tpq);
     if (threadIdx.x == 0) {
-        cntP[t] = tp;
-        cntQ[t] = tq;
+        cntP[t] = tpq & 0xFFFF;
+        cntQ[t] = tpq >> 16;
     }
 }
 
@@ -187,9 +179,10 @@ __global__ __launch_bounds__(kTile) void split_scatter(const int32_t *__restrict
         fp = p < h;
         fq = q < h;
     }
-    int tp, tq;
-    const int exP = block_excl_scan(fp, tmp, tp) + offP[t];
-    const int exQ = block_excl_scan(fq, tmp, tq) + offQ[t];
+    int tpq;   // both flags in one scan: each count is at most kTile < 2^16
+    const int ex = block_excl_scan(fp | (fq << 16), tmp, tpq);
+    const int exP = (ex & 0xFFFF) + offP[t];
+    const int exQ = (ex >> 16) + offQ[t];
     if (in) {
         const int64_t seg0 = pb.base + (lx / S) * S;
         const int r = (int)(lx % S);
@@ -546,11 +539,14 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
         const int p = in ? sP[cur][x] : 0, q = in ? sQ[cur][x] : 0;
         const int fp = in && p < h, fq = in && q < h;
         int tot;
-        // segmented exclusive scans: the block scan minus its value at the segment start
-        const int exP_all = block_excl_scan(fp, tmp, tot);
-        if (in && r == 0) segP[x] = (int16_t)exP_all;
-        const int exQ_all = block_excl_scan(fq, tmp, tot);
-        if (in && r == 0) segQ[x] = (int16_t)exQ_all;
+        // segmented exclusive scans: the block scan minus its value at the
+        // segment start; both flags in one scan (each count < 2^16)
+        const int ex_all = block_excl_scan(fp | (fq << 16), tmp, tot);
+        const int exP_all = ex_all & 0xFFFF, exQ_all = ex_all >> 16;
+        if (in && r == 0) {
+            segP[x] = (int16_t)exP_all;
+            segQ[x] = (int16_t)exQ_all;
+        }
         __syncthreads();
         if (in) {
             const int exP = exP_all - segP[seg0];
