@@ -969,10 +969,24 @@ __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__res
                 const int Da = pD - __builtin_amdgcn_readfirstlane(wave_sum(acc));
                 cd = wave_window_search<kIncPer>(Ux, Ui + sg.x0, off, pc, min(pc + kSegSteps, hi), Da);
             } else {
+                // delta is nondecreasing in r and in c, so the count tables'
+                // corners around (R, c) bound D(c): delta(i kBlk, j kBlk) <=
+                // D(c) <= delta((i+1) kBlk, (j+1) kBlk).  Most halvings are
+                // decided by the two table entries; only where the bounds
+                // straddle 0 does a probe read the strips.
+                const int W = sg.nb + 1;
+                const int32_t *th = cntHi + sg.tab_off2, *tl = cntLo + sg.tab_off2;
                 int a = lo, b = hi;
                 while (b - a > kWinSpan) {
                     const int mid = (a + b) >> 1;
-                    if (wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0) b = mid;
+                    const int i = (mid + off) / kBlk, j = mid / kBlk;
+                    const int i1 = min(i + 1, sg.nb), j1 = min(j + 1, sg.nb);
+                    const int64_t t0i = (int64_t)j * W + i, t1i = (int64_t)j1 * W + i1;
+                    bool ge;
+                    if (th[t0i] - tl[t0i] >= 0) ge = true;
+                    else if (th[t1i] - tl[t1i] < 0) ge = false;
+                    else ge = wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0;
+                    if (ge) b = mid;
                     else a = mid + 1;
                 }
                 const int Da = wave_delta(U, Ui, cntHi, cntLo, sg, a + off, a);
