@@ -582,13 +582,16 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
         if (in) sR[x] = (int16_t)(sU[x] & 0x7FFFu);
         __syncthreads();
         if (S < kSmallBoxS) {
-            // short walks (< 2 kSmallBoxS steps): one lane per subproblem, from the start
-            if (in && x == seg0) {
-                const int s = min(S, len - x);
+            // short walks (< 2 kSmallBoxS steps): one lane per subproblem,
+            // from the start; lane x takes subproblem x, so the walks fill
+            // the first waves instead of a few lanes of every wave
+            const int nsub = (len + S - 1) / S;
+            if (x < nsub) {
+                const int s0 = x * S, s = min(S, len - s0);
                 if (s > h) {
-                    const uint16_t *U16 = sU + x, *Ui16 = sUi + x;
-                    int16_t *R = sR + x;
-                    ant_walk(s - 1, 0, (U16[s - 1] & kHi16) ? 0 : -1, s, s + sCend[x / S],
+                    const uint16_t *U16 = sU + s0, *Ui16 = sUi + s0;
+                    int16_t *R = sR + s0;
+                    ant_walk(s - 1, 0, (U16[s - 1] & kHi16) ? 0 : -1, s, s + sCend[x],
                              [&](int rr) { return widen16(U16[rr]); }, [&](int cc) { return widen16(Ui16[cc]); },
                              [&](int rr, int cc) { R[rr] = (int16_t)cc; });
                 }
