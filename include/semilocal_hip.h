@@ -123,10 +123,11 @@ int sl_batch_set_schedule(sl_batch *bt, int schedule);
 int sl_batch_get_schedule(const sl_batch *bt, int *schedule);
 /* Autotuner: times every tile shape -- rows per lane {4, 8, 16} x persistent
  * waves per CU {4, 8, 12, 16, 20, 24, 32} x queue order {0..3} -- on this batch
- * (`iters` runs each, HIP events), re-times the four fastest after a warm-up
- * run with max(10, 2 iters + 1) runs and keeps the fastest of those; reports
- * the shape through the
- * out-pointers (the order through sl_batch_get_schedule). */
+ * (`iters` runs each, HIP events), re-times the four fastest warm in 7
+ * interleaved rounds (per round and shape one warm-up run, then
+ * max(3, iters) timed runs) and keeps the one with the fastest median round;
+ * reports the shape through the out-pointers (the order through
+ * sl_batch_get_schedule). */
 int sl_batch_autotune(sl_batch *bt, int iters, int *rows_per_lane, int *waves_per_cu);
 /* Sum of m*n over the batch (cells), number of queue tiles, number of pairs. */
 int sl_batch_stats(const sl_batch *bt, int64_t *cells, int64_t *tiles, int64_t *pairs);

@@ -920,9 +920,12 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
         return rc;
     };
     // every shape `iters` cold runs (each run is a whole launch), then the
-    // four fastest again after a warm-up run with more runs, as a caller's
-    // steady state runs them: shapes a few percent apart trade places under
-    // the first pass's noise and cold launches
+    // four fastest again warm, as a caller's steady state runs them: shapes a
+    // few percent apart trade places under the first pass's noise and cold
+    // launches.  The finalists are timed in interleaved rounds (a warm-up run
+    // and a few timed runs each, shape after shape) and ranked by their
+    // median round, so a drift of the clock over the seconds of tuning does
+    // not favour the shape timed first or last.
     const int wpc_cand[] = {4, 8, 12, 16, 20, 24, 32};
     std::vector<Shape> shapes;
     for (int sched = 0; sched <= kMaxSchedule; ++sched)
@@ -935,9 +938,19 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
             }
     std::sort(shapes.begin(), shapes.end(), [](const Shape &x, const Shape &y) { return x.ms < y.ms; });
     const size_t finalists = std::min<size_t>(4, shapes.size());
+    constexpr int kRounds = 7;
+    std::vector<std::vector<double>> round_ms(finalists);
+    for (int round = 0; round < kRounds; ++round)
+        for (size_t i = 0; i < finalists; ++i) {
+            Shape sh = shapes[i];
+            const int rc = time_shape(sh, 1, std::max(3, iters));
+            if (rc) return rc;
+            round_ms[i].push_back(sh.ms);
+        }
     for (size_t i = 0; i < finalists; ++i) {
-        const int rc = time_shape(shapes[i], 1, std::max(10, 2 * iters + 1));
-        if (rc) return rc;
+        std::vector<double> &v = round_ms[i];
+        std::nth_element(v.begin(), v.begin() + kRounds / 2, v.end());
+        shapes[i].ms = v[kRounds / 2];
     }
     const Shape best = *std::min_element(shapes.begin(), shapes.begin() + (std::ptrdiff_t)finalists,
                                          [](const Shape &x, const Shape &y) { return x.ms < y.ms; });
