@@ -37,7 +37,6 @@ __global__ void load_pq(const int32_t *, const int32_t *, const int64_t *, const
 __global__ void store_r(const int32_t *, const int32_t *, const SaProb *, const int64_t *, int32_t *);
 __global__ void split_count(const int32_t *, const int32_t *, const int32_t *, const SaProb *, int, int32_t *,
                             int32_t *);
-__global__ void seg_tile_scan(const SaSeg *, int32_t *, int32_t *);
 __global__ void split_scatter(const int32_t *, const int32_t *, const int32_t *, const SaProb *, int,
                               const int32_t *, const int32_t *, int32_t *, int32_t *, int32_t *, int32_t *);
 __global__ void solve_small(const int32_t *, const int32_t *, const int32_t *, const SaProb *, int, int32_t *);
@@ -264,10 +263,8 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
     int cur = 0;
     for (size_t li = 0; li < p->lv.size(); ++li) {
         const Level &L = p->lv[li];
-        const SaSeg *segs = p->d_segs + L.seg0;
         hipLaunchKernelGGL(split_count, dim3(nt), dim3(kTile), 0, s, p->d_P[cur], p->d_Qi[cur], p->d_tile_prob,
                            p->d_probs, L.S, p->d_cntP, p->d_cntQ);
-        hipLaunchKernelGGL(seg_tile_scan, dim3(L.nseg), dim3(kTile), 0, s, segs, p->d_cntP, p->d_cntQ);
         hipLaunchKernelGGL(split_scatter, dim3(nt), dim3(kTile), 0, s, p->d_P[cur], p->d_Qi[cur], p->d_tile_prob,
                            p->d_probs, L.S, p->d_cntP, p->d_cntQ, p->d_P[cur ^ 1], p->d_Qi[cur ^ 1],
                            p->d_PR + li * ntot, p->d_QC + li * ntot);

@@ -138,39 +138,46 @@ __global__ __launch_bounds__(kTile) void split_count(const int32_t *__restrict__
     }
 }
 
-// Exclusive scan of the tile counts of one segment (one block per segment).
-__global__ __launch_bounds__(kTile) void seg_tile_scan(const SaSeg *__restrict__ segs, int32_t *__restrict__ cntP,
-                                                       int32_t *__restrict__ cntQ) {
-    __shared__ int tmp[kTile / 64 + 1];
-    const SaSeg sg = segs[blockIdx.x];
-    for (int which = 0; which < 2; ++which) {
-        int32_t *cnt = which == 0 ? cntP : cntQ;
-        int carry = 0;
-        for (int b0 = 0; b0 < sg.ntiles; b0 += kTile) {
-            const int k = b0 + (int)threadIdx.x;
-            const bool ok = k < sg.ntiles;
-            const int v = ok ? cnt[sg.tile0 + k] : 0;
-            int tot;
-            const int ex = block_excl_scan(v, tmp, tot);
-            if (ok) cnt[sg.tile0 + k] = carry + ex;
-            carry += tot;
-        }
-    }
-}
-
+// Each block first sums the counts of its segment's earlier tiles (one
+// coalesced read of at most S / kTile counts and a block reduction): the
+// tile's exclusive offsets, without a separate scan launch.  Segments start
+// on tile boundaries (problem bases and S are multiples of kTile).
 __global__ __launch_bounds__(kTile) void split_scatter(const int32_t *__restrict__ P, const int32_t *__restrict__ Qi,
                                                        const int32_t *__restrict__ tile_prob,
                                                        const SaProb *__restrict__ probs, int S,
-                                                       const int32_t *__restrict__ offP,
-                                                       const int32_t *__restrict__ offQ, int32_t *__restrict__ P1,
+                                                       const int32_t *__restrict__ cntP,
+                                                       const int32_t *__restrict__ cntQ, int32_t *__restrict__ P1,
                                                        int32_t *__restrict__ Qi1, int32_t *__restrict__ PR1,
                                                        int32_t *__restrict__ QC1) {
     __shared__ int tmp[kTile / 64 + 1];
+    __shared__ int red[2 * (kTile / 64)];
     const int t = blockIdx.x;
     const SaProb pb = probs[tile_prob[t]];
     const int64_t x = (int64_t)t * kTile + threadIdx.x;
     const int64_t lx = x - pb.base;
     const bool in = lx < pb.n;
+    int offp = 0, offq = 0;
+    {
+        const int64_t lx0 = (int64_t)t * kTile - pb.base;
+        const int64_t tile0 = (pb.base + (lx0 / S) * S) / kTile;
+        int vp = 0, vq = 0;
+        for (int64_t u = tile0 + threadIdx.x; u < t; u += kTile) {
+            vp += cntP[u];
+            vq += cntQ[u];
+        }
+        vp = wave_sum(vp);
+        vq = wave_sum(vq);
+        if ((threadIdx.x & 63) == 0) {
+            red[2 * (threadIdx.x >> 6)] = vp;
+            red[2 * (threadIdx.x >> 6) + 1] = vq;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kTile / 64; ++w) {
+            offp += red[2 * w];
+            offq += red[2 * w + 1];
+        }
+    }
     const int h = S >> 1;
     int fp = 0, fq = 0, p = 0, q = 0;
     if (in) {
@@ -181,8 +188,8 @@ __global__ __launch_bounds__(kTile) void split_scatter(const int32_t *__restrict
     }
     int tpq;   // both flags in one scan: each count is at most kTile < 2^16
     const int ex = block_excl_scan(fp | (fq << 16), tmp, tpq);
-    const int exP = (ex & 0xFFFF) + offP[t];
-    const int exQ = (ex >> 16) + offQ[t];
+    const int exP = (ex & 0xFFFF) + offp;
+    const int exQ = (ex >> 16) + offq;
     if (in) {
         const int64_t seg0 = pb.base + (lx / S) * S;
         const int r = (int)(lx % S);
