@@ -350,96 +350,6 @@ __device__ __forceinline__ void ant_walk_window4(int r, int c, int d, int s, int
     }
 }
 
-// The same walk by a whole wave, 8 x 8 cells at a time.  Lane (i, j) =
-// (lane >> 3, lane & 7) evaluates delta at (r - i, c + j):
-//     delta(r - i, c + j) = d + sum_{j' < j} wcol(c + j', r)
-//                             - sum_{1 <= i' <= i} wrow(r - i', c + j),
-// where wcol(col, row) is delta's step when the walk moves right past column
-// col in that row and wrow(row, col) its step when it moves up into that row.
-// Both sums are population counts of two ballots under per-lane masks, so a
-// box costs one round trip of reads and no cross-lane shuffles.  A third
-// ballot, of (delta < 0), holds every move of the walk inside the box; a
-// scalar loop follows it one row at a time (count of trailing ones = the run
-// of right moves) until the path reaches the box's last row or column, and
-// every visited cell then tests the good-point condition at once.  rowv /
-// colv clamp out-of-range indices themselves: such values only reach cells
-// the walk never visits.
-template <class RowF, class ColF, class PutF>
-__device__ __forceinline__ void ant_walk_box(int r, int c, int d, int s, int steps, RowF rowv, ColF colv, PutF put) {
-    const int lane = threadIdx.x & 63, li = lane >> 3, lj = lane & 7;
-    const uint64_t colmask = 0x0101010101010101ull << lj;   // lanes of column j
-    const uint64_t upto = ~0ull >> (63 - lane);              // lanes 0 .. lane
-    const uint32_t jmask = (1u << lj) - 1u;                  // columns before j
-    int t = 0;
-    for (;;) {
-        const uint32_t ur = rowv(r - li);        // point of row r - i
-        const uint32_t uc = colv(c + lj);        // point of column c + j
-        const uint32_t ul = colv(c + lj - 1);    // point of column c + j - 1
-        const int rowc = (int)(uc & kValMask);
-        const bool wcol = (uc & kHiBit) ? (rowc < r) : (rowc >= r);
-        const int colr = (int)(ur & kValMask);
-        const bool wrow = (ur & kHiBit) ? (colr < c + lj) : (colr >= c + lj);
-        const uint64_t W = __builtin_amdgcn_ballot_w64(li == 0 && wcol);
-        const uint64_t Bm = __builtin_amdgcn_ballot_w64(li >= 1 && wrow);
-        const int delta = d + __builtin_popcount((uint32_t)W & jmask) - __builtin_popcountll(Bm & colmask & upto);
-        const uint64_t M = __builtin_amdgcn_ballot_w64(delta < 0);
-        int i = 0, j = 0, ni = 0, nj = 0;
-        uint64_t V = 0;
-        bool fin = false;
-        for (;;) {
-            const uint32_t rowb = (uint32_t)(M >> (8 * i)) & 0xFFu;
-            const int run = __builtin_ctz(~(rowb >> j));   // delta < 0 at columns j .. j + run - 1
-            const int jc = max(j, s - c);                  // moving right from column >= s ends the walk
-            int term, kind;                                // kind 0: ends, 1: leaves right, 2: moves up
-            if (run > 0 && jc <= min(j + run - 1, 6)) {
-                term = jc;
-                kind = 0;
-            } else if (j + run >= 7) {
-                term = 6;
-                kind = 1;
-            } else {
-                term = j + run;
-                kind = 2;
-            }
-            int cnt = term - j + 1;
-            if (cnt > steps - t) {
-                cnt = steps - t;
-                kind = 0;
-            }
-            if (cnt > 0) V |= (((1ull << cnt) - 1ull) << j) << (8 * i);
-            t += cnt;
-            if (kind == 0) {
-                fin = true;
-                break;
-            }
-            if (kind == 1) {
-                ni = i;
-                nj = 7;
-                break;
-            }
-            if (r - i == 0) {   // moving up from row 0 ends the walk
-                fin = true;
-                break;
-            }
-            ++i;
-            j = term;
-            if (i == 7) {
-                ni = 7;
-                nj = j;
-                break;
-            }
-        }
-        const int rowl = (int)(ul & kValMask);
-        const bool wleft = (ul & kHiBit) ? (rowl < r - li) : (rowl >= r - li);
-        if (((V >> lane) & 1ull) && c + lj > 0 && delta - (int)wleft < 0 && delta + (int)wrow > 0)
-            put(r - li, c + lj - 1);
-        if (fin) break;
-        d = __builtin_amdgcn_readlane(delta, ni * 8 + nj);
-        r -= ni;
-        c += nj;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Small subproblems: one block solves every subproblem of size <= S0 <= kTile
 // inside its tile, from the split down to size 1 and back up, in LDS.
