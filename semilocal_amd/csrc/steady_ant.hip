@@ -677,26 +677,41 @@ __global__ void table_cols(const SaSeg *__restrict__ segs, const int2 *__restric
     const int nb = sg.nb, W = nb + 1, j = job.y;
     int32_t *ch = cntHi + sg.tab_off2 + (int64_t)j * W;
     int32_t *cl = cntLo + sg.tab_off2 + (int64_t)j * W;
-    int carry = 0;
-    for (int i0 = 0; i0 <= nb; i0 += 64) {
-        const int i = i0 + lane;
-        const int v = i < nb ? ch[i] : 0;
-        const int inc = wave_incl_scan(v);
-        if (i <= nb) ch[i] = carry + inc - v;
-        carry += __shfl(inc, 63);
+    // loads issued G chunks of 64 entries at a time, so a column of nb + 1
+    // entries costs a few memory round trips, not one per chunk
+    constexpr int G = 8;
+    int total = 0;   // lo over all block-rows
+    for (int i0 = 0; i0 < nb; i0 += 64 * G) {
+        int v[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int i = i0 + 64 * g + lane;
+            v[g] = i < nb ? cl[i] : 0;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) total += v[g];
     }
-    int total = 0;
-    for (int i0 = 0; i0 < nb; i0 += 64) {
-        const int i = i0 + lane;
-        total += wave_sum(i < nb ? cl[i] : 0);
-    }
-    carry = 0;
-    for (int i0 = 0; i0 <= nb; i0 += 64) {
-        const int i = i0 + lane;
-        const int v = i < nb ? cl[i] : 0;
-        const int inc = wave_incl_scan(v);
-        if (i <= nb) cl[i] = total - (carry + inc - v);
-        carry += __shfl(inc, 63);
+    total = wave_sum(total);
+    int carryH = 0, carryL = 0;
+    for (int i0 = 0; i0 <= nb; i0 += 64 * G) {
+        int vh[G], vl[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int i = i0 + 64 * g + lane;
+            vh[g] = i < nb ? ch[i] : 0;
+            vl[g] = i < nb ? cl[i] : 0;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int i = i0 + 64 * g + lane;
+            const int ih = wave_incl_scan(vh[g]), il = wave_incl_scan(vl[g]);
+            if (i <= nb) {
+                ch[i] = carryH + ih - vh[g];
+                cl[i] = total - (carryL + il - vl[g]);
+            }
+            carryH += __shfl(ih, 63);
+            carryL += __shfl(il, 63);
+        }
     }
 }
 
