@@ -650,9 +650,10 @@ __global__ __launch_bounds__(256) void table_rows(const uint32_t *__restrict__ U
     for (int j0 = 0; j0 <= nb; j0 += 256) {
         const int j = j0 + (int)threadIdx.x;
         const int vh = j < nb ? hh[j] : 0, vl = j < nb ? hl[j] : 0;
-        int th, tl;
-        const int eh = block_excl_scan(vh, tmp, th);
-        const int el = block_excl_scan(vl, tmp, tl);
+        // hi and lo in one scan: a block-row holds at most kBlk < 2^16 points
+        int tt;
+        const int e = block_excl_scan(vh | (vl << 16), tmp, tt);
+        const int eh = e & 0xFFFF, el = e >> 16, th = tt & 0xFFFF, tl = tt >> 16;
         if (j <= nb) {
             ch[(int64_t)j * W + i] = carryH + eh;
             cl[(int64_t)j * W + i] = totalL - (carryL + el);
