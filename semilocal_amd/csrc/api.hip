@@ -37,6 +37,7 @@ struct DeviceState {
     int clock_khz = 0;
     size_t hbm = 0;
     char name[256] = {0};
+    char arch[256] = {0};
 };
 static DeviceState g_dev;
 static std::mutex g_dev_mu;
@@ -65,7 +66,9 @@ int ensure_device(int device) {
     g_dev.cus = prop.multiProcessorCount;
     g_dev.clock_khz = prop.clockRate;
     g_dev.hbm = prop.totalGlobalMem;
-    std::snprintf(g_dev.name, sizeof g_dev.name, "%s", prop.name);
+    // the marketing name can be empty on some boxes; the ISA name is always set
+    std::snprintf(g_dev.arch, sizeof g_dev.arch, "%s", prop.gcnArchName);
+    std::snprintf(g_dev.name, sizeof g_dev.name, "%s", prop.name[0] ? prop.name : g_dev.arch);
     g_dev.ready = true;
     return kOk;
 }
@@ -186,8 +189,9 @@ static void remap_pairs_dense(std::vector<PairDesc> &pairs, PackedPool &pool) {
 // ---------------------------------------------------------------------------
 static const int kTileHeights[3] = {4, 8, 16};
 
-int model_rows_per_lane(const std::vector<PairDesc> &pairs, int64_t total_cells, int32_t m_max, int32_t n_max,
-                        int cus) {
+// `passes_at(R)`: the batch's total passes at tile height R.
+template <typename PassesAt>
+static int model_rows_core(PassesAt passes_at, int64_t total_cells, int32_t m_max, int32_t n_max, int cus) {
     const double rate[3] = {6.2e12, 7.2e12, 8.0e12};   // single-pass cells/s on 256 CUs
     const double scale = cus > 0 ? cus / 256.0 : 1.0;
     const double simds = 4.0 * (cus > 0 ? cus : 256);
@@ -195,9 +199,7 @@ int model_rows_per_lane(const std::vector<PairDesc> &pairs, int64_t total_cells,
     double best_t = 1e30;
     for (int k = 2; k >= 0; --k) {
         const int R = kTileHeights[k];
-        double total_passes = 0.0;
-        for (const auto &d : pairs)
-            if (d.m > 0 && d.n > 0) total_passes += std::ceil(d.m / (64.0 * R));
+        const double total_passes = passes_at(R);
         const double sharing = std::min(3.0, std::max(1.0, total_passes / simds));
         const double passes = std::ceil(m_max / (64.0 * R));
         const double steps = n_max + passes * 160.0;
@@ -211,6 +213,23 @@ int model_rows_per_lane(const std::vector<PairDesc> &pairs, int64_t total_cells,
         }
     }
     return best;
+}
+
+int model_rows_per_lane(const std::vector<PairDesc> &pairs, int64_t total_cells, int32_t m_max, int32_t n_max,
+                        int cus) {
+    auto passes_at = [&](int R) {
+        double total = 0.0;
+        for (const auto &d : pairs)
+            if (d.m > 0 && d.n > 0) total += std::ceil(d.m / (64.0 * R));
+        return total;
+    };
+    return model_rows_core(passes_at, total_cells, m_max, n_max, cus);
+}
+
+// The same model for one oriented pair (m <= n), with no allocation.
+int model_rows_one_pair(int32_t m, int32_t n, int cus) {
+    auto passes_at = [&](int R) { return std::ceil(m / (64.0 * R)); };
+    return model_rows_core(passes_at, (int64_t)m * n, m, n, cus);
 }
 
 // ---------------------------------------------------------------------------
@@ -239,7 +258,8 @@ struct Batch {
     int32_t *d_rc = nullptr;
     int32_t *d_cr = nullptr;
     uint64_t *d_buf = nullptr;         // tagged inter-pass strands, one entry per column of each pair
-    uint32_t epoch = 0;                // run counter; part of the strand tags
+    uint32_t epoch = 0;                // this run's tag (next_run_tag); part of the strand tags
+    uint64_t cleared_at = 0;           // run serial when d_buf was last cleared
     LaunchCtl *d_ctl = nullptr;    // [0] batched launch, [1+i] per-pair launch of pair i
     unsigned waves_cap = 0;
     int cus = 0;                       // CUs the queue is planned for; 0: the device's
@@ -513,6 +533,7 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     // stream, and the memory may hold another batch's tagged entries)
     if (e == hipSuccess)
         e = launch_fill_u32((uint32_t *)bt->d_buf, 0u, 2u * (uint64_t)std::max<int64_t>(bt->buf_total, 1), bt->stream);
+    bt->cleared_at = run_serial_now();
     if (e == hipSuccess) e = bt_alloc(bt, (void **)&bt->d_ctl, sizeof(LaunchCtl) * (size_t)(npairs + 1));
     // the control blocks hold the error bits batch_sync reports: cleared
     // here, so a sync before the first run reads no memory left by others
@@ -562,22 +583,30 @@ static int batch_check_ready(const Batch *bt) {
     return kOk;
 }
 
-// Run epochs come from one counter for the whole process, so no two runs of
-// any two batches tag their strand hand-offs alike (until the 16-bit counter
-// wraps, 65,535 runs later); memory a batch inherits from another can then
-// never hold an entry that matches.  Tag 0 is a cleared buffer's content.
-static std::atomic<uint32_t> g_epoch{0};
-uint32_t next_run_epoch() {
-    for (;;) {
-        const uint32_t v = (g_epoch.fetch_add(1u) + 1u) & 0xFFFFu;
-        if (v != 0) return v;
+// Run tags come from one serial for the whole process, so no two runs of any
+// two handles within 65,535 runs of each other tag their hand-offs alike;
+// memory a handle inherits from another is cleared at creation.  A handle
+// whose buffer was last cleared 65,535 or more runs ago (a resident batch in
+// a long-lived process) clears it again before its next run, so a stale entry
+// can never carry the tag a consumer waits for.  Tag 0 is a cleared entry.
+static std::atomic<uint64_t> g_run_serial{0};
+uint64_t run_serial_now() { return g_run_serial.load(); }
+bool next_run_tag(uint64_t &cleared_at, uint32_t *tag) {
+    const uint64_t s = g_run_serial.fetch_add(1u) + 1u;
+    *tag = (uint32_t)(s % 65535u) + 1u;
+    if (s - cleared_at >= 65535u) {
+        cleared_at = s;
+        return true;
     }
+    return false;
 }
 
 static int batch_reset(Batch *bt) {
     const int rc = batch_check_ready(bt);
     if (rc) return rc;
-    bt->epoch = next_run_epoch();   // new tags for this run's strand hand-offs
+    if (next_run_tag(bt->cleared_at, &bt->epoch))   // new tags for this run's strand hand-offs
+        SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_buf, 0u, (uint64_t)std::max<int64_t>(bt->buf_total, 1) * 2u,
+                                   bt->stream));
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
     return kOk;
 }
@@ -608,8 +637,9 @@ static int batch_run(Batch *bt) {
 static int batch_run_replayable(Batch *bt) {
     const int rc = batch_check_ready(bt);
     if (rc) return rc;
-    if (bt->epoch == 0) bt->epoch = next_run_epoch();
+    if (bt->epoch == 0) (void)next_run_tag(bt->cleared_at, &bt->epoch);
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_buf, 0u, (uint64_t)bt->buf_total * 2u, bt->stream));
+    bt->cleared_at = run_serial_now();
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
     SL_HIP_TRY(launch_items(bt, bt->d_items + bt->items.size(), (uint32_t)bt->order.size(), bt->d_ctl));
     return kOk;
@@ -687,7 +717,7 @@ int sl_device_info(char *buf, int len) {
     }
     int rc = ensure_device();
     if (rc) return rc;
-    std::snprintf(buf, (size_t)len, "%s;%d;%d;%zu", g_dev.name, g_dev.cus, g_dev.clock_khz, g_dev.hbm);
+    std::snprintf(buf, (size_t)len, "%s;%d;%d;%zu;%s", g_dev.name, g_dev.cus, g_dev.clock_khz, g_dev.hbm, g_dev.arch);
     return kOk;
 }
 

@@ -29,6 +29,7 @@ struct BitPlan {
     hipEvent_t ev[2] = {nullptr, nullptr};
     uint32_t spin_limit = 1u << 23;
     unsigned waves = 0;
+    uint64_t cleared_at = 0;       // run serial when d_buf was last cleared
 };
 
 static void bitplan_free(BitPlan *p) {
@@ -47,7 +48,9 @@ static int bitplan_run(BitPlan *p) {
         set_error("bit-parallel plan is missing a device buffer (internal defect); nothing launched");
         return kErrHip;
     }
-    p->d.epoch = next_run_epoch();   // unique tags for this run's hand-offs; tag 0 is a cleared buffer
+    // unique tags for this run's hand-offs; tag 0 is a cleared buffer
+    if (next_run_tag(p->cleared_at, &p->d.epoch))
+        SL_HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(p->d_buf), 0u, 2u * (uint64_t)p->d.N, p->stream));
     SL_HIP_TRY(launch_fill_u32(p->d_scratch, 0u, 4, p->stream));
     SL_HIP_TRY(launch_bitpar(p->d, p->waves, reinterpret_cast<LaunchCtl *>(p->d_scratch), p->spin_limit, p->stream));
     return kOk;
@@ -91,6 +94,7 @@ int sl_bitpar_create(sl_bitpar **out, const uint64_t *a_rev, int a_words, const 
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_buf, sizeof(uint64_t) * (size_t)N);
     // cleared on the handle's own stream, ordered before every launch
     if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t *>(p->d_buf), 0u, 2u * (uint64_t)N, p->stream);
+    p->cleared_at = run_serial_now();
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_scratch, sizeof(uint32_t) * 4);
     if (e == hipSuccess) e = launch_fill_u32(p->d_scratch, 0u, 4, p->stream);   // no bits left by others
     // the word uploads go through the null stream, which is not ordered with

@@ -21,7 +21,15 @@ void set_error(const std::string &msg);
 const char *last_error();
 int ensure_device(int device = -1);   // selects (and on first use initialises) the HIP device
 int device_cus();                     // compute units of the selected device
-uint32_t next_run_epoch();            // nonzero 16-bit run tag, unique across every handle of the process
+// Run tags of the tagged hand-off buffers (combing strands, bit-parallel and
+// Hyyro carries).  Every run of any handle draws the next value of one
+// process-wide 64-bit serial; its 16-bit tag is (serial mod 65535) + 1, never
+// 0 (a cleared entry).  Two serials share a tag only 65,535 or more runs
+// apart, so a handle keeps `cleared_at`, the serial when its buffer was last
+// cleared: every tag in the buffer then belongs to a later serial, and
+// next_run_tag() asks for a clear before the tag could repeat one of them.
+uint64_t run_serial_now();
+bool next_run_tag(uint64_t &cleared_at, uint32_t *tag);   // true: clear the buffer before this run
 
 #define SL_HIP_TRY(expr)                                                              \
     do {                                                                              \

@@ -462,6 +462,8 @@ struct Hyyro {
     std::vector<int64_t> item_off;
     int64_t *d_xoff = nullptr;          // per pair: its segments' exchange entries in d_xbuf
     uint64_t *d_xbuf = nullptr;
+    int64_t xbuf_entries = 0;
+    uint64_t cleared_at = 0;            // run serial when d_xbuf was last cleared
     LaunchCtl *d_ctl = nullptr;         // one per segmented class
     // a segment waits a few groups (well under a millisecond) for its
     // neighbour; 2^20 polls is over a second of margin and fails fast
@@ -584,7 +586,9 @@ int hy_launch(Hyyro *h) {
     // tickets and the error flags first
     SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_llcs, 0u, (uint64_t)std::max(h->npairs, 1), h->stream));
     SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_ctl, 0u, 2u * kSegClasses, h->stream));
-    const uint32_t epoch = next_run_epoch();
+    uint32_t epoch = 0;
+    if (next_run_tag(h->cleared_at, &epoch) && h->xbuf_entries > 0)
+        SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_xbuf, 0u, 2u * (uint64_t)h->xbuf_entries, h->stream));
     SL_HIP_TRY(hipEventRecord(h->fork, h->stream));
     const int cus = device_cus();
     // widest class first: it holds the longest pairs
@@ -765,6 +769,8 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     if (e == hipSuccess) e = hipMemcpy(h->d_xoff, xoff.data(), sizeof(int64_t) * xoff.size(), hipMemcpyHostToDevice);
     // exchange entries start with tag 0, which no run's (nonzero) epoch matches
     if (e == hipSuccess && xtotal > 0) e = launch_fill_u32((uint32_t *)h->d_xbuf, 0u, 2u * (uint64_t)xtotal, h->stream);
+    h->xbuf_entries = xtotal;
+    h->cleared_at = run_serial_now();
     if (e == hipSuccess && !rows.empty())
         e = hipMemcpy(h->d_rows, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !cols.empty()) e = hipMemcpy(h->d_cols, cols.data(), cols.size(), hipMemcpyHostToDevice);

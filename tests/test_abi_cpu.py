@@ -47,3 +47,23 @@ def test_product_path_has_no_oracle_import():
     for path in glob.glob(os.path.join(ROOT, "semilocal_amd", "**", "*.py"), recursive=True):
         src = open(path).read()
         assert "oracle" not in re.findall(r"^\s*(?:from|import)\s+(\w+)", src, flags=re.M), path
+
+
+def test_symbols_are_checked_not_cast():
+    # distinct symbols must never collapse to one by a silent narrowing cast
+    import numpy as np
+
+    from semilocal_amd._lib import _perm, as_symbols
+
+    assert as_symbols(np.array([1, 2], np.int64))[1] == 4
+    assert as_symbols(np.array([True, False]))[1] == 1
+    with pytest.raises(ValueError):
+        as_symbols(np.array([1, 2**32 + 1], np.int64))
+    with pytest.raises(ValueError):
+        as_symbols(np.array([1.2, 1.7]))
+    with pytest.raises(ValueError):
+        as_symbols(np.array([2**31], np.uint32))
+    with pytest.raises(ValueError):
+        _perm(np.array([0.0, 1.0]))
+    with pytest.raises(ValueError):
+        _perm(np.array([0, 2**40], np.int64))
