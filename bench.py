@@ -97,7 +97,8 @@ def bench_serving(S, W, count: int, rate_hz: float, pool_gb: float):
         if r.kind == "compare":
             ids[r.rid] = srv.submit_compare(r.a, r.b)
         else:
-            ids[r.rid] = srv.submit_query(ids[r.target], srv.STRING_SUBSTRING, *r.window)
+            kind = srv.SUBSTRING_STRING if r.qkind == "substring_string" else srv.STRING_SUBSTRING
+            ids[r.rid] = srv.submit_query(ids[r.target], kind, *r.window)
     lat = {"compare": [], "query": []}
     vals = {}
     for r in reqs:
@@ -118,15 +119,21 @@ def bench_serving(S, W, count: int, rate_hz: float, pool_gb: float):
     for r in qs:
         t = reqs[r.target]
         perm = S.semi_local_comb(t.a, t.b)
+        m, n = len(t.a), len(t.b)
         lo, hi = r.window
-        h = hi - lo - int(np.count_nonzero(perm[lo + len(t.a):] < hi))
+        if r.qkind == "string_substring":    # H(lo, hi)
+            h = hi - lo - int(np.count_nonzero(perm[lo + m:] < hi))
+        else:                                 # H(-lo, n+m-hi) - lo - (m-hi)
+            jj = n + m - hi
+            h = jj + lo - int(np.count_nonzero(perm[m - lo:] < jj)) - lo - (m - hi)
         assert vals[r.rid] == h, "serving query parity failure"
         checked += 1
     srv.close()
     allv = lat["compare"] + lat["query"]
     return {
         "workload": f"{count} requests, Poisson {rate_hz:g}/s, lengths log-normal (median 5000, sigma_ln 1.0) "
-                    f"clipped to [200, 200000], alphabet 4, 30% follow-up string-substring queries, seed 5",
+                    f"clipped to [200, 200000], alphabet 4, 30% follow-up queries (half substring-vs-string, half "
+                    f"string-vs-substring), seed 5",
         "rate_hz": rate_hz, "requests": count,
         "p50_ms": round(percentile(allv, 50), 3), "p99_ms": round(percentile(allv, 99), 3),
         "compare_p50_ms": round(percentile(lat["compare"], 50), 3),
@@ -137,6 +144,7 @@ def bench_serving(S, W, count: int, rate_hz: float, pool_gb: float):
         "offered_gcups": round(st["cells_done"] / reqs[-1].arrival_s / 1e9, 2),
         "wall_s": round(wall, 3), "max_submit_lag_ms": round(lag * 1e3, 3),
         "launches": st["launches"], "deferrals": st["deferrals"], "evictions": st["evictions"],
+        "peak_inflight": st["peak_inflight"], "peak_inflight_compares": st["peak_inflight_compares"],
         "pool_gb": round(st["pool_bytes"] / 2**30, 2), "pool_peak_mb": round(st["pool_peak"] / 2**20, 1),
         "parity_spot_checks": checked,
     }

@@ -48,7 +48,9 @@ const char *sl_last_error(void);
  * current device) lazily. */
 int sl_init(int device);
 
-/* Writes "name;compute_units;clock_khz;hbm_bytes" into buf. */
+/* Writes "name;compute_units;clock_khz;hbm_bytes;arch" into buf (name is the
+ * device's marketing name, or its ISA name where that is empty; arch is the
+ * ISA name with its features, e.g. "gfx950:sramecc+:xnack-"). */
 int sl_device_info(char *buf, int len);
 
 /* Diagnostics (no reference counterpart): one plain device-to-device copy of
@@ -296,6 +298,8 @@ typedef struct sl_server_stats_t {
     double busy_seconds;       /* first submission to last completion */
     int64_t indexed_braids;    /* index builds completed (sl_server_set_index) */
     int64_t index_queries;     /* queries answered from an index */
+    int64_t peak_inflight;     /* most micro-batches on the device at one time (launched, not yet complete) */
+    int64_t peak_inflight_compares;   /* the same, counting comparison micro-batches only */
 } sl_server_stats_t;
 
 /* pool_bytes 0: 90% of the free HBM; max_inflight 0: 4 concurrent

@@ -150,20 +150,31 @@ def _ptr(x: np.ndarray):
 
 
 def as_symbols(x) -> tuple[np.ndarray, int]:
-    """Contiguous symbol array and its element width (1, 2 or 4 bytes)."""
+    """Contiguous symbol array and its element width (1, 2 or 4 bytes).
+
+    Symbols must be integers (or booleans); wider integer types are accepted
+    when every value fits int32, so that no two distinct symbols collapse to
+    one.  Anything else raises ValueError."""
     arr = np.asarray(x)
+    if arr.dtype == np.bool_:
+        arr = arr.astype(np.uint8)
     if arr.dtype == np.uint8:
         return np.ascontiguousarray(arr), 1
     if arr.dtype == np.uint16:
         return np.ascontiguousarray(arr), 2
+    if arr.dtype.kind not in "iu":
+        raise ValueError(f"symbols must be an integer array, not {arr.dtype}")
+    if arr.dtype.itemsize > 4 or arr.dtype == np.uint32:
+        if arr.size and (int(arr.min()) < -(1 << 31) or int(arr.max()) >= (1 << 31)):
+            raise ValueError(f"symbol values outside the int32 range ({arr.dtype})")
     return np.ascontiguousarray(arr, dtype=np.int32), 4
 
 
 def device_info() -> dict:
     buf = C.create_string_buffer(512)
     check(lib().sl_device_info(buf, 512))
-    name, cus, clk, hbm = buf.value.decode().split(";")
-    return {"name": name, "compute_units": int(cus), "clock_khz": int(clk), "hbm_bytes": int(hbm)}
+    name, cus, clk, hbm, arch = buf.value.decode().split(";")
+    return {"name": name, "compute_units": int(cus), "clock_khz": int(clk), "hbm_bytes": int(hbm), "arch": arch}
 
 
 def semi_local_comb(a, b, with_col_to_row: bool = False):
@@ -553,7 +564,15 @@ def llcs_bitparallel(a_bits, b_bits) -> int:
 # Braid multiplication (kernel (c)), staggered glue and the hybrid scheduler.
 # ---------------------------------------------------------------------------
 def _perm(x) -> np.ndarray:
-    return np.ascontiguousarray(np.asarray(x), dtype=np.int32)
+    """int32 copy of a permutation (row -> col); integer input only, values
+    inside the int32 range (the engine rejects anything not a permutation)."""
+    arr = np.asarray(x)
+    if arr.dtype.kind not in "iu":
+        raise ValueError(f"a permutation must be an integer array, not {arr.dtype}")
+    if arr.dtype.itemsize > 4 or arr.dtype == np.uint32:
+        if arr.size and (int(arr.min()) < -(1 << 31) or int(arr.max()) >= (1 << 31)):
+            raise ValueError(f"permutation values outside the int32 range ({arr.dtype})")
+    return np.ascontiguousarray(arr, dtype=np.int32)
 
 
 def braid_multiply(p, q) -> np.ndarray:
@@ -688,7 +707,8 @@ class ServerStats(C.Structure):
                 ("launches", C.c_int64), ("deferrals", C.c_int64), ("evictions", C.c_int64),
                 ("pool_bytes", C.c_int64), ("pool_in_use", C.c_int64), ("pool_peak", C.c_int64),
                 ("resident_braids", C.c_int64), ("busy_seconds", C.c_double),
-                ("indexed_braids", C.c_int64), ("index_queries", C.c_int64)]
+                ("indexed_braids", C.c_int64), ("index_queries", C.c_int64),
+                ("peak_inflight", C.c_int64), ("peak_inflight_compares", C.c_int64)]
 
 
 class Server:

@@ -282,9 +282,22 @@ struct Server {
     // statistics
     int64_t n_compares = 0, n_queries = 0, cells_done = 0, deferrals = 0, evictions = 0, launches = 0;
     int64_t n_indexed = 0, n_index_queries = 0;
+    int64_t peak_inflight = 0, peak_inflight_compares = 0;   // micro-batches on the device at one time
     Clock::time_point t_first_submit, t_last_done;
     bool any_submit = false;
 };
+
+// Counts the micro-batches launched and not yet complete (the flights that
+// are busy) right after a launch, and keeps the peaks.  Called with the lock.
+static void note_inflight(Server *S) {
+    int64_t all = 0, compares = 0;
+    for (const auto &F : S->flights) {
+        all += F.busy;
+        compares += F.busy && !F.query;
+    }
+    S->peak_inflight = std::max(S->peak_inflight, all);
+    S->peak_inflight_compares = std::max(S->peak_inflight_compares, compares);
+}
 
 static void finish(Server *S, Req *r, int status, int64_t result, const std::string &err) {
     r->status = status;
@@ -582,6 +595,7 @@ static bool launch_compares(Server *S, Flight &F, std::unique_lock<std::mutex> &
         F.reqs = take;
         F.cells = cells;
         ++S->launches;
+        note_inflight(S);
         return true;
     }
 }
@@ -632,6 +646,7 @@ static bool launch_queries(Server *S, Flight &F, std::unique_lock<std::mutex> &l
     lk.lock();
     if (e != hipSuccess) S->fatal = std::string("request loop launch failed: ") + hipGetErrorString(e);
     ++S->launches;
+    note_inflight(S);
     return true;
 }
 
@@ -934,6 +949,8 @@ int sl_server_stats(sl_server *h, sl_server_stats_t *st) {
     st->resident_braids = resident;
     st->indexed_braids = S->n_indexed;
     st->index_queries = S->n_index_queries;
+    st->peak_inflight = S->peak_inflight;
+    st->peak_inflight_compares = S->peak_inflight_compares;
     st->busy_seconds = S->any_submit ? std::chrono::duration<double>(S->t_last_done - S->t_first_submit).count() : 0;
     return kOk;
 }

@@ -115,31 +115,51 @@ class Request:
     a: np.ndarray | None = None
     b: np.ndarray | None = None
     target: int = -1               # rid of the comparison a query reads
-    window: tuple = field(default_factory=tuple)   # (start, stop) of the substring of b
+    window: tuple = field(default_factory=tuple)   # (start, stop) of the substring
+    qkind: str = ""                # "substring_string": LCS(a[start:stop), b); "string_substring": LCS(a, b[start:stop))
 
 
 def serving_stream(seed: int = 5, count: int = 2000, rate_hz: float = 500.0, median: float = 5000,
                    sigma_ln: float = 1.0, lo: int = 200, hi: int = 200_000, sigma: int = 4,
-                   query_share: float = 0.3):
+                   query_share: float = 0.3, substring_string_share: float = 0.5, long_pairs=()):
     """Poisson arrivals at `rate_hz`; a `query_share` of requests are follow-up
-    string-substring LCS queries LCS(a, b[start:stop)) on an earlier comparison
-    (uniform window start and length)."""
+    LCS queries on an earlier comparison (uniform window start, then uniform
+    stop): a `substring_string_share` of them substring-vs-string,
+    LCS(a[start:stop), b), the rest string-vs-substring, LCS(a, b[start:stop)).
+    `long_pairs`: (request index, m, n) triples forcing those requests to be
+    comparisons of these lengths (e.g. the clip maximum)."""
     g = rng(seed)
     gaps = g.exponential(1.0 / rate_hz, size=count)
     t = np.cumsum(gaps)
+    forced = {int(i): (int(m), int(n)) for i, m, n in long_pairs}
     reqs: list[Request] = []
     compares: list[int] = []
     for i in range(count):
-        if compares and g.random() < query_share:
+        if i not in forced and compares and g.random() < query_share:
             tgt = compares[int(g.integers(0, len(compares)))]
-            n = len(reqs[tgt].b)
-            start = int(g.integers(0, n + 1))
-            stop = int(g.integers(start, n + 1))
-            reqs.append(Request(i, float(t[i]), "query", target=tgt, window=(start, stop)))
+            if g.random() < substring_string_share:
+                qkind, length = "substring_string", len(reqs[tgt].a)
+            else:
+                qkind, length = "string_substring", len(reqs[tgt].b)
+            start = int(g.integers(0, length + 1))
+            stop = int(g.integers(start, length + 1))
+            reqs.append(Request(i, float(t[i]), "query", target=tgt, window=(start, stop), qkind=qkind))
         else:
             la, lb = lognormal_lengths(g, 2, median, sigma_ln, lo, hi)
+            if i in forced:
+                la, lb = forced[i]
             a = uniform_string(g, int(la), sigma)
             b = uniform_string(g, int(lb), sigma)
             reqs.append(Request(i, float(t[i]), "compare", a=a, b=b))
             compares.append(i)
     return reqs
+
+
+def serving_parity_stream(count: int = 320, rate_hz: float = 2000.0):
+    """The request stream of the serving parity fixture: config 5's law
+    (Poisson arrivals, lengths log-normal median 5,000 clipped to [200,
+    200,000], alphabet 4, 30% follow-up queries of both kinds), seed 55, with
+    two comparisons at the clip maximum early in the stream: 200,000 x 200,000
+    (request 3) and 200,000 x 700 (request 9)."""
+    return serving_stream(seed=55, count=count, rate_hz=rate_hz,
+                          long_pairs=((3, 200_000, 200_000), (9, 200_000, 700)))

@@ -14,6 +14,12 @@ reference's outputs):
   bitpar.json        LLCS of binary pairs from llcs_2symbol_smart_combing
   configs.json       full-size workloads: seed, input SHA-256, SHA-256 of the
                      row->col permutation, LLCS, sampled entries
+  serving_stream.json  the serving parity stream (workloads.serving_parity_stream:
+                     config 5's law, 320 requests, both query kinds): per
+                     comparison the input SHA-256, the permutation's SHA-256
+                     and LLCS from sticky_braid_mpi; per follow-up query the
+                     LCS read from the reference's permutation by the semi-local
+                     H formulas (approximate_matching/utils.h:281-293, 331-342)
 --full also runs the long ones (the 2^20 binary pair and the 10^6 hybrid).
 """
 from __future__ import annotations
@@ -168,13 +174,51 @@ def configs(ref: Reference, full: bool):
         json.dump(out, f, indent=1)
 
 
+def H(perm, m, n, i, j):
+    """H(i, j) = j - i - #{r >= i + m : P[r] < j} (approximate_matching/utils.h:281-293)."""
+    return j - i - int(np.count_nonzero(perm[i + m:] < j))
+
+
+def serving(ref: Reference):
+    reqs = W.serving_parity_stream()
+    cmp = [r for r in reqs if r.kind == "compare"]
+    t = time.time()
+    perms = ref.comb_batch([(r.a.astype(np.int32), r.b.astype(np.int32)) for r in cmp],
+                           threads=os.cpu_count() or 1)
+    by_rid = {}
+    out_cmp = []
+    for r, p in zip(cmp, perms):
+        m, n = len(r.a), len(r.b)
+        by_rid[r.rid] = (p, m, n)
+        out_cmp.append({"rid": r.rid, "m": m, "n": n, "input_sha256": W.sha256(r.a, r.b),
+                        "perm_sha256": W.sha256(p), "llcs": int(m - np.count_nonzero(p[:m] >= n))})
+    out_q = []
+    for r in reqs:
+        if r.kind != "query":
+            continue
+        p, m, n = by_rid[r.target]
+        s_, e_ = r.window
+        if r.qkind == "string_substring":          # LCS(a, b[s:e)) = H(s, e)
+            v = H(p, m, n, s_, e_)
+        else:                                      # LCS(a[s:e), b) = H(-s, n+m-e) - s - (m-e)
+            v = H(p, m, n, -s_, n + m - e_) - s_ - (m - e_)
+        out_q.append({"rid": r.rid, "target": r.target, "qkind": r.qkind, "start": s_, "stop": e_, "lcs": int(v)})
+    rec = {"generator": "workloads.serving_parity_stream(count=320, rate_hz=2000.0)",
+           "reference": "semi_local::sticky_braid_mpi<int,false,false> per comparison; queries by the H formulas "
+                        "of approximate_matching/utils.h:281-342 on the reference's permutation",
+           "requests": len(reqs), "compares": out_cmp, "queries": out_q}
+    with open(os.path.join(HERE, "serving_stream.json"), "w") as f:
+        json.dump(rec, f, indent=0)
+    print("serving:", len(out_cmp), "comparisons,", len(out_q), "queries", time.time() - t)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true")
     ap.add_argument("--only", default="")
     args = ap.parse_args()
     ref = Reference()
-    steps = {"comb": comb_sweep, "steady": steady_ant, "hybrid": hybrid_small, "bitpar": bitpar}
+    steps = {"comb": comb_sweep, "steady": steady_ant, "hybrid": hybrid_small, "bitpar": bitpar, "serving": serving}
     for k, fn in steps.items():
         if not args.only or k in args.only.split(","):
             fn(ref)

@@ -45,3 +45,8 @@ def bitpar():
 def configs():
     with open(os.path.join(GOLDEN, "configs.json")) as f:
         return json.load(f)
+
+
+def serving_stream():
+    with open(os.path.join(GOLDEN, "serving_stream.json")) as f:
+        return json.load(f)

@@ -78,6 +78,7 @@ def test_results_in_the_mix_equal_results_alone(gpu, oracle, index_min):
     st = srv.stats()
     assert st["compares_done"] == len(pairs) and st["queries_done"] == len(queries)
     assert st["launches"] > len(pairs) // 12     # several micro-batches
+    assert st["peak_inflight_compares"] >= 2      # ... on the device at the same time
     if index_min > 0:
         on_big = sum(len(pairs[t][0]) + len(pairs[t][1]) >= index_min for _, t, _, _, _ in queries)
         assert st["indexed_braids"] == big and st["index_queries"] == on_big > 0
@@ -154,4 +155,47 @@ def test_release_and_argument_errors(gpu):
     perm = gpu.semi_local_comb(a, b)
     assert srv.result(nid)[0] == gpu.llcs_from_kernel(perm, len(a), len(b))
     assert srv.stats()["resident_braids"] == 0
+    srv.close()
+
+
+def test_config5_stream_against_the_reference_fixture(gpu):
+    """Config 5's own workload through the request loop: Poisson arrivals
+    (2,000/s, replayed in real time), lengths log-normal (median 5,000)
+    clipped to [200, 200,000] with two comparisons at the maximum, alphabet 4,
+    30% follow-up queries of both kinds.  Every comparison's braid and LLCS
+    and every query's answer equal the fixture written from the reference's
+    own combing (tests/golden/serving_stream.json), and comparison
+    micro-batches were on the device at the same time."""
+    import golden_io as G
+    from semilocal_amd import workloads as W
+
+    fx = G.serving_stream()
+    reqs = W.serving_parity_stream()
+    want_cmp = {c["rid"]: c for c in fx["compares"]}
+    want_q = {q["rid"]: q for q in fx["queries"]}
+    srv = gpu.Server(pool_bytes=4 << 30)
+    ids = {}
+    t0 = time.perf_counter()
+    for r in reqs:
+        delay = r.arrival_s - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        if r.kind == "compare":
+            assert want_cmp[r.rid]["input_sha256"] == W.sha256(r.a, r.b)
+            ids[r.rid] = srv.submit_compare(r.a, r.b)
+        else:
+            kind = srv.SUBSTRING_STRING if r.qkind == "substring_string" else srv.STRING_SUBSTRING
+            ids[r.rid] = srv.submit_query(ids[r.target], kind, *r.window)
+    for r in reqs:
+        val, lat = srv.result(ids[r.rid])
+        assert lat >= 0
+        if r.kind == "compare":
+            w = want_cmp[r.rid]
+            assert val == w["llcs"], r.rid
+            assert W.sha256(srv.braid(ids[r.rid], w["m"] + w["n"])) == w["perm_sha256"], r.rid
+        else:
+            assert val == want_q[r.rid]["lcs"], (r.rid, r.qkind, r.window)
+    st = srv.stats()
+    assert st["compares_done"] == len(want_cmp) and st["queries_done"] == len(want_q)
+    assert st["peak_inflight_compares"] >= 2, st     # micro-batches overlapped on the device
     srv.close()
