@@ -19,6 +19,25 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libref.so")
+REF_V4_SO = os.path.join(HERE, "_ref", "libref_v4.so")
+_V4_FLAGS = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+
+
+def host_has_avx512() -> bool:
+    """The x86-64-v4 feature set (AVX-512 F/BW/CD/DQ/VL) on this host."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return _V4_FLAGS <= set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return False
+
+
+def reference_so() -> str:
+    """The reference build for this host: x86-64-v4 where AVX-512 is there."""
+    return REF_V4_SO if os.path.exists(REF_V4_SO) and host_has_avx512() else REF_SO
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
@@ -162,9 +181,12 @@ def _batch_call(fn, pairs, threads):
 class Reference:
     """The reference's own OpenMP implementation, compiled in place."""
 
-    def __init__(self, path: str = REF_SO):
+    def __init__(self, path: str | None = None):
+        path = path or reference_so()
         if not os.path.exists(path):
             raise FileNotFoundError(path)
+        self.path = path
+        self.march = "x86-64-v4" if path.endswith("_v4.so") else "x86-64-v3"
         L = C.CDLL(path)
         self.L = L
         L.ref_sticky_braid_mpi.argtypes = [_i32p, C.c_int, _i32p, C.c_int, _i32p, C.c_int]
@@ -176,6 +198,7 @@ class Reference:
         L.ref_llcs_2symbol_smart.restype = C.c_int
         L.ref_hyyro.argtypes = [_i32p, C.c_int, _i32p, C.c_int]
         L.ref_hyyro.restype = C.c_int
+        L.ref_precalc_warmup.argtypes = []
 
     @staticmethod
     def available(path: str = REF_SO) -> bool:
@@ -272,6 +295,7 @@ class RefAMatch:
         self.L = C.CDLL(path)
         self.L.ref_dup_search.argtypes = [C.c_int, _i32p, C.c_int, _i32p, C.c_int, C.c_double, _i32p, _i32p, _f64p,
                                           C.c_int]
+        self.L.ref_semi_local_H.argtypes = [_i32p, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _i64p]
 
     @staticmethod
     def available(path: str = REF_AMATCH_SO) -> bool:
@@ -279,3 +303,10 @@ class RefAMatch:
 
     def dup_search(self, p, t, k, variant):
         return _dup_call(self.L.ref_dup_search, variant, p, t, k)
+
+    def semi_local_H(self, perm, m, qi, qj):
+        """H(i, j) for every (qi[k], qj[k]) by the reference's O(m+n) dominance scan."""
+        perm, qi, qj = _i32(perm), _i32(qi), _i32(qj)
+        out = np.empty(len(qi), np.int64)
+        self.L.ref_semi_local_H(perm, len(perm), m, len(qi), qi, qj, out)
+        return out

@@ -28,3 +28,20 @@ extern "C" int ref_dup_search(int variant, const int32_t *p, int m, const int32_
     }
     return (int)res.size();
 }
+
+// The reference's own semi-local query on a braid: H(i, j) = j - i -
+// dominanceSumRandom(i, j), the O(m+n) scan of
+// SemiLocalStringSubstringWrapper (approximate_matching/utils.h:268-293,
+// 331-334) with v = 1 (no blow-up), for nq (i, j) pairs on one permutation
+// row_to_col[size] of a pair with m rows.  Used to time the reference's
+// follow-up query beside the request loop's.
+extern "C" void ref_semi_local_H(const int32_t *row_to_col, int size, int m, int nq, const int32_t *qi,
+                                 const int32_t *qj, int64_t *out) {
+    Permutation perm(size, size);
+    for (int r = 0; r < size; ++r) perm.set_point(r, row_to_col[r]);
+    approximate_matching::utils::SemiLocalStringSubstringWrapper w(&perm, nullptr, m, 1);
+    for (int k = 0; k < nq; ++k) {
+        const int range = w.dominanceSumRandom(qi[k], qj[k]);
+        out[k] = (int64_t)qj[k] - qi[k] - range;
+    }
+}
