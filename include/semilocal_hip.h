@@ -71,6 +71,26 @@ int sl_probe_copy(int64_t bytes, double *ms);
 int sl_semi_local_comb(const void *a, int m, const void *b, int n, int sym_bytes,
                        int32_t *row_to_col, int32_t *col_to_row);
 
+/* Allocation-free variant of sl_semi_local_comb.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Inputs and outputs are DEVICE pointers owned by the caller: d_a (m symbols),
+ * d_b (n symbols), d_row_to_col (m+n entries) and optionally d_col_to_row
+ * (m+n entries, or NULL).  The work is enqueued on `stream` (a hipStream_t;
+ * NULL = the null stream) and the call waits for it, then reports a
+ * kernel-side timeout or defect like sl_semi_local_comb.  No hipMalloc, no
+ * hipFree, no allocating library routine: the per-call state (launch control
+ * block, pair descriptor, pass queue and inter-pass strand buffer) lives in a
+ * fixed device array of about 8.4 MB that each GPU holds until the process
+ * ends, rewritten at the start of every call.  Calls on one device share that
+ * array and are serialised by a host lock.  Limits: max(m, n) <=
+ * SL_NOALLOC_MAX_LEN (longer strings: SL_ERR_ARG), sym_bytes 1 or 2 (symbols
+ * are compared as given, with no remapping).  Results equal
+ * sl_semi_local_comb's on the same strings. */
+#define SL_NOALLOC_MAX_LEN 1048576
+int sl_semi_local_comb_noalloc(const void *d_a, int m, const void *d_b, int n, int sym_bytes,
+                               int32_t *d_row_to_col, int32_t *d_col_to_row, void *stream);
+
 /* LLCS read from a kernel: replaces lcs::LCSBySemiLocal::compute's count,
  * include/LCS/LCS.h:223-233.  Pure host arithmetic over row_to_col. */
 int sl_llcs_from_kernel(const int32_t *row_to_col, int m, int n);

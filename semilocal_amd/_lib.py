@@ -46,6 +46,7 @@ SIGNATURES = {
     "sl_device_info": ([C.c_char_p, C.c_int], C.c_int),
     "sl_probe_copy": ([C.c_int64, C.POINTER(C.c_double)], C.c_int),
     "sl_semi_local_comb": ([_vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "sl_semi_local_comb_noalloc": ([_vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _vp], C.c_int),
     "sl_llcs_from_kernel": ([_i32p, C.c_int, C.c_int], C.c_int),
     "sl_batch_create": ([C.POINTER(_vp), C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, C.c_int, C.c_int],
                         C.c_int),
