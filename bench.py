@@ -26,25 +26,83 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS batched vs one-pair-per-launch, 1 MI355X, bit-exact; serving p50/p99"
 VALU_OPS_PER_CELL = 5          # 2 compares + 1 or + 2 selects: the algorithmic work of one cell
+BITPAR_OPS_PER_BIT_CELL = 0.93 # bit-parallel combing: 127 steps x 15 64-bit ops per 64 x 64 word-cell, as int32
+HYYRO_OPS_PER_WORD_ROW = 8     # Hyyro's row: 64-bit AND, ADD, XOR, OR = 8 int32 lane-ops per 64 cells
 LANES_PER_CU_CLK = 64          # int32 VALU lane-ops per CU per clock on CDNA4
 PEAK_CLOCK_GHZ = 2.4           # MI355X peak engine clock
+CUS = 256
+
+
+def peak_tops(cus: int = CUS) -> float:
+    return cus * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12
+
+
+def valu_roofline(ops: float, ms: float, what: str, cus: int = CUS) -> dict:
+    """Roofline object of a VALU-bound kernel: algorithmic int32 lane-ops
+    per second against the int32 VALU peak."""
+    achieved = ops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak_tops(cus), 2),
+            "unit": f"T lane-ops/s (algorithmic: {what})", "frac": round(achieved / peak_tops(cus), 4),
+            "traffic": None}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pairs, target_s: float = 12.0):
-    """Reference OpenMP path (oracle/_ref/libref.so) on a bounded sample of the
-    same pairs; falls back to the oracle's restatement when the reference
-    build is absent.  Returns a dict for the JSON line."""
+# The reference's OpenMP runtime reads its placement once, when libgomp
+# loads: threads bound to cores, packed (the CPU baseline's settings).
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def host_info() -> dict:
+    """The host the CPU baselines run on: CPU model, logical CPUs, the CPUs
+    this process may use, and the thread count used for "all cores" (the
+    OMP_NUM_THREADS share where set, else the usable CPUs)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or usable)
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable, "threads_all": max(1, threads),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES")}
+
+
+def _median_time(fn, runs: int, warmup: int = 1) -> float:
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def _reference():
     from oracle.binding import Oracle, Reference
 
-    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
     if Reference.available():
-        eng, kind = Reference(), "reference"
-    else:
-        eng, kind = Oracle(), "port"
+        r = Reference()
+        return r, "reference", r.march
+    return Oracle(), "port", "x86-64-v3"
+
+
+def cpu_baseline(pairs, target_s: float = 12.0, target_1t_s: float = 4.0):
+    """Reference OpenMP path (oracle/_ref/libref*.so) on a bounded sample of the
+    same pairs, on all cores and on one thread; falls back to the oracle's
+    restatement when the reference build is absent.  Returns a dict for the
+    JSON line."""
+    host = host_info()
+    threads = host["threads_all"]
+    eng, kind, march = _reference()
     order = np.random.default_rng(99).permutation(len(pairs))
 
     def sample(cells_target):
@@ -56,23 +114,128 @@ def cpu_baseline(pairs, target_s: float = 12.0):
                 break
         return sel, tot
 
-    sel, cells = sample(5e8)
-    sub = [(pairs[i][0].astype(np.int32), pairs[i][1].astype(np.int32)) for i in sel]
-    t = time.perf_counter()
-    eng.comb_batch(sub, threads=threads)
-    dt = time.perf_counter() - t
-    rate = cells / dt
-    want = min(rate * target_s, sum(len(a) * len(b) for a, b in pairs))
-    if want > 2 * cells:
-        sel, cells = sample(want)
+    def timed(th, tgt):
+        sel, cells = sample(5e8)
         sub = [(pairs[i][0].astype(np.int32), pairs[i][1].astype(np.int32)) for i in sel]
         t = time.perf_counter()
-        eng.comb_batch(sub, threads=threads)
+        eng.comb_batch(sub, threads=th)
         dt = time.perf_counter() - t
+        want = min(cells / dt * tgt, sum(len(a) * len(b) for a, b in pairs))
+        if want > 2 * cells:
+            sel, cells = sample(want)
+            sub = [(pairs[i][0].astype(np.int32), pairs[i][1].astype(np.int32)) for i in sel]
+            t = time.perf_counter()
+            eng.comb_batch(sub, threads=th)
+            dt = time.perf_counter() - t
+        return sel, cells, dt
+
+    sel, cells, dt = timed(threads, target_s)
+    sel1, cells1, dt1 = timed(1, target_1t_s)
     return {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": kind,
             "sample": f"{len(sel)} of the {len(pairs)} pairs (seeded random order), {cells:.3e} cells, "
                       f"{dt:.1f} s; sticky_braid_mpi per pair in an OpenMP parallel-for over pairs",
-            "seconds": dt}
+            "seconds": dt,
+            "one_thread": {"value": round(cells1 / dt1 / 1e9, 3), "unit": "GCUPS", "cores": 1,
+                           "sample": f"{len(sel1)} pairs, {cells1:.3e} cells, {dt1:.1f} s"},
+            "build": f"g++ -O3 -march={march} -fopenmp", "host": host}
+
+
+def cpu_baselines_configs(golden):
+    """The reference's OpenMP paths for configs 1, 2, 3 and 5 on bounded
+    samples of the same seeded inputs, on all cores and on one thread (one
+    warm-up, then the median of several runs; one run for the long
+    one-thread samples).  Bit-exactness of these functions against the
+    engine is covered by the fixtures; here they are timed only."""
+    from oracle.binding import RefAMatch
+    from semilocal_amd import workloads as W
+
+    host = host_info()
+    T = host["threads_all"]
+    eng, kind, march = _reference()
+    meta = {"kind": kind, "build": f"g++ -O3 -march={march} -fopenmp", "threads_all": T}
+    out = {}
+
+    a, b = W.plain_pair()
+    a, b = a.astype(np.int32), b.astype(np.int32)
+    cells = len(a) * len(b)
+    tA = _median_time(lambda: eng.comb(a, b, threads=T), 5)
+    t1 = _median_time(lambda: eng.comb(a, b, threads=1), 5)
+    out["plain_pair_10k"] = {"unit": "GCUPS", "all_cores": round(cells / tA / 1e9, 3), "one_thread":
+                             round(cells / t1 / 1e9, 3), "function": "sticky_braid_mpi<int,false,true>",
+                             "sample": "the whole 10,000 x 10,000 pair", **meta}
+
+    a, b = W.binary_pair()
+    if kind == "reference":
+        sa = a[:1 << 16]
+        bc = len(sa) * len(b)
+        tA = _median_time(lambda: eng.llcs_binary(sa, b, threads=T), 3)
+        t1 = _median_time(lambda: eng.llcs_binary(sa, b, threads=1), 1, warmup=0)
+        out["binary_pair_2p20_score"] = {
+            "unit": "G bit-cells/s", "all_cores": round(bc / tA / 1e9, 1), "one_thread": round(bc / t1 / 1e9, 1),
+            "function": "binary::llcs_2symbol_smart_combing<uint64_t>",
+            "sample": "a[:65536] x b (all 2^20 bits): 2^36 bit-cells", **meta}
+    pa, pb = a[:1 << 13].astype(np.int32), b.astype(np.int32)
+    pc = len(pa) * len(pb)
+    tA = _median_time(lambda: eng.comb(pa, pb, threads=T), 3, warmup=0)
+    t1 = _median_time(lambda: eng.comb(pa, pb, threads=1), 1, warmup=0)
+    out["binary_pair_2p20_permutation"] = {
+        "unit": "GCUPS", "all_cores": round(pc / tA / 1e9, 3), "one_thread": round(pc / t1 / 1e9, 3),
+        "function": "sticky_braid_mpi<int,false,true>", "sample": "a[:8192] x b (2^20): 8.6e9 cells", **meta}
+
+    a, b = W.similar_pair()
+    ha, hb = a[:100_000].astype(np.int32), b[:100_000].astype(np.int32)
+    hc = len(ha) * len(hb)
+    if kind == "reference":
+        t0 = time.perf_counter()
+        eng.L.ref_precalc_warmup()
+        pre = time.perf_counter() - t0
+    else:
+        pre = 0.0
+    tA = _median_time(lambda: eng.hybrid_iterative(ha, hb, 8, 8, threads=T), 3, warmup=0)
+    t1 = _median_time(lambda: eng.hybrid_iterative(ha, hb, 8, 8, threads=1), 1, warmup=0)
+    out["similar_pair_1e6_hybrid"] = {
+        "unit": "GCUPS", "all_cores": round(hc / tA / 1e9, 3), "one_thread": round(hc / t1 / 1e9, 3),
+        "function": "hybrid_iterative<int,false>, 8x8 blocks", "precalc_map5_s": round(pre, 3),
+        "sample": "a[:100000] x b[:100000] of the similar pair: 1e10 cells (precalc timed separately)", **meta}
+
+    reqs = W.serving_stream(count=2000, rate_hz=1000.0)
+    sel, tot = [], 0
+    for r in reqs:
+        if r.kind == "compare":
+            sel.append(r)
+            tot += len(r.a) * len(r.b)
+            if tot >= 4e9:
+                break
+    sub = [(r.a.astype(np.int32), r.b.astype(np.int32)) for r in sel]
+    t0 = time.perf_counter()
+    perms = eng.comb_batch(sub, threads=T)
+    tA = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    eng.comb_batch(sub[:max(1, len(sub) // 4)], threads=1)
+    t1 = time.perf_counter() - t0
+    c1 = sum(len(x) * len(y) for x, y in sub[:max(1, len(sub) // 4)])
+    rec = {"unit": "GCUPS", "all_cores": round(tot / tA / 1e9, 3), "one_thread": round(c1 / t1 / 1e9, 3),
+           "function": "sticky_braid_mpi<int,false,false> per request, OpenMP parallel-for over requests",
+           "sample": f"the first {len(sel)} comparisons of the serving stream ({tot:.2e} cells; one thread: "
+                     f"the first {max(1, len(sub) // 4)})", **meta}
+    if RefAMatch.available():
+        ra = RefAMatch()
+        pos = {r.rid: k for k, r in enumerate(sel)}
+        qs = [r for r in reqs if r.kind == "query" and r.target in pos]
+        nq, dq = 0, 0.0
+        for r in qs:
+            k = pos[r.target]
+            m, n = len(sel[k].a), len(sel[k].b)
+            lo, hi = r.window
+            i, j = (lo, hi) if r.qkind == "string_substring" else (-lo, n + m - hi)
+            dq += ra.semi_local_H(perms[k], m, np.array([i]), np.array([j]))[1]
+            nq += 1
+        rec["queries_per_s_one_thread"] = round(nq / dq, 1) if nq else None
+        rec["query_function"] = "SemiLocalStringSubstringWrapper::dominanceSumRandom (O(m+n) scan)"
+        rec["queries_timed"] = nq
+    out["serving"] = rec
+    out["host"] = host
+    return out
 
 
 def percentile(xs, q):
@@ -148,6 +311,25 @@ def bench_serving(S, W, count: int, rate_hz: float, pool_gb: float):
         "pool_gb": round(st["pool_bytes"] / 2**30, 2), "pool_peak_mb": round(st["pool_peak"] / 2**20, 1),
         "parity_spot_checks": checked,
     }
+
+
+def bench_serving_sweep(S, W, rates=(1000, 4000, 16000, 32000, 64000), slo_ms: float = 20.0,
+                        pool_gb: float = 16.0):
+    """Config 5 at several offered rates: each rate replays its own Poisson
+    stream (about 1.5 s of arrivals, 2,000 to 16,000 requests) in real time;
+    p50/p99 latency and served GCUPS per rate, and the highest rate whose p99
+    stays within `slo_ms`."""
+    rows = []
+    for rate in rates:
+        count = int(min(16000, max(2000, rate * 1.5)))
+        r = bench_serving(S, W, count, float(rate), pool_gb)
+        rows.append({k: r[k] for k in ("rate_hz", "requests", "p50_ms", "p99_ms", "compare_p99_ms", "query_p99_ms",
+                                       "gcups", "offered_gcups", "max_submit_lag_ms", "launches", "deferrals",
+                                       "peak_inflight_compares")})
+        log(f"serving at {rate}/s: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, {r['gcups']} GCUPS served")
+    ok = [row["rate_hz"] for row in rows if row["p99_ms"] is not None and row["p99_ms"] <= slo_ms]
+    return {"workload": "the serving stream (seed 5) at each offered rate, replayed in real time",
+            "slo_p99_ms": slo_ms, "max_rate_within_slo_hz": max(ok) if ok else None, "rates": rows}
 
 
 def bench_index(S, W):
@@ -320,6 +502,7 @@ def bench_hyyro(S, W, pairs=None):
             "batch_ms": round(ms, 3), "batch_cells": cells, "batch_gcups": round(cells / (ms * 1e-3) / 1e9, 1),
             "batch_word_rows": st["word_rows"],
             "batch_word_rows_per_s": round(st["word_rows"] / (ms * 1e-3), 1),
+            "batch_roofline": valu_roofline(st["word_rows"] * HYYRO_OPS_PER_WORD_ROW, ms, "8 per 64-bit word-row"),
             "batch_llcs_sum": int(scores.sum()),
             "long_pair_ms": round(one_ms, 2), "long_pair_gcups": round(2e10 / (one_ms * 1e-3) / 1e9, 1),
             "long_pair_word_rows": one_st["word_rows"],
@@ -370,6 +553,7 @@ def bench_hybrid(S, W, golden):
     return {"workload": "similar pair: a 1e6 symbols (alphabet 4), b = a with 5% subs and 1% indels (seed 3); "
                         "8x8 blocks", "ms": round(tot, 3), "comb_ms": round(comb, 3), "merge_ms": round(merge, 3),
             "graph_ms": round(gtot, 3), "merge_levels": levels, "gcups": round(cells / (tot * 1e-3) / 1e9, 2),
+            "comb_roofline": valu_roofline(cells * VALU_OPS_PER_CELL, comb, "5 per cell"),
             "graph_gcups": round(cells / (gtot * 1e-3) / 1e9, 2),
             "comb_tiles": {"rows_per_lane": tiles[0], "waves_per_cu": tiles[1], "autotune_s": round(tune_s, 1)},
             "parity": "bit-exact vs the reference hybrid's fixture (eager and graph)" if ok else "MISMATCH"}
@@ -409,6 +593,8 @@ def bench_binary(S, W, golden):
             "bitparallel_ms": round(ms, 3), "bitparallel_gcups": round(cells / (ms * 1e-3) / 1e9, 1),
             "permutation_ms": round(kms, 2), "permutation_gcups": round(cells / (kms * 1e-3) / 1e9, 1),
             "permutation_ms_incl_transfers": round(pms, 1),
+            "bitparallel_roofline": valu_roofline(cells * BITPAR_OPS_PER_BIT_CELL, ms, "0.93 per bit-cell"),
+            "permutation_roofline": valu_roofline(cells * VALU_OPS_PER_CELL, kms, "5 per cell"),
             "parity": "bit-exact" if (llcs == rec["llcs_bitparallel"] and W.sha256(perm) == rec["perm_sha256"])
             else "MISMATCH"}
 
@@ -421,7 +607,8 @@ def bench_plain(S, W, golden):
     ms, kms = bt.time(warmup=2, iters=10)
     bt.close()
     return {"workload": "plain pair 10000 x 10000, alphabet 4 (seed 1)", "ms": round(ms, 4),
-            "gcups": round(len(a) * len(b) / (ms * 1e-3) / 1e9, 1),
+            "gcups": round(len(a) * len(b) / (ms * 1e-3) / 1e9, 1), "kernel_ms": round(kms, 4),
+            "roofline": valu_roofline(len(a) * len(b) * VALU_OPS_PER_CELL, kms, "5 per cell"),
             "parity": "bit-exact" if W.sha256(res) == golden["plain_pair_10k"]["perm_sha256"] else "MISMATCH"}
 
 
@@ -454,11 +641,12 @@ def main():
                     help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
     ap.add_argument("--no-extras", action="store_true", help="only the ragged batch (configs 1, 2, 3, 5 skipped)")
     ap.add_argument("--only", choices=["plain", "binary", "hybrid", "multiply", "index", "serving", "amatch",
-                                       "hyyro"],
+                                       "hyyro", "serving_sweep"],
                     help="run one extra section alone (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--serve-pool-gb", type=float, default=16.0)
+    ap.add_argument("--serve-slo-ms", type=float, default=20.0, help="p99 latency SLO of the serving rate sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -481,7 +669,9 @@ def main():
               "amatch": lambda: bench_amatch(semilocal_amd),
               "hyyro": lambda: bench_hyyro(semilocal_amd, W),
               "serving": lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
-                                               args.serve_pool_gb)}[args.only]
+                                               args.serve_pool_gb),
+              "serving_sweep": lambda: bench_serving_sweep(semilocal_amd, W, slo_ms=args.serve_slo_ms,
+                                                           pool_gb=args.serve_pool_gb)}[args.only]
         print(json.dumps({"section": args.only, **fn()}), flush=True)
         if args.copy_probe_mb:
             copy_probe(args.copy_probe_mb)
@@ -540,12 +730,21 @@ def main():
     peak_tops = dev["compute_units"] * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12
     # HBM bytes per launch of this kernel, measured by two rocprofv3 --pmc
     # passes and scaled by a copy kernel of known size (profiles/)
-    traffic = None
+    # and the executed VALU instructions and VALU busy of the same kernel at
+    # this tile shape from a third pass (SQ_INSTS_VALU, GRBM_GUI_ACTIVE)
+    traffic = executed = None
     tf = os.path.join(ROOT, "profiles", "comb_batch_kernel_ragged4096.traffic.json")
     if os.path.exists(tf):
         rec = json.load(open(tf)).get("kernels", {}).get(f"comb_batch_kernel<uint8_t,{tiles[0]}>")
         if rec:
             traffic = rec["traffic_bytes_per_launch"]
+            if rec.get("sq_insts_valu_per_dispatch"):
+                ex_tops = rec["sq_insts_valu_per_dispatch"] * 64 / (kms * 1e-3) / 1e12
+                executed = {"lane_ops_per_cell": round(rec["sq_insts_valu_per_dispatch"] * 64 / cells, 3),
+                            "achieved": round(ex_tops, 3), "frac": round(ex_tops / peak_tops, 4),
+                            "valu_busy": rec.get("valu_busy"),
+                            "source": "rocprofv3 --pmc SQ_INSTS_VALU (profiles/comb_batch_kernel_ragged4096.md) "
+                                      "over this run's kernel time"}
     achieved_tops = cells * VALU_OPS_PER_CELL / (kms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -569,10 +768,12 @@ def main():
         "batched_vs_per_pair": round(gcups / gcups_pp, 2) if args.per_pair_steps > 0 else None,
         "parity": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 2),
-                     "unit": "Tops/s (int32 VALU lane-ops)", "frac": round(achieved_tops / peak_tops, 4),
+                     "unit": "T lane-ops/s (algorithmic: 5 int32 lane-ops per cell, the generic branchless "
+                             "cell; the gfx950 cell executes 3)",
+                     "frac": round(achieved_tops / peak_tops, 4),
                      "traffic": traffic,
                      "kernel": f"comb_batch_kernel<uint8_t,{tiles[0]}>", "kernel_ms": round(kms, 4),
-                     "ops_per_cell": VALU_OPS_PER_CELL},
+                     "ops_per_cell": VALU_OPS_PER_CELL, "executed": executed},
     }
     batch.close()
     if not args.no_extras:
@@ -585,7 +786,9 @@ def main():
                         ("approximate_matching_1e6", lambda: bench_amatch(semilocal_amd)),
                         ("hyyro_llcs", lambda: bench_hyyro(semilocal_amd, W, pairs)),
                         ("serving", lambda: bench_serving(semilocal_amd, W, args.serve_requests, args.serve_rate,
-                                                          args.serve_pool_gb))):
+                                                          args.serve_pool_gb)),
+                        ("serving_sweep", lambda: bench_serving_sweep(semilocal_amd, W, slo_ms=args.serve_slo_ms,
+                                                                      pool_gb=args.serve_pool_gb))):
             t0 = time.perf_counter()
             out[key] = fn()
             log(f"{key}: {out[key]}  ({time.perf_counter() - t0:.1f}s)")
@@ -599,6 +802,17 @@ def main():
         if "approximate_matching_1e6" in out:
             out["approximate_matching_1e6"]["cpu_reference"] = cpu_baseline_amatch()
             log("approximate matching cpu reference:", out["approximate_matching_1e6"]["cpu_reference"])
+        if not args.no_extras:
+            t0 = time.perf_counter()
+            cfg = cpu_baselines_configs(golden)
+            for key in ("plain_pair_10k", "similar_pair_1e6_hybrid", "serving"):
+                if key in out and key in cfg:
+                    out[key]["cpu_baseline"] = cfg[key]
+            if "binary_pair_2p20" in out:
+                out["binary_pair_2p20"]["cpu_baseline"] = {k: cfg[k] for k in
+                                                           ("binary_pair_2p20_score", "binary_pair_2p20_permutation")
+                                                           if k in cfg}
+            log(f"config cpu baselines: {cfg}  ({time.perf_counter() - t0:.1f}s)")
     print(json.dumps(out), flush=True)
 
 

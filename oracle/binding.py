@@ -296,6 +296,7 @@ class RefAMatch:
         self.L.ref_dup_search.argtypes = [C.c_int, _i32p, C.c_int, _i32p, C.c_int, C.c_double, _i32p, _i32p, _f64p,
                                           C.c_int]
         self.L.ref_semi_local_H.argtypes = [_i32p, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _i64p]
+        self.L.ref_semi_local_H.restype = C.c_double
 
     @staticmethod
     def available(path: str = REF_AMATCH_SO) -> bool:
@@ -305,8 +306,9 @@ class RefAMatch:
         return _dup_call(self.L.ref_dup_search, variant, p, t, k)
 
     def semi_local_H(self, perm, m, qi, qj):
-        """H(i, j) for every (qi[k], qj[k]) by the reference's O(m+n) dominance scan."""
+        """(H(i, j) for every (qi[k], qj[k]) by the reference's O(m+n)
+        dominance scan, seconds spent in the queries alone)."""
         perm, qi, qj = _i32(perm), _i32(qi), _i32(qj)
         out = np.empty(len(qi), np.int64)
-        self.L.ref_semi_local_H(perm, len(perm), m, len(qi), qi, qj, out)
-        return out
+        sec = self.L.ref_semi_local_H(perm, len(perm), m, len(qi), qi, qj, out)
+        return out, sec

@@ -3,6 +3,7 @@
 // from /root/reference by oracle/Makefile into oracle/_ref/libref_amatch.so,
 // behind a C entry point.  Built without fused multiply-add contraction, as
 // the reference's default x86-64 build computes its thresholds.
+#include <chrono>
 #include <cstdint>
 #include <iostream>
 #include <vector>
@@ -34,14 +35,18 @@ extern "C" int ref_dup_search(int variant, const int32_t *p, int m, const int32_
 // SemiLocalStringSubstringWrapper (approximate_matching/utils.h:268-293,
 // 331-334) with v = 1 (no blow-up), for nq (i, j) pairs on one permutation
 // row_to_col[size] of a pair with m rows.  Used to time the reference's
-// follow-up query beside the request loop's.
-extern "C" void ref_semi_local_H(const int32_t *row_to_col, int size, int m, int nq, const int32_t *qi,
-                                 const int32_t *qj, int64_t *out) {
+// follow-up query beside the request loop's.  Returns the seconds spent in
+// the queries alone (the braid is held as a Permutation beforehand, as a
+// caller of the reference would hold it).
+extern "C" double ref_semi_local_H(const int32_t *row_to_col, int size, int m, int nq, const int32_t *qi,
+                                   const int32_t *qj, int64_t *out) {
     Permutation perm(size, size);
     for (int r = 0; r < size; ++r) perm.set_point(r, row_to_col[r]);
     approximate_matching::utils::SemiLocalStringSubstringWrapper w(&perm, nullptr, m, 1);
+    const auto t0 = std::chrono::steady_clock::now();
     for (int k = 0; k < nq; ++k) {
         const int range = w.dominanceSumRandom(qi[k], qj[k]);
         out[k] = (int64_t)qj[k] - qi[k] - range;
     }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
