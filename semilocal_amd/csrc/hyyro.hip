@@ -308,13 +308,19 @@ __global__ __launch_bounds__(64 * kSegWaves) void hy_seg(const HyPair *__restric
     static_assert(64 % D == 0, "the ring must divide a group of 64 steps");
     constexpr int NW = kSegWaves;
     __shared__ uint64_t hand[3][NW];
-    __shared__ int zsum[NW];
     __shared__ uint32_t ticket;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // The ticket is read into a scalar register, so the compiler sees a
+    // uniform loop exit, and thread 0's ticket add is the loop's only
+    // thread-0 region: with a second one at the tail (a block-wide score
+    // add), the compiler merged the two into an outer loop and left the
+    // barriers and the segment work in an inner loop that only thread 0
+    // exits at the end of an item -- the other threads then re-read the
+    // unchanged ticket and redo the same segment without end.
     for (;;) {
         if (threadIdx.x == 0) ticket = atomicAdd(&ctl->ticket, 1u);
         __syncthreads();
-        const uint32_t tk = ticket;
+        const uint32_t tk = __builtin_amdgcn_readfirstlane(ticket);
         __syncthreads();   // every wave has read the ticket before it is replaced
         if (tk >= nitems) break;
         const SegItem it = items[tk];
@@ -433,13 +439,7 @@ __global__ __launch_bounds__(64 * kSegWaves) void hy_seg(const HyPair *__restric
         for (int r = 0; r < R; ++r) z += w0 + r < K ? 64 - __popcll(V[r]) : 0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
-        if (lane == 0) zsum[wave] = z;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int tot = 0;
-            for (int j = 0; j < NW; ++j) tot += zsum[j];
-            atomicAdd(&llcs[P.out], tot);
-        }
+        if (lane == 0) atomicAdd(&llcs[P.out], z);   // each wave adds its own zero count
     }
 }
 
@@ -488,9 +488,9 @@ namespace {
 // one workgroup of the systolic kernel (hy_sys) on 2 or 4 waves.  Wider pairs
 // (up to 16,384 words) are cut into segments of 256 R words on workgroups of
 // their own (hy_seg, R = 1 up to 2,048 words, R = 2 beyond), so a long pair
-// spreads over as many compute units as it has segments.  (The segmented form
-// with 4 words per lane is not used: it ran past a test's time limit on a
-// 7 x 524,289 pair, cause not found.)  Class 5 (one 16-wave workgroup of
+// spreads over as many compute units as it has segments.  (A form with 4
+// words per lane would halve the segments but double each lane's step, so it
+// is no faster for any row count; it is not built.)  Class 5 (one 16-wave workgroup of
 // hy_sys with 16 words per lane) holds no pairs with these limits.
 int hy_class(int K) {
     static const int limits[] = {64, 128, 256, 2048, 16384, 16384};

@@ -14,11 +14,7 @@ pytestmark = pytest.mark.gpu
 # packed-string lengths straddling the launch-class and segment boundaries
 # (64, 128, 256, 2,048, 8,192 words; segments of 256 and 512 words)
 BOUNDARY_N = [1, 63, 64, 65, 4095, 4096, 4097, 8192, 8193, 16384, 16385, 32769, 65536, 65537, 131071, 131073,
-              262144, 262145, 524288,
-              pytest.param(524289, marks=pytest.mark.skip(
-                  reason="ran past the 120 s limit on the segmented kernel with 4 words per lane (that path is "
-                         "switched off, cause not found); the boundary is covered by 524353 on the current path")),
-              524353]
+              262144, 262145, 524288, 524289, 524353]
 
 
 @pytest.mark.parametrize("n", BOUNDARY_N)
