@@ -7,15 +7,6 @@
 
 namespace slhip {
 
-struct BitDesc {
-    const uint32_t *a_rev;
-    const uint32_t *b;
-    uint64_t *buf;
-    uint32_t *dis;
-    int32_t M, N;
-    int32_t npasses;
-    uint32_t epoch;
-};
 hipError_t launch_bitpar(const BitDesc &d, unsigned waves, LaunchCtl *ctl, uint32_t spin_limit, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t stream);
 
@@ -111,7 +102,7 @@ int sl_bitpar_create(sl_bitpar **out, const uint64_t *a_rev, int a_words, const 
     p->d.dis = p->d_scratch + 2;
     p->d.M = (int32_t)M;
     p->d.N = (int32_t)N;
-    p->d.npasses = (int32_t)((M + 63) / 64);
+    p->d.npasses = (int32_t)((M + kBitRowLanes - 1) / kBitRowLanes);
     p->d.epoch = 0;
     p->waves = (unsigned)std::min<int64_t>(p->d.npasses, (int64_t)device_cus() * 16);
     *out = reinterpret_cast<sl_bitpar *>(p);

@@ -82,6 +82,20 @@ struct DeviceArena {
     virtual void release(void *p) = 0;
 };
 
+// One bit-parallel problem (kernel (b)): M 32-bit row-words of the reversed
+// a, N column-words of b.  A pass is kBitRowLanes row-words: one wave whose
+// lane 0 feeds the column strands and whose lanes 1..63 own a row-word each.
+constexpr int kBitRowLanes = 63;
+struct BitDesc {
+    const uint32_t *a_rev;   // M words: word j = a_rev word j (32-bit view)
+    const uint32_t *b;       // N words
+    uint64_t *buf;           // N tagged entries: the column words a pass hands to the next
+    uint32_t *dis;           // popcount accumulator (dis-braid)
+    int32_t M, N;
+    int32_t npasses;         // ceil(M / kBitRowLanes)
+    uint32_t epoch;
+};
+
 // Device-side control block of one launch.
 struct LaunchCtl {
     uint32_t ticket;    // next queue entry to hand out
