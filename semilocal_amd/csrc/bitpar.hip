@@ -38,17 +38,20 @@
 // the step).  A pass's critical path is n + 64*33 steps for 63 row-words.
 //
 // Edges.  Columns before 0 carry strand 1 in both directions, which leaves a
-// row's strand 1 -- its initial value -- unchanged whatever the symbols; in
-// the last blocks a per-lane mask clears the match of columns past n, whose
-// strands are 0, so rows keep their final strands.  Rows past |a| (the last
-// pass's padding lanes) have both row masks 0: they pass t through.  b words
-// outside [0, N) are read clamped: no cell ever uses them.
+// row's strand 1 -- its initial value -- unchanged whatever the symbols.
+// Past column n the lanes keep combing whatever arrives (clamped b words,
+// zero strands): a cell only feeds cells to its right and below, so this
+// never reaches a column before n, though it does overwrite the rows'
+// strands.  The score is therefore read at the bottom edge instead of the
+// right one: every left strand leaves at the right or at the bottom, so
+//     LLCS = |a| - #(leave right) = |a| - 32M + #(leave bottom),
+// the popcount of the last pass's column words.  Rows past the 32M of the
+// a words (the last pass's padding lanes) have both row masks 0: they pass
+// t through.
 //
 // Passes of 63 row-words hand their finished column words to the next pass
 // as tagged 64-bit entries (see comb.hip), one per 32 columns, read a group
 // of blocks ahead by the next pass.
-#include <type_traits>
-
 #include "common.h"
 
 namespace slhip {
@@ -81,12 +84,9 @@ __device__ __forceinline__ void st_co(uint64_t *p, uint64_t v) {
 // one lane 1 takes in two steps later); the bits it shifts in meanwhile are
 // lane 63's leaving columns, so at the block's end it holds them in order.
 // Returns that register as it was at the start (lane 0: the previous block's
-// 32 leaving columns).  TAIL: a block where some lane meets columns past n;
-// lim0 = n + 31 - (the lane's diagonal at e = 0), so bit p meets a column
-// before n iff p < lim0 - e.
-template <bool TAIL>
+// 32 leaving columns).
 __device__ __forceinline__ uint32_t bit_block(uint32_t &L, uint32_t &tn, uint32_t &tn1, uint32_t Vhi, uint32_t Vlo,
-                                              uint32_t feed, uint32_t M1, uint32_t M0, int32_t lim0, bool feeder) {
+                                              uint32_t feed, uint32_t M1, uint32_t M0, bool feeder) {
     const uint32_t collected = tn;
     if (feeder) tn = feed;
 #pragma unroll
@@ -94,11 +94,7 @@ __device__ __forceinline__ uint32_t bit_block(uint32_t &L, uint32_t &tn, uint32_
         const uint32_t x = ror1(tn1);                          // lane k-1's column from two steps back
         const uint32_t TS = funnel(x, tn, 1);                  // slide: that column enters at bit 31
         const uint32_t BS = e == 31 ? Vhi : funnel(Vhi, Vlo, (uint32_t)e + 1u);
-        uint32_t match = (BS & M1) | (~BS & M0);
-        if (TAIL) {
-            const int32_t lim = lim0 - e;
-            match &= lim >= 32 ? 0xFFFFFFFFu : (lim <= 0 ? 0u : (1u << lim) - 1u);
-        }
+        const uint32_t match = (BS & M1) | (~BS & M0);
         const uint32_t t2 = L & (TS | match);
         L = TS | (L & ~match);
         tn1 = tn;
@@ -109,8 +105,9 @@ __device__ __forceinline__ uint32_t bit_block(uint32_t &L, uint32_t &tn, uint32_
 
 }  // namespace
 
-// One pass = kBitRowLanes consecutive row-words; returns the wave's count of left
-// strands leaving at the right (uniform across the wave).
+// One pass = kBitRowLanes consecutive row-words.  The last pass returns the
+// count of left strands leaving at the bottom (uniform across the wave); the
+// others return 0 and hand their column words to the next pass.
 //
 // Memory is read one group of kGroup blocks ahead: at a group's start the
 // wave checks the tags of the group's incoming entries (spinning, bounded,
@@ -148,8 +145,8 @@ __device__ uint32_t bit_pass(const BitDesc &d, int pass, LaunchCtl *ctl, uint32_
     uint32_t Vprev = funnel(Wold, bword(cofs - 1), r0);            // the lane's b bits of block -1
     bool ok = true;
     uint32_t tin_prev = 0xFFFFFFFFu;   // incoming word of block -1: columns before 0 carry strand 1
-    auto group = [&](int32_t w0, auto tail) {
-        constexpr bool TAIL = decltype(tail)::value;
+    uint32_t bottom = 0;   // feeder: left strands leaving at the bottom (last pass)
+    auto group = [&](int32_t w0) {
         // A pass first lets its predecessor get kSlack blocks ahead of what it
         // needs (bounded wait, sleeps growing while the predecessor is far):
         // a pass that starts right at its predecessor's heels keeps catching
@@ -199,35 +196,31 @@ __device__ uint32_t bit_pass(const BitDesc &d, int pass, LaunchCtl *ctl, uint32_
             const int32_t w = w0 + j;
             const uint32_t Wnew = bcur[j];
             const uint32_t V = funnel(Wnew, Wold, r0);
-            const int32_t lim0 = n + 31 - (32 * w - kSkew * lane);
             const uint32_t feed = j == 0 ? tin_prev : tin[j - 1];   // incoming word of block w-1
-            const uint32_t out = bit_block<TAIL>(L, tn, tn1, V, Vprev, feed, M1, M0, lim0, feeder);
+            const uint32_t out = bit_block(L, tn, tn1, V, Vprev, feed, M1, M0, feeder);
             Vprev = V;
             Wold = Wnew;
             // the feeder held lane 63's columns of block w-1: column word w - 67
             const int32_t wo = w - 67;
-            if (feeder && wo >= 0 && wo < N && !last) st_co(&d.buf[wo], my_tag | out);
+            if (feeder && wo >= 0 && wo < N) {
+                if (last) bottom += (uint32_t)__builtin_popcount(out);
+                else st_co(&d.buf[wo], my_tag | out);
+            }
         }
         tin_prev = tin[kGroup - 1];
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) bcur[j] = bnxt[j];
         tcur = tnxt;
     };
-    const int32_t n_main = (N / kGroup) * kGroup;                          // blocks with no column past n
     const int32_t n_all = ((N + 67 + kGroup - 1) / kGroup) * kGroup;      // through the last column word out
-    int32_t w0 = 0;
-    for (; w0 < n_main && ok; w0 += kGroup) group(w0, std::false_type{});
-    for (; w0 < n_all && ok; w0 += kGroup) group(w0, std::true_type{});
-    // dis-braid: left strands that leave at the right (:648-659)
-    uint32_t cnt = (ok && row_ok) ? (uint32_t)__builtin_popcount(L) : 0u;
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-    return __builtin_amdgcn_readfirstlane(cnt);
+    for (int32_t w0 = 0; w0 < n_all && ok; w0 += kGroup) group(w0);
+    return ok && last ? __builtin_amdgcn_readfirstlane(bottom) : 0u;   // lane 0 holds the count
 }
 
 __global__ __launch_bounds__(64) void bitpar_kernel(BitDesc d, LaunchCtl *ctl, uint32_t spin_limit) {
     const int lane = threadIdx.x & 63;
-    // The wave's passes add their dis-braid counts here; one atomic after the
-    // queue is drained keeps lane-0-only work out of the ticket loop, whose
+    // The last pass's bottom count is added here; one atomic after the queue
+    // is drained keeps lane-0-only work out of the ticket loop, whose
     // readfirstlane needs the whole wave converged at the top of every trip.
     uint32_t dis = 0;
     for (;;) {

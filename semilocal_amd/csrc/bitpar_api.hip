@@ -15,7 +15,7 @@ struct BitPlan {
     int64_t a_total = 0;
     uint32_t *d_words = nullptr;   // a_rev (M words) then b (N words), 32-bit view
     uint64_t *d_buf = nullptr;
-    uint32_t *d_scratch = nullptr; // [0,2): LaunchCtl, [2]: dis-braid counter
+    uint32_t *d_scratch = nullptr; // [0,2): LaunchCtl, [2]: left strands leaving at the bottom
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     uint32_t spin_limit = 1u << 23;
@@ -55,7 +55,8 @@ static int bitplan_result(BitPlan *p, int *llcs) {
         set_error("bit-parallel kernel: a bounded wait timed out; result is void");
         return kErrTimeout;
     }
-    if (llcs) *llcs = (int)(p->a_total - (int64_t)h[2]);
+    // LLCS = |a| - 32M + (left strands leaving at the bottom), counted by the last pass
+    if (llcs) *llcs = (int)(p->a_total - 32 * (int64_t)p->d.M + (int64_t)h[2]);
     return kOk;
 }
 

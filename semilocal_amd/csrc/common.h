@@ -90,7 +90,7 @@ struct BitDesc {
     const uint32_t *a_rev;   // M words: word j = a_rev word j (32-bit view)
     const uint32_t *b;       // N words
     uint64_t *buf;           // N tagged entries: the column words a pass hands to the next
-    uint32_t *dis;           // popcount accumulator (dis-braid)
+    uint32_t *dis;           // left strands leaving at the bottom (added by the last pass)
     int32_t M, N;
     int32_t npasses;         // ceil(M / kBitRowLanes)
     uint32_t epoch;
