@@ -60,7 +60,7 @@ namespace {
 constexpr int kDppRor1 = 0x13C;   // wave_ror:1: lane k <- lane k-1, lane 0 <- lane 63
 constexpr int kGroup = 4;         // blocks of 32 steps per group: loads are issued one group ahead
 constexpr int kSkew = 33;         // steps between lane k-1 and lane k
-constexpr int kSlack = 12;        // blocks a pass lets its predecessor get ahead before it starts
+constexpr int kSlack = 2;         // blocks a pass lets its predecessor get ahead before it starts
 
 __device__ __forceinline__ uint32_t ror1(uint32_t src) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)src, kDppRor1, 0xf, 0xf, false);
