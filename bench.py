@@ -613,6 +613,25 @@ def bench_plain(S, W, golden):
 
 
 
+def tile_buckets(batch, pairs):
+    """The batch's tile choice per length bucket (rows = the shorter string):
+    pairs at each tile height and the critical chains (raised priority)."""
+    rows, prio = batch.pair_tiles()
+    m = np.array([min(len(x), len(y)) for x, y in pairs], np.int64)
+    n = np.array([max(len(x), len(y)) for x, y in pairs], np.int64)
+    edges = [0, 256, 512, 1024, 2048, 4096, 8192, 16384, 1 << 31]
+    out = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        k = (m >= lo) & (m < hi) & (rows > 0)
+        if not k.any():
+            continue
+        out.append({"rows": f"[{lo}, {hi})" if hi < (1 << 31) else f">= {lo}", "pairs": int(k.sum()),
+                    "cells": int((m[k] * n[k]).sum()),
+                    "rows_per_lane": {str(R): int((rows[k] == R).sum()) for R in (4, 8, 16) if (rows[k] == R).any()},
+                    "critical": int(prio[k].sum())})
+    return out
+
+
 def copy_probe(mb):
     """One plain device copy of `mb` MiB each way: the scale for the
     rocprofv3 FETCH_SIZE / WRITE_SIZE byte counters, profiled in the same
@@ -633,10 +652,13 @@ def main():
     ap.add_argument("--per-pair-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--rows-per-lane", type=int, default=0,
-                    help="fixed tile shape instead of the autotuner (with --waves-per-cu, --schedule)")
-    ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--schedule", type=int, default=0, help="queue order 0-3 (with --rows-per-lane)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="fixed tile shape (--rows-per-lane, --waves-per-cu, --schedule, --critical-fraction) "
+                         "instead of the autotuner")
+    ap.add_argument("--rows-per-lane", type=int, default=0, help="with --no-autotune: 4, 8, 16, or 0 = per pair")
+    ap.add_argument("--waves-per-cu", type=int, default=8)
+    ap.add_argument("--schedule", type=int, default=3, help="queue order 0-8 (with --no-autotune)")
+    ap.add_argument("--critical-fraction", type=float, default=0.6, help="with --no-autotune and per-pair tiles")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (PMC byte-counter scale)")
     ap.add_argument("--no-extras", action="store_true", help="only the ragged batch (configs 1, 2, 3, 5 skipped)")
@@ -680,19 +702,20 @@ def main():
     pairs = W.ragged_batch()
     log(f"workload generated in {time.perf_counter() - t0:.1f}s")
     batch = semilocal_amd.Batch(pairs, rows_per_lane=args.rows_per_lane)
-    if args.rows_per_lane:
+    if args.no_autotune:
         batch.set_schedule(args.schedule)
-        if args.waves_per_cu:
-            batch.set_tiles(args.rows_per_lane, args.waves_per_cu)
+        batch.set_critical_fraction(args.critical_fraction)
+        batch.set_tiles(args.rows_per_lane, args.waves_per_cu)
     else:
         t0 = time.perf_counter()
         tiles = batch.autotune(iters=2)
         log(f"autotuner picked rows_per_lane={tiles[0]} waves_per_cu={tiles[1]} schedule={batch.schedule()} "
-            f"in {time.perf_counter() - t0:.1f}s")
+            f"critical_fraction={batch.critical_fraction()} in {time.perf_counter() - t0:.1f}s")
     tiles = batch.tiles()
     st = batch.stats()
     cells = st["cells"]
-    log("batch:", st)
+    buckets = tile_buckets(batch, pairs)
+    log("batch:", st, "tile buckets:", buckets)
 
     parity = None
     if not args.no_parity:
@@ -762,7 +785,10 @@ def main():
         "config": {"workload": "ragged batch: 4096 pairs, lengths log-normal (median 2000, sigma_ln 1.0) "
                                "clipped to [100, 50000], alphabet 20, half mutated (10% subs, 2% indels), seed 4",
                    "pairs": st["pairs"], "cells": cells, "tiles": st["tiles"],
-                   "rows_per_lane": tiles[0], "waves_per_cu": tiles[1], "queue_schedule": batch.schedule()},
+                   "rows_per_lane": tiles[0] if tiles[0] else "per pair", "waves_per_cu": tiles[1],
+                   "queue_schedule": batch.schedule(),
+                   "critical_fraction": batch.critical_fraction() if tiles[0] == 0 else None,
+                   "tile_buckets": buckets},
         "one_pair_per_launch": ({"value": round(gcups_pp, 2), "unit": "GCUPS", "ms_per_step": round(pms, 3),
                                  "rows_per_lane": pp_rows} if args.per_pair_steps > 0 else None),
         "batched_vs_per_pair": round(gcups / gcups_pp, 2) if args.per_pair_steps > 0 else None,

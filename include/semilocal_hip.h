@@ -110,8 +110,8 @@ typedef struct sl_batch sl_batch;
 #define SL_BATCH_WITH_COL_TO_ROW 1   /* also produce col_to_row */
 
 /* Uploads the batch to HBM and prepares the queue.  `rows_per_lane` = 0
- * lets the built-in cost model choose the tile height from the batch's length
- * profile; 4, 8 or 16 forces one (see sl_batch_set_tiles). */
+ * gives every pair its own tile height from a cost model (see
+ * sl_batch_set_tiles); 4, 8 or 16 forces one for all pairs. */
 int sl_batch_create(sl_batch **out, int npairs, const void *a_cat, const int64_t *a_off,
                     const int32_t *a_len, const void *b_cat, const int64_t *b_off,
                     const int32_t *b_len, int sym_bytes, int flags, int rows_per_lane);
@@ -130,26 +130,45 @@ int sl_batch_download(sl_batch *bt, int32_t *row_to_col_cat, int32_t *col_to_row
  * (events around the kernel launches only). */
 int sl_batch_time(sl_batch *bt, int mode, int warmup, int iters, double *ms_per_run,
                   double *kernel_ms_out);
-/* Tile shape of the batch: rows per lane R (4, 8 or 16; one queue tile = one
- * pass of 64R rows of one pair) and persistent waves per compute unit
- * (default 8; R defaults to a cost model's choice).  Results never depend on
- * it (the semi-local kernel is unique). */
+/* Tile shape of the batch: rows per lane R (one queue tile = one pass of 64R
+ * rows of one pair) and persistent waves per compute unit (default 8).
+ * R = 4, 8 or 16 puts every pair at that height.  R = 0 (the default) gives
+ * each pair its own, in the one launch: the height with the fewest
+ * instructions for its padded rows (ceil(m / 64R) passes of n + 63 steps);
+ * a pair whose chain of passes would outlast the critical fraction of the
+ * batch's modelled time (sl_batch_set_critical_fraction) becomes a critical
+ * chain instead -- the height with the shortest chain, a short queue lag and
+ * raised wave priority; a batch too small to give every SIMD two waves at
+ * R = 16 takes one height for all from a latency model.  sl_batch_pair_tiles
+ * reports the choice.  sl_batch_get_tiles reports the policy (0 = per pair).
+ * Results never depend on any of it (the semi-local kernel is unique). */
 int sl_batch_set_tiles(sl_batch *bt, int rows_per_lane, int waves_per_cu);
 int sl_batch_get_tiles(const sl_batch *bt, int *rows_per_lane, int *waves_per_cu);
+/* Per-pair policy: critical chains are the pairs whose modelled chain of
+ * passes exceeds `fraction` x the batch's modelled throughput time (default
+ * 0.6; 0 = none).  Re-plans the queue. */
+int sl_batch_set_critical_fraction(sl_batch *bt, double fraction);
+int sl_batch_get_critical_fraction(const sl_batch *bt, double *fraction);
+/* Tile height (4, 8 or 16; 0 for a pair with an empty string) and priority
+ * flag (1 = critical chain) of every pair, in caller order; either array may
+ * be NULL. */
+int sl_batch_pair_tiles(const sl_batch *bt, int32_t *rows_per_lane, int32_t *priority);
 /* Queue order of the batched launch: 0 = pair by pair, largest pairs first;
- * s = 1, 2, 3 = a list-scheduling simulation that hands out a pair's passes
+ * s = 1 .. 8 = a list-scheduling simulation that hands out a pair's passes
  * about when each can start behind the previous one, assuming a pass starts
- * 128 s steps after its predecessor (default 2).  Results never depend on
- * it. */
+ * 128 s steps after its predecessor (256 for a critical chain; default 2).
+ * Results never depend on it. */
 int sl_batch_set_schedule(sl_batch *bt, int schedule);
 int sl_batch_get_schedule(const sl_batch *bt, int *schedule);
-/* Autotuner: times every tile shape -- rows per lane {4, 8, 16} x persistent
- * waves per CU {4, 8, 12, 16, 20, 24, 32} x queue order {0..3} -- on this batch
- * (`iters` runs each, HIP events), re-times the four fastest warm in 7
- * interleaved rounds (per round and shape one warm-up run, then
- * max(3, iters) timed runs) and keeps the one with the fastest median round;
- * reports the shape through the out-pointers (the order through
- * sl_batch_get_schedule). */
+/* Autotuner: times every tile shape -- rows per lane {4, 8, 16, and per pair
+ * with critical fraction 0, 0.4, 0.6, 0.8} x persistent waves per CU
+ * {4, 8, 12, 16} x queue order {0, 2, 3, 4, 6} -- on this batch (`iters` runs
+ * each, HIP events), re-times the four fastest warm in 7 interleaved rounds
+ * (per round and shape one warm-up run, then max(3, iters) timed runs) and
+ * keeps the one with the fastest median round; reports the shape through the
+ * out-pointers (0 rows per lane = per pair; the order through
+ * sl_batch_get_schedule, the fraction through
+ * sl_batch_get_critical_fraction). */
 int sl_batch_autotune(sl_batch *bt, int iters, int *rows_per_lane, int *waves_per_cu);
 /* Sum of m*n over the batch (cells), number of queue tiles, number of pairs. */
 int sl_batch_stats(const sl_batch *bt, int64_t *cells, int64_t *tiles, int64_t *pairs);
@@ -163,12 +182,13 @@ int sl_batch_stats(const sl_batch *bt, int64_t *cells, int64_t *tiles, int64_t *
  * waves share a CU.  Results never depend on it. */
 int sl_batch_cell_modes(const sl_batch *bt, int64_t *code_pairs, int64_t *onehot_pairs, int64_t *table_pairs);
 /* Host-only planning, no device needed: the queue a batch of pairs with
- * these string lengths gets at tile height `rows_per_lane`, `waves_per_cu`
+ * these string lengths gets at tile policy `rows_per_lane` (0 = per pair, as
+ * in sl_batch_set_tiles, with the default critical fraction), `waves_per_cu`
  * waves on `cus` compute units and queue order `schedule` (as in
  * sl_batch_set_schedule).  Writes the (pair, pass) tickets in launch order
  * (pair_out / pass_out may be NULL to ask for *nitems only; otherwise they
- * hold *nitems entries: one per pass of 64 * rows_per_lane rows of every
- * pair with both lengths > 0). */
+ * hold *nitems entries: one per pass of 64 R rows of every pair with both
+ * lengths > 0, R its tile height). */
 int sl_plan_queue(int npairs, const int32_t *a_len, const int32_t *b_len, int rows_per_lane, int waves_per_cu,
                   int cus, int schedule, int32_t *pair_out, int32_t *pass_out, int64_t *nitems);
 void sl_batch_destroy(sl_batch *bt);

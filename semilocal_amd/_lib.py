@@ -59,6 +59,9 @@ SIGNATURES = {
     "sl_batch_get_tiles": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "sl_batch_autotune": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "sl_batch_set_schedule": ([_vp, C.c_int], C.c_int),
+    "sl_batch_set_critical_fraction": ([_vp, C.c_double], C.c_int),
+    "sl_batch_get_critical_fraction": ([_vp, C.POINTER(C.c_double)], C.c_int),
+    "sl_batch_pair_tiles": ([_vp, _vp, _vp], C.c_int),
     "sl_batch_get_schedule": ([_vp, C.POINTER(C.c_int)], C.c_int),
     "sl_batch_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "sl_batch_cell_modes": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
@@ -446,6 +449,7 @@ class Batch:
         return ms.value, kms.value
 
     def set_tiles(self, rows_per_lane: int, waves_per_cu: int = 16):
+        """rows_per_lane 4, 8 or 16 for every pair, or 0 for a height per pair."""
         check(lib().sl_batch_set_tiles(self.h, rows_per_lane, waves_per_cu))
 
     def tiles(self):
@@ -454,13 +458,31 @@ class Batch:
         return r.value, w.value
 
     def set_schedule(self, schedule: int):
-        """Queue order of the batched launch: 0 pair by pair (largest first), 1-3 simulated (lag 128 s)."""
+        """Queue order of the batched launch: 0 pair by pair (largest first), 1-8 simulated (lag 128 s)."""
         check(lib().sl_batch_set_schedule(self.h, int(schedule)))
 
     def schedule(self) -> int:
         out = C.c_int()
         check(lib().sl_batch_get_schedule(self.h, C.byref(out)))
         return out.value
+
+    def set_critical_fraction(self, fraction: float):
+        """Per-pair tiles: pairs whose modelled chain exceeds `fraction` x the
+        batch's modelled time become critical chains (0 = none)."""
+        check(lib().sl_batch_set_critical_fraction(self.h, float(fraction)))
+
+    def critical_fraction(self) -> float:
+        out = C.c_double()
+        check(lib().sl_batch_get_critical_fraction(self.h, C.byref(out)))
+        return out.value
+
+    def pair_tiles(self):
+        """(rows_per_lane, priority) per pair, caller order (0 rows for an empty pair)."""
+        n = len(self.a_len)
+        r = np.zeros(max(n, 1), np.int32)
+        p = np.zeros(max(n, 1), np.int32)
+        check(lib().sl_batch_pair_tiles(self.h, _ptr(r), _ptr(p)))
+        return r[:n], p[:n]
 
     def autotune(self, iters: int = 1):
         r, w = C.c_int(), C.c_int()

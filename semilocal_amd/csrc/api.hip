@@ -189,6 +189,17 @@ static void remap_pairs_dense(std::vector<PairDesc> &pairs, PackedPool &pool) {
 // ---------------------------------------------------------------------------
 static const int kTileHeights[3] = {4, 8, 16};
 
+// Per-pair tile model (assign_tiles).  Instructions per step beside the R
+// cells: the DPP hand-off, the LDS write of the outgoing strand and the
+// block's LDS reads (measured on MI355X: 3.66 VALU lane-ops per cell at R = 8,
+// 6.7 % of them padding rows).
+constexpr double kStepOverheadOps = 3.4;
+constexpr double kValuEfficiency = 0.75;     // VALU busy a full batch sustains
+constexpr double kBulkCyclesPerOp = 8.0;     // two waves share a SIMD, 4 issue cycles each
+constexpr double kCriticalCyclesPerOp = 6.5; // a wave at raised priority: its own issue rate
+constexpr double kDefaultCriticalFraction = 0.6;
+constexpr double kCriticalLag = 256.0;       // queue lag (steps) between passes of a critical chain
+
 // `passes_at(R)`: the batch's total passes at tile height R.
 template <typename PassesAt>
 static int model_rows_core(PassesAt passes_at, int64_t total_cells, int32_t m_max, int32_t n_max, int cus) {
@@ -248,7 +259,9 @@ struct Batch {
     int64_t buf_total = 0;
     int64_t cells = 0;
     uint32_t spin_limit = 1u << 23;
-    int rows_per_lane = 16;            // tile height of every pass of this batch
+    int rows_per_lane = 0;             // tile policy: 4, 8 or 16 for every pair, 0 = per pair (assign_tiles)
+    double critical_fraction = kDefaultCriticalFraction;   // per-pair policy: see assign_tiles
+    int launch_rows = 16;              // tile height the kernel is instantiated for (0: per pair)
     unsigned waves_per_cu = 8;         // with schedule 2: the best default shape measured on MI355X
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -299,18 +312,125 @@ static void batch_free(Batch *b) {
     delete b;
 }
 
-constexpr int kMaxSchedule = 3;
+constexpr int kMaxSchedule = 8;
 
 // Queue order of the persistent launch.  Waves take tickets in order, and a
 // pass can only run some way behind its predecessor (it reads the strands
-// the previous pass leaves, at least 64 steps behind), so handing out the
+// the previous pass leaves, at least 128 steps behind), so handing out the
 // ~200 passes of a large pair back to back would park hundreds of waves on
 // waits.  The order comes from a list-scheduling simulation instead: W
-// waves; a pass lasts n + 64 steps; pass p of a pair becomes ready `lag`
+// waves; a pass lasts n + 64 steps, each step costing in proportion to the
+// instructions of its tile height; pass p of a pair becomes ready `lag`
 // steps after pass p-1 started; a free wave takes the ready pass of the
 // largest pair.  Passes of a pair keep their order, so a pass only ever
 // waits on a pass handed out before it.  A longer lag interleaves more pairs
-// (fewer waits, a longer tail); the autotuner picks it (128, 256 or 384).
+// and leaves a pass more slack behind its predecessor (a pass that catches
+// up with a slower predecessor waits), at the price of a longer chain per
+// pair: priority pairs (the critical chains) keep the short kCriticalLag,
+// the bulk takes `lag` (128 x schedule, chosen by the autotuner).
+
+// VALU instructions per cell of the pair's symbol test at tile height R
+// (choose_cells makes the same choice).
+static int cell_ops(const PairDesc &d, int R) {
+    if (d.tag_shift + d.sigma <= 32) return 3;                                      // one-hot tags
+    if (d.sigma <= kTableMaxSymbols && d.tag_shift <= 32 - R) return 3;             // row-match table
+    return 4;                                                                       // symbol codes
+}
+static int64_t passes_at(const PairDesc &d, int R) { return ((int64_t)d.m + 64LL * R - 1) / (64LL * R); }
+static double step_ops(const PairDesc &d, int R) { return cell_ops(d, R) * (double)R + kStepOverheadOps; }
+// Cycles from the start of the pair's first pass to the end of its last: the
+// last pass starts (P - 1) lags after the first and runs n + 64 steps.
+static double chain_cycles(const PairDesc &d, int R, double lag, double cycles_per_op) {
+    return ((double)d.n + 64.0 + (double)(passes_at(d, R) - 1) * lag) * step_ops(d, R) * cycles_per_op;
+}
+
+// Tile height (d.rows_per_lane) and priority (d.prio) of every pair.
+//  * Policy 4, 8 or 16: every pair at that height, no priority.
+//  * Policy 0, per pair (the default):
+//    - a batch whose passes at R = 16 cannot give every SIMD two waves is
+//      latency-bound: one height for all from model_rows_per_lane;
+//    - otherwise each pair takes the height with the fewest instructions for
+//      its padded rows, ceil(m / 64R) passes x (n + 63) steps x step_ops(R)
+//      (a pair of 600 rows wastes 40 % of a 1,024-row pass, 15 % of two
+//      512-row passes), and the batch's modelled throughput time is those
+//      instructions x 4 cycles / (SIMDs x kValuEfficiency);
+//    - a pair whose chain of passes at that height and the bulk lag would
+//      take longer than critical_fraction x that time is a critical chain:
+//      it takes the height with the shortest chain at kCriticalLag (short
+//      passes: R = 4 for the largest pairs, whatever it costs in
+//      instructions) and runs at raised wave priority.  critical_fraction 0
+//      marks none.
+// The reference sizes its blocks for the same two forces -- enough parallel
+// work, few row pieces -- in hybrid_iterative_wrapper (semi_local.h:740-790).
+static void assign_tiles(Batch *bt) {
+    for (auto &d : bt->pairs) d.prio = 0;
+    if (bt->rows_per_lane != 0) {
+        for (auto &d : bt->pairs) d.rows_per_lane = bt->rows_per_lane;
+        bt->launch_rows = bt->rows_per_lane;
+        return;
+    }
+    const int cus = bt->cus > 0 ? bt->cus : device_cus();
+    const double simds = 4.0 * std::max(cus, 1);
+    double passes16 = 0.0;
+    int32_t m_max = 0, n_max = 0;
+    for (const auto &d : bt->pairs) {
+        if (d.m <= 0 || d.n <= 0) continue;
+        passes16 += (double)passes_at(d, 16);
+        m_max = std::max(m_max, d.m);
+        n_max = std::max(n_max, d.n);
+    }
+    if (passes16 < 2.0 * simds) {
+        const int R = model_rows_per_lane(bt->pairs, bt->cells, m_max, n_max, cus);
+        for (auto &d : bt->pairs) d.rows_per_lane = R;
+        bt->launch_rows = R;
+        return;
+    }
+    double instr = 0.0;
+    for (auto &d : bt->pairs) {
+        d.rows_per_lane = 16;
+        if (d.m <= 0 || d.n <= 0) continue;
+        double best = 1e300;
+        for (int R : kTileHeights) {
+            const double c = (double)passes_at(d, R) * ((double)d.n + 63.0) * step_ops(d, R);
+            if (c < best) {
+                best = c;
+                d.rows_per_lane = R;
+            }
+        }
+        instr += best;
+    }
+    const double t_thru = instr * 4.0 / (simds * kValuEfficiency);
+    const double lag = 128.0 * std::max(bt->schedule, 1);
+    if (bt->critical_fraction > 0.0) {
+        for (auto &d : bt->pairs) {
+            if (d.m <= 0 || d.n <= 0) continue;
+            if (chain_cycles(d, d.rows_per_lane, lag, kBulkCyclesPerOp) <= bt->critical_fraction * t_thru) continue;
+            double best = 1e300;
+            for (int R : kTileHeights) {
+                const double c = chain_cycles(d, R, kCriticalLag, kCriticalCyclesPerOp);
+                if (c < best) {
+                    best = c;
+                    d.rows_per_lane = R;
+                }
+            }
+            d.prio = 1;
+        }
+    }
+    int common = -1;
+    for (const auto &d : bt->pairs) {
+        if (d.m <= 0 || d.n <= 0) continue;
+        if (common == -1) common = d.rows_per_lane;
+        else if (common != d.rows_per_lane) common = 0;
+    }
+    bt->launch_rows = common == -1 ? 16 : common;
+    if (bt->launch_rows != 0)
+        for (auto &d : bt->pairs) d.rows_per_lane = bt->launch_rows;
+}
+
+static double step_cost(const PairDesc &d) {   // relative to a tile height of 8
+    return (3.0 * d.rows_per_lane + kStepOverheadOps) / (3.0 * 8 + kStepOverheadOps);
+}
+
 static void schedule_items(Batch *bt, double lag) {
     const size_t nitems = bt->items.size();
     bt->order.clear();
@@ -343,12 +463,13 @@ static void schedule_items(Batch *bt, double lag) {
         ready.pop();
         const PairDesc &d = bt->pairs[r.pair];
         bt->order.push_back(WorkItem{r.pair, next[r.pair]});
-        waves.push(t + (double)d.n + 64.0);
-        if (++next[r.pair] < d.npasses) pending.push(Pending{t + lag, r.pair});
+        const double sc = step_cost(d);
+        waves.push(t + ((double)d.n + 64.0) * sc);
+        if (++next[r.pair] < d.npasses) pending.push(Pending{t + (d.prio ? kCriticalLag : lag) * sc, r.pair});
     }
 }
 
-// Symbol test of each pair's cell for the batch's tile height R: one-hot
+// Symbol test of each pair's cell for its tile height R: one-hot
 // tags when bits(m+n-1) + sigma <= 32; else the per-lane row-match table when
 // sigma <= kTableMaxSymbols and the R mask bits fit above the ids
 // (bits(m+n-1) <= 32 - R); else symbol codes.  The table takes
@@ -358,7 +479,7 @@ static void choose_cells(Batch *bt) {
     for (auto &d : bt->pairs) {
         if (d.tag_shift + d.sigma <= 32) {
             d.cell = kCellOneHot;
-        } else if (d.sigma <= kTableMaxSymbols && d.tag_shift <= 32 - bt->rows_per_lane) {
+        } else if (d.sigma <= kTableMaxSymbols && d.tag_shift <= 32 - d.rows_per_lane) {
             d.cell = kCellTable;
             table_sigma = std::max(table_sigma, d.sigma);
         } else {
@@ -368,13 +489,14 @@ static void choose_cells(Batch *bt) {
     bt->table_words = table_sigma ? (uint32_t)(table_sigma + 1) * 64u : 0u;
 }
 
-// Splits every pair into passes of 64R rows.  `items` lists the passes pair
-// by pair (largest pairs first; the one-launch-per-pair form launches a
-// pair's consecutive run), `order` is the ticket order of the batched launch:
-// `items` itself (longest-processing-time first) or the simulated schedule;
-// the autotuner measures both.
+// Splits every pair into passes of 64R rows at its tile height (assign_tiles).
+// `items` lists the passes pair by pair (largest pairs first; the
+// one-launch-per-pair form launches a pair's consecutive run), `order` is the
+// ticket order of the batched launch: `items` itself
+// (longest-processing-time first) or the simulated schedule; the autotuner
+// measures both.
 static void plan_items(Batch *bt) {
-    const int64_t rpp = 64LL * bt->rows_per_lane;
+    assign_tiles(bt);
     std::vector<int> order(bt->npairs);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -383,7 +505,7 @@ static void plan_items(Batch *bt) {
     bt->items.clear();
     for (int i : order) {
         PairDesc &d = bt->pairs[i];
-        d.rows_per_lane = bt->rows_per_lane;
+        const int64_t rpp = 64LL * d.rows_per_lane;
         d.npasses = (d.m > 0 && d.n > 0) ? (int32_t)((d.m + rpp - 1) / rpp) : 0;
         d.item0 = (int32_t)bt->items.size();
         for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
@@ -447,7 +569,7 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
         return kErrArg;
     }
     if (rows_per_lane != 0 && rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16) {
-        set_error("rows_per_lane must be 0 (model), 4, 8 or 16");
+        set_error("rows_per_lane must be 0 (per pair), 4, 8 or 16");
         return kErrArg;
     }
     for (int i = 0; i < npairs; ++i) {
@@ -488,7 +610,6 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
     // orientation and output/buffer offsets (caller order)
     bt->pairs.resize(npairs);
     bt->perm_off.resize(npairs);
-    int32_t m_max = 0, n_max = 0;
     for (int i = 0; i < npairs; ++i) {
         PairDesc &d = bt->pairs[i];
         const bool rev = a_len[i] > b_len[i];
@@ -503,12 +624,9 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
         bt->cells += (int64_t)d.m * d.n;
         d.buf_off = bt->buf_total;   // strand buffer for multi-pass tiling (sized for any R)
         bt->buf_total += d.n;
-        m_max = std::max(m_max, d.m);
-        n_max = std::max(n_max, d.n);
     }
     remap_pairs_dense(bt->pairs, pool);
-    bt->rows_per_lane =
-        rows_per_lane ? rows_per_lane : model_rows_per_lane(bt->pairs, bt->cells, m_max, n_max, device_cus());
+    bt->rows_per_lane = rows_per_lane;   // 0: per pair (assign_tiles)
     plan_items(bt);
 
     hipError_t e = hipSuccess;
@@ -619,7 +737,7 @@ static hipError_t launch_items(Batch *bt, const WorkItem *items, uint32_t nitems
     const void *pool = bt->d_pool;
     const uint32_t table_words = waves >= (unsigned)device_cus() ? bt->table_words : 0u;
     // per-pair pointers are resolved inside the kernel from PairDesc offsets
-    return launch_comb_batch(bt->sym_bytes, bt->rows_per_lane, waves, bt->stream, bt->d_pairs, items, nitems, pool,
+    return launch_comb_batch(bt->sym_bytes, bt->launch_rows, waves, bt->stream, bt->d_pairs, items, nitems, pool,
                              bt->d_rc, bt->d_cr, bt->d_buf, bt->epoch, ctl, bt->spin_limit, table_words);
 }
 
@@ -883,9 +1001,9 @@ int sl_batch_time(sl_batch *h, int mode, int warmup, int iters, double *ms_per_r
 
 int sl_batch_set_tiles(sl_batch *h, int rows_per_lane, int waves_per_cu) {
     Batch *bt = reinterpret_cast<Batch *>(h);
-    if (!bt || (rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16) || waves_per_cu < 1 ||
-        waves_per_cu > 64) {
-        set_error("sl_batch_set_tiles: rows_per_lane must be 4, 8 or 16 and waves_per_cu 1..64");
+    if (!bt || (rows_per_lane != 0 && rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16) ||
+        waves_per_cu < 1 || waves_per_cu > 64) {
+        set_error("sl_batch_set_tiles: rows_per_lane must be 0 (per pair), 4, 8 or 16 and waves_per_cu 1..64");
         return kErrArg;
     }
     SL_HIP_TRY(hipStreamSynchronize(bt->stream));
@@ -907,10 +1025,48 @@ int sl_batch_get_tiles(const sl_batch *h, int *rows_per_lane, int *waves_per_cu)
     return kOk;
 }
 
+int sl_batch_set_critical_fraction(sl_batch *h, double fraction) {
+    Batch *bt = reinterpret_cast<Batch *>(h);
+    if (!bt || !(fraction >= 0.0) || fraction > 1e6) {
+        set_error("sl_batch_set_critical_fraction: fraction must be >= 0 (0: no critical chains)");
+        return kErrArg;
+    }
+    SL_HIP_TRY(hipStreamSynchronize(bt->stream));
+    bt->critical_fraction = fraction;
+    plan_items(bt);
+    SL_HIP_TRY(upload_items(bt));
+    return kOk;
+}
+
+int sl_batch_get_critical_fraction(const sl_batch *h, double *fraction) {
+    const Batch *bt = reinterpret_cast<const Batch *>(h);
+    if (!bt || !fraction) {
+        set_error("sl_batch_get_critical_fraction: bad argument");
+        return kErrArg;
+    }
+    *fraction = bt->critical_fraction;
+    return kOk;
+}
+
+int sl_batch_pair_tiles(const sl_batch *h, int32_t *rows_per_lane, int32_t *priority) {
+    const Batch *bt = reinterpret_cast<const Batch *>(h);
+    if (!bt || (bt->npairs > 0 && !rows_per_lane && !priority)) {
+        set_error("sl_batch_pair_tiles: bad argument");
+        return kErrArg;
+    }
+    for (int i = 0; i < bt->npairs; ++i) {
+        const PairDesc &d = bt->pairs[(size_t)i];
+        const bool empty = d.m <= 0 || d.n <= 0;
+        if (rows_per_lane) rows_per_lane[i] = empty ? 0 : d.rows_per_lane;
+        if (priority) priority[i] = empty ? 0 : d.prio;
+    }
+    return kOk;
+}
+
 int sl_batch_set_schedule(sl_batch *h, int schedule) {
     Batch *bt = reinterpret_cast<Batch *>(h);
     if (!bt || schedule < 0 || schedule > kMaxSchedule) {
-        set_error("sl_batch_set_schedule: schedule must be 0, 1, 2 or 3");
+        set_error("sl_batch_set_schedule: schedule must be 0..8");
         return kErrArg;
     }
     SL_HIP_TRY(hipStreamSynchronize(bt->stream));
@@ -938,10 +1094,12 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
     }
     struct Shape {
         int sched, r, w;
+        double crit;
         double ms;
     };
     auto time_shape = [&](Shape &sh, int warmup, int runs) -> int {
         bt->schedule = sh.sched;
+        bt->critical_fraction = sh.crit;
         int rc = sl_batch_set_tiles(h, sh.r, sh.w);
         if (!rc) rc = sl_batch_time(h, 0, warmup, runs, &sh.ms, nullptr);
         if (rc)
@@ -956,16 +1114,19 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
     // and a few timed runs each, shape after shape) and ranked by their
     // median round, so a drift of the clock over the seconds of tuning does
     // not favour the shape timed first or last.
-    const int wpc_cand[] = {4, 8, 12, 16, 20, 24, 32};
+    const int wpc_cand[] = {4, 8, 12, 16};
+    const int sched_cand[] = {0, 2, 3, 4, 6};
+    const double crit_cand[] = {0.0, 0.4, 0.6, 0.8};
     std::vector<Shape> shapes;
-    for (int sched = 0; sched <= kMaxSchedule; ++sched)
-        for (int r : kTileHeights)
-            for (int w : wpc_cand) {
-                Shape sh{sched, r, w, 0.0};
-                const int rc = time_shape(sh, 0, iters);
-                if (rc) return rc;
-                shapes.push_back(sh);
-            }
+    for (int sched : sched_cand)
+        for (int w : wpc_cand) {
+            for (int r : kTileHeights) shapes.push_back(Shape{sched, r, w, 0.0, 0.0});
+            for (double c : crit_cand) shapes.push_back(Shape{sched, 0, w, c, 0.0});
+        }
+    for (Shape &sh : shapes) {
+        const int rc = time_shape(sh, 0, iters);
+        if (rc) return rc;
+    }
     std::sort(shapes.begin(), shapes.end(), [](const Shape &x, const Shape &y) { return x.ms < y.ms; });
     const size_t finalists = std::min<size_t>(4, shapes.size());
     constexpr int kRounds = 7;
@@ -985,6 +1146,7 @@ int sl_batch_autotune(sl_batch *h, int iters, int *rows_per_lane, int *waves_per
     const Shape best = *std::min_element(shapes.begin(), shapes.begin() + (std::ptrdiff_t)finalists,
                                          [](const Shape &x, const Shape &y) { return x.ms < y.ms; });
     bt->schedule = best.sched;
+    bt->critical_fraction = best.crit;
     int rc = sl_batch_set_tiles(h, best.r, best.w);
     if (rc) return rc;
     if (rows_per_lane) *rows_per_lane = best.r;
@@ -1024,8 +1186,8 @@ void sl_batch_destroy(sl_batch *h) { batch_free(reinterpret_cast<Batch *>(h)); }
 int sl_plan_queue(int npairs, const int32_t *a_len, const int32_t *b_len, int rows_per_lane, int waves_per_cu, int cus,
                   int schedule, int32_t *pair_out, int32_t *pass_out, int64_t *nitems) {
     if (npairs < 0 || (npairs > 0 && (!a_len || !b_len)) || !nitems ||
-        (rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16) || waves_per_cu < 1 || waves_per_cu > 64 ||
-        cus < 1 || schedule < 0 || schedule > kMaxSchedule) {
+        (rows_per_lane != 0 && rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16) ||
+        waves_per_cu < 1 || waves_per_cu > 64 || cus < 1 || schedule < 0 || schedule > kMaxSchedule) {
         set_error("sl_plan_queue: bad argument");
         return kErrArg;
     }
@@ -1042,6 +1204,8 @@ int sl_plan_queue(int npairs, const int32_t *a_len, const int32_t *b_len, int ro
         d.m = std::min(a_len[i], b_len[i]);   // rows = the shorter string, as in a batch
         d.n = std::max(a_len[i], b_len[i]);
         d.sigma = 1;
+        d.tag_shift = 1;
+        bt.cells += (int64_t)d.m * d.n;
     }
     bt.rows_per_lane = rows_per_lane;
     bt.waves_per_cu = (unsigned)waves_per_cu;

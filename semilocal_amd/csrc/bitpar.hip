@@ -59,7 +59,6 @@ namespace {
 
 constexpr int kDppRor1 = 0x13C;   // wave_ror:1: lane k <- lane k-1, lane 0 <- lane 63
 constexpr int kGroup = 4;         // blocks of 32 steps per group: loads are issued one group ahead
-constexpr int kSkew = 33;         // steps between lane k-1 and lane k
 constexpr int kSlack = 2;         // blocks a pass lets its predecessor get ahead before it starts
 
 __device__ __forceinline__ uint32_t ror1(uint32_t src) {
@@ -115,7 +114,7 @@ __device__ __forceinline__ uint32_t bit_block(uint32_t &L, uint32_t &tn, uint32_
 // only the group's end waits for them -- so no wait sits inside the blocks.
 __device__ uint32_t bit_pass(const BitDesc &d, int pass, LaunchCtl *ctl, uint32_t spin_limit) {
     const int lane = threadIdx.x & 63;
-    const int32_t M = d.M, N = d.N, n = 32 * d.N;
+    const int32_t M = d.M, N = d.N;
     const bool feeder = lane == 0;
     const int32_t g = pass * kBitRowLanes + lane - 1;    // row-word of lane >= 1
     const bool row_ok = !feeder && g < M;

@@ -133,8 +133,10 @@ __device__ __forceinline__ uint32_t sym_value(uint32_t code, uint32_t tag_shift)
     return MODE == kCellOneHot ? (1u << (tag_shift + code)) : MODE == kCellTable ? code * 256u : code;
 }
 
+// Always inlined into the queue loop: the pair descriptor and the loop's
+// uniform values then stay in scalar registers.
 template <int R, int MODE, typename Sym>
-__device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ pool, int32_t *__restrict__ rc,
+__device__ __attribute__((always_inline)) void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ pool, int32_t *__restrict__ rc,
                           int32_t *__restrict__ cr, uint64_t *__restrict__ buf, uint32_t epoch,
                           LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W, uint32_t *T) {
     constexpr bool ONEHOT = MODE != kCellCode;
@@ -302,13 +304,30 @@ __device__ void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ p
     }
 }
 
+// One pass at tile height R, with the pair's symbol test.
+template <int R, typename Sym>
+__device__ __attribute__((always_inline)) void comb_pass_cells(const PairDesc &d, int pass, const Sym *pool,
+                                                               int32_t *rc, int32_t *cr, uint64_t *buf,
+                                                               uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit,
+                                                               uint32_t table_words, uint32_t *W, uint32_t *T) {
+    if (d.cell == kCellOneHot)
+        comb_pass<R, kCellOneHot, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+    else if (d.cell == kCellTable && table_words != 0)
+        comb_pass<R, kCellTable, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+    else
+        comb_pass<R, kCellCode, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+}
 
 }  // namespace
 
 // Persistent queue consumer: every wave takes (pair, pass) tickets until the
-// queue is empty.  One instantiation per tile height R (rows per lane); each
-// holds both cell modes, chosen per pair.
-template <typename Sym, int R>
+// queue is empty.  RSET = 4, 8 or 16: every pair of the launch has that tile
+// height (one instantiation per height); RSET = 0: each pair has its own
+// (d.rows_per_lane, any of 4, 8, 16), dispatched per ticket like the cell
+// mode.  A pair marked `prio` (a chain of passes on the batch's critical path)
+// runs its passes at raised wave priority, so on a SIMD shared with bulk
+// passes its instructions issue first.
+template <typename Sym, int RSET>
 __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restrict__ pairs,
                                                          const WorkItem *__restrict__ items, uint32_t nitems,
                                                          const Sym *__restrict__ pool, int32_t *__restrict__ rc,
@@ -326,12 +345,21 @@ __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restr
         if (tk >= nitems) break;     // every wave leaves once the queue is empty
         const WorkItem it = items[tk];
         const PairDesc d = pairs[it.pair];
-        if (d.cell == kCellOneHot)
-            comb_pass<R, kCellOneHot, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
-        else if (d.cell == kCellTable && table_words != 0)
-            comb_pass<R, kCellTable, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+        const int R = d.rows_per_lane;
+        if (RSET != 0 ? R != RSET : (R != 4 && R != 8 && R != 16)) {
+            // a descriptor this instantiation cannot comb (a planning defect):
+            // reported, never run with the wrong row layout
+            if (lane == 0) __hip_atomic_fetch_or(&ctl->error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        if (d.prio) __builtin_amdgcn_s_setprio(3);
+        if (RSET == 4 || (RSET == 0 && R == 4))
+            comb_pass_cells<4, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T);
+        else if (RSET == 8 || (RSET == 0 && R == 8))
+            comb_pass_cells<8, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T);
         else
-            comb_pass<R, kCellCode, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+            comb_pass_cells<16, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T);
+        if (d.prio) __builtin_amdgcn_s_setprio(0);
     }
 }
 
@@ -348,18 +376,20 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStrea
     return hipGetLastError();
 }
 
-template <typename Sym, int R>
+template <typename Sym, int RSET>
 static void launch_one(unsigned blocks, hipStream_t stream, const PairDesc *pairs, const WorkItem *items,
                        uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch,
                        LaunchCtl *ctl, uint32_t spin_limit, uint32_t table_words) {
     const size_t lds_bytes = sizeof(uint32_t) * kWavesPerBlock * (kWaveLds + table_words);
-    hipLaunchKernelGGL((comb_batch_kernel<Sym, R>), dim3(blocks), dim3(kWavesPerBlock * 64), lds_bytes, stream, pairs,
+    hipLaunchKernelGGL((comb_batch_kernel<Sym, RSET>), dim3(blocks), dim3(kWavesPerBlock * 64), lds_bytes, stream, pairs,
                        items, nitems, (const Sym *)pool, rc, cr, buf, epoch, ctl, spin_limit, table_words);
 }
 
-// table_words: (largest sigma of the batch's kCellTable pairs + 1) x 64, or 0:
-// then kCellTable pairs take the code cell (one fewer LDS read per step, the
-// lower-latency choice for launches with few waves).
+// rows_per_lane: 4, 8 or 16 when every pair of the launch has that tile
+// height, 0 when the pairs carry their own.  table_words: (largest sigma of
+// the batch's kCellTable pairs + 1) x 64, or 0: then kCellTable pairs take
+// the code cell (one fewer LDS read per step, the lower-latency choice for
+// launches with few waves).
 hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
                              const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
                              int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch, LaunchCtl *ctl,
@@ -369,12 +399,16 @@ hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, h
     const unsigned blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
 #define SL_LAUNCH(SYM, RR) \
     launch_one<SYM, RR>(blocks, stream, pairs, items, nitems, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words)
+    if (rows_per_lane != 0 && rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16)
+        return hipErrorInvalidValue;
     if (sym_bytes == 1) {
-        if (rows_per_lane == 4) SL_LAUNCH(uint8_t, 4);
+        if (rows_per_lane == 0) SL_LAUNCH(uint8_t, 0);
+        else if (rows_per_lane == 4) SL_LAUNCH(uint8_t, 4);
         else if (rows_per_lane == 8) SL_LAUNCH(uint8_t, 8);
         else SL_LAUNCH(uint8_t, 16);
     } else {
-        if (rows_per_lane == 4) SL_LAUNCH(uint16_t, 4);
+        if (rows_per_lane == 0) SL_LAUNCH(uint16_t, 0);
+        else if (rows_per_lane == 4) SL_LAUNCH(uint16_t, 4);
         else if (rows_per_lane == 8) SL_LAUNCH(uint16_t, 8);
         else SL_LAUNCH(uint16_t, 16);
     }

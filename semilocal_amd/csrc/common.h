@@ -56,7 +56,7 @@ struct PairDesc {
     int32_t cell;       // how the cell tests symbols: kCellCode, kCellOneHot or kCellTable
     int32_t tag_shift;  // bits of the largest strand id (one-hot tags start here)
     int32_t sigma;      // distinct symbols of the pair (dense codes 0..sigma-1)
-    int32_t pad_;
+    int32_t prio;       // 1: the pair's passes run at raised wave priority (a critical chain)
 };
 
 // Symbol test of the combing cell, chosen per pair (and per tile height R).

@@ -91,12 +91,24 @@ def test_ragged_batch_4096_matches_reference_fixture(gpu):
     c = G.configs()["ragged_batch_4096"]
     pairs = W.ragged_batch()
     b = gpu.Batch(pairs)
+    # the default plan: a tile height per pair, the largest pairs as
+    # critical chains at raised priority
+    assert b.tiles()[0] == 0
+    rows, prio = b.pair_tiles()
+    assert set(rows.tolist()) == {4, 8, 16} and 1 <= prio.sum() <= 64
     res = b.run().results()
     bad = [i for i, p in enumerate(res) if W.sha256(p) != c["perm_sha256_each"][i]]
     assert not bad, bad[:10]
     # the same pairs, one launch per pair: identical
     res2 = b.run(per_pair=True).results()
     assert all(np.array_equal(x, y) for x, y in zip(res, res2))
+    # per-pair heights with no critical chains, long queue lags, 3 waves per SIMD
+    for crit, sched, w in [(0.0, 6, 8), (0.8, 4, 12), (0.4, 0, 8)]:
+        b.set_critical_fraction(crit)
+        b.set_schedule(sched)
+        b.set_tiles(0, w)
+        res3 = b.run().results()
+        assert all(np.array_equal(x, y) for x, y in zip(res, res3)), (crit, sched, w)
     b.close()
 
 
@@ -107,14 +119,14 @@ def test_autotune_and_tiles_do_not_change_results(gpu, oracle):
     want = [oracle.comb(x, y) for x, y in pairs]
     b = gpu.Batch(pairs)
     r, w = b.autotune(iters=1)
-    assert r in (4, 8, 16) and 1 <= w <= 64
+    assert r in (0, 4, 8, 16) and 1 <= w <= 64
     assert b.tiles() == (r, w)
     for got, exp in zip(b.run().results(), want):
         assert np.array_equal(got, exp)
-    for sched in (0, 1, 2, 3):                # queue order: pair by pair, or the simulated schedules
+    for sched in (0, 1, 2, 3, 6, 8):          # queue order: pair by pair, or the simulated schedules
         b.set_schedule(sched)
         assert b.schedule() == sched
-        for r2, w2 in [(4, 8), (8, 12), (16, 16), (4, 32)]:
+        for r2, w2 in [(4, 8), (8, 12), (16, 16), (4, 32), (0, 8), (0, 12)]:
             b.set_tiles(r2, w2)
             for got, exp in zip(b.run().results(), want):
                 assert np.array_equal(got, exp)

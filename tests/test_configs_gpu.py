@@ -52,7 +52,7 @@ def test_similar_pair_1e6_hybrid_8x8_matches_reference_hybrid(gpu):
 def test_similar_pair_blocks_every_tile_shape_gives_the_same_braids(gpu):
     # The hybrid's combing launch on this configuration: 64 blocks of ~125k x
     # 125k combed as one batch.  Every shape of the autotuner's grid (queue
-    # order x rows per lane x waves per CU) must give the braids of the
+    # order x rows per lane, fixed or per pair, x waves per CU) must give the braids of the
     # default shape, whose hybrid result matches the reference's fixture.
     a, b = W.similar_pair()
     ba = [a[i * len(a) // 8:(i + 1) * len(a) // 8] for i in range(8)]
@@ -60,18 +60,18 @@ def test_similar_pair_blocks_every_tile_shape_gives_the_same_braids(gpu):
     bt = gpu.Batch([(x, y) for x in ba for y in bb])
     want = bt.run().results()
     bad = []
-    for sched in (0, 1, 2, 3):
+    shapes = [(sched, R, w) for sched in (0, 1, 2, 3) for R in (4, 8, 16) for w in (4, 8, 12, 16, 20, 24, 32)]
+    shapes += [(sched, 0, w) for sched in (2, 4, 6) for w in (8, 12)]   # a height per pair, longer lags
+    for sched, R, w in shapes:
         bt.set_schedule(sched)
-        for R in (4, 8, 16):
-            for w in (4, 8, 12, 16, 20, 24, 32):
-                bt.set_tiles(R, w)
-                try:
-                    got = bt.run().results()
-                    ok = all(np.array_equal(x, y) for x, y in zip(got, want))
-                except gpu.SemiLocalError as e:
-                    ok = False
-                    print(f"shape sched={sched} R={R} wpc={w}: {e}", flush=True)
-                if not ok:
-                    bad.append((sched, R, w))
+        bt.set_tiles(R, w)
+        try:
+            got = bt.run().results()
+            ok = all(np.array_equal(x, y) for x, y in zip(got, want))
+        except gpu.SemiLocalError as e:
+            ok = False
+            print(f"shape sched={sched} R={R} wpc={w}: {e}", flush=True)
+        if not ok:
+            bad.append((sched, R, w))
     bt.close()
     assert not bad, bad

@@ -18,7 +18,7 @@ def _shapes(seed, n):
 
 
 @pytest.mark.parametrize("rows_per_lane", [4, 8, 16])
-@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 6, 8])
 @pytest.mark.parametrize("waves_per_cu", [4, 32])
 def test_every_pass_once_in_pass_order(rows_per_lane, schedule, waves_per_cu):
     a, b = _shapes(rows_per_lane * 10 + schedule, 300)
@@ -33,6 +33,63 @@ def test_every_pass_once_in_pass_order(rows_per_lane, schedule, waves_per_cu):
     for i, p in got:                             # passes of a pair in order
         assert p == last.get(i, -1) + 1
         last[i] = p
+
+
+def _per_pair_passes(a, b, pair, pas):
+    """Passes per pair of a per-pair plan, checked to be one of the three tile heights' counts."""
+    m = np.minimum(a, b).astype(np.int64)
+    got = np.zeros(len(a), np.int64)
+    last = {}
+    for i, p in zip(pair.tolist(), pas.tolist()):   # every pass once, in pass order
+        assert p == last.get(i, -1) + 1
+        last[i] = p
+        got[i] += 1
+    for i in range(len(a)):
+        if a[i] == 0 or b[i] == 0:
+            assert got[i] == 0
+        else:
+            assert got[i] in {(m[i] + 64 * R - 1) // (64 * R) for R in (4, 8, 16)}, (i, m[i], got[i])
+    return got
+
+
+@pytest.mark.parametrize("schedule", [0, 2, 3, 6])
+@pytest.mark.parametrize("waves_per_cu", [8, 12])
+def test_per_pair_tiles_every_pass_once_in_pass_order(schedule, waves_per_cu):
+    a, b = _shapes(40 + schedule, 3000)
+    pair, pas = S.plan_queue(a, b, 0, waves_per_cu, 256, schedule)
+    _per_pair_passes(a, b, pair, pas)
+
+
+def test_per_pair_tiles_follow_padding_and_critical_chains():
+    # A large batch: each pair takes the tile height with the fewest
+    # instructions for its padded rows, and the pairs whose chain of passes
+    # would outlast the batch take the shortest chain (R = 4, most passes).
+    a, b = _shapes(7, 4096)
+    a[:3] = 50000
+    b[:3] = 50000                                # three 50,000^2 pairs: the critical chains
+    pair, pas = S.plan_queue(a, b, 0, 8, 256, 3)
+    got = _per_pair_passes(a, b, pair, pas)
+    assert list(got[:3]) == [(50000 + 255) // 256] * 3
+    m = np.minimum(a, b).astype(np.int64)
+    # rows just above a tall tile boundary are not padded to 1,024 rows
+    small = [i for i in range(3, len(a)) if 1025 <= m[i] <= 1100 and max(a[i], b[i]) < 4000]
+    assert small and all(got[i] > 1 for i in small)
+    # a pair of exactly 1,024 rows, short columns, takes one R = 16 pass
+    a2, b2 = a.copy(), b.copy()
+    a2[3], b2[3] = 1024, 1500
+    pair2, pas2 = S.plan_queue(a2, b2, 0, 8, 256, 3)
+    assert _per_pair_passes(a2, b2, pair2, pas2)[3] == 1
+
+
+def test_small_per_pair_batch_takes_one_height():
+    # too few passes to give every SIMD two waves: one height for every pair
+    a = np.array([3000, 5000, 200, 9000], np.int32)
+    b = np.array([4000, 5000, 7000, 9000], np.int32)
+    pair, pas = S.plan_queue(a, b, 0, 8, 256, 2)
+    got = _per_pair_passes(a, b, pair, pas)
+    m = np.minimum(a, b)
+    heights = [R for R in (4, 8, 16) if all(got[i] == (m[i] + 64 * R - 1) // (64 * R) for i in range(len(a)))]
+    assert heights, got
 
 
 def test_pair_by_pair_order_is_largest_first():
@@ -62,6 +119,6 @@ def test_bad_arguments_are_rejected():
     with pytest.raises(S.SemiLocalError):
         S.plan_queue([10], [10], 5)
     with pytest.raises(S.SemiLocalError):
-        S.plan_queue([10], [10], 4, schedule=4)
+        S.plan_queue([10], [10], 4, schedule=9)
     with pytest.raises(S.SemiLocalError):
         S.plan_queue([-1], [10], 4)
