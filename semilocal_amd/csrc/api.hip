@@ -21,7 +21,7 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStrea
 hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
                              const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
                              int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch, LaunchCtl *ctl,
-                             uint32_t spin_limit, uint32_t table_words);
+                             uint32_t spin_limit, uint32_t table_words, uint32_t *wait_flags);
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -267,6 +267,7 @@ struct Batch {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     PairDesc *d_pairs = nullptr;
     WorkItem *d_items = nullptr;
+    uint32_t *d_flags = nullptr;       // wait flags, one word per entry of d_items (see comb_pass)
     void *d_pool = nullptr;
     int32_t *d_rc = nullptr;
     int32_t *d_cr = nullptr;
@@ -301,6 +302,7 @@ static void batch_free(Batch *b) {
     if (!b) return;
     bt_free(b, b->d_pairs);
     bt_free(b, b->d_items);
+    bt_free(b, b->d_flags);
     bt_free(b, b->d_pool);
     bt_free(b, b->d_rc);
     bt_free(b, b->d_cr);
@@ -322,8 +324,10 @@ constexpr int kMaxSchedule = 8;
 // waves; a pass lasts n + 64 steps, each step costing in proportion to the
 // instructions of its tile height; pass p of a pair becomes ready `lag`
 // steps after pass p-1 started; a free wave takes the ready pass of the
-// largest pair.  Passes of a pair keep their order, so a pass only ever
-// waits on a pass handed out before it.  A longer lag interleaves more pairs
+// pair with the longest remaining chain (its remaining passes' lags plus one
+// pass: critical-path list scheduling, so a skinny pair of one long pass is
+// not left for the end of the launch).  Passes of a pair keep their order, so
+// a pass only ever waits on a pass handed out before it.  A longer lag interleaves more pairs
 // and leaves a pass more slack behind its predecessor (a pass that catches
 // up with a slower predecessor waits), at the price of a longer chain per
 // pair: priority pairs (the critical chains) keep the short kCriticalLag,
@@ -437,9 +441,14 @@ static void schedule_items(Batch *bt, double lag) {
     bt->order.reserve(nitems);
     if (nitems == 0) return;
     struct Ready {
-        int64_t cells;
+        double chain;   // steps (at tile height 8) from the pass's start to the end of the pair's last pass
         int pair;
-        bool operator<(const Ready &o) const { return cells < o.cells || (cells == o.cells && pair > o.pair); }
+        bool operator<(const Ready &o) const { return chain < o.chain || (chain == o.chain && pair > o.pair); }
+    };
+    auto chain = [&](int i, int done) {
+        const PairDesc &d = bt->pairs[(size_t)i];
+        const double rest = (double)(d.npasses - done - 1);
+        return ((double)d.n + 64.0 + rest * (d.prio ? kCriticalLag : lag)) * step_cost(d);
     };
     using Pending = std::pair<double, int>;   // (ready time, pair)
     std::priority_queue<Ready> ready;
@@ -447,7 +456,7 @@ static void schedule_items(Batch *bt, double lag) {
     std::priority_queue<double, std::vector<double>, std::greater<double>> waves;
     std::vector<int> next((size_t)bt->npairs, 0);
     for (int i = 0; i < bt->npairs; ++i)
-        if (bt->pairs[i].npasses > 0) ready.push(Ready{(int64_t)bt->pairs[i].m * bt->pairs[i].n, i});
+        if (bt->pairs[i].npasses > 0) ready.push(Ready{chain(i, 0), i});
     const size_t w = std::max<size_t>(1, std::min<size_t>(nitems, bt->waves_cap));
     for (size_t k = 0; k < w; ++k) waves.push(0.0);
     while (bt->order.size() < nitems) {
@@ -457,7 +466,7 @@ static void schedule_items(Batch *bt, double lag) {
         while (!pending.empty() && pending.top().first <= t) {
             const int i = pending.top().second;
             pending.pop();
-            ready.push(Ready{(int64_t)bt->pairs[i].m * bt->pairs[i].n, i});
+            ready.push(Ready{chain(i, next[i]), i});
         }
         const Ready r = ready.top();
         ready.pop();
@@ -535,9 +544,14 @@ static hipError_t upload_items(Batch *bt) {
     const size_t need = 2 * std::max(bt->items.size(), most);
     if (e == hipSuccess && (!bt->d_items || need > bt->items_cap)) {
         bt_free(bt, bt->d_items);
+        bt_free(bt, bt->d_flags);
         bt->d_items = nullptr;
+        bt->d_flags = nullptr;
         bt->items_cap = std::max<size_t>(need, 1);
         e = bt_alloc(bt, (void **)&bt->d_items, sizeof(WorkItem) * bt->items_cap);
+        if (e == hipSuccess) e = bt_alloc(bt, (void **)&bt->d_flags, sizeof(uint32_t) * bt->items_cap);
+        // new memory may hold any word: cleared before the first launch
+        if (e == hipSuccess) e = launch_fill_u32(bt->d_flags, 0u, bt->items_cap, bt->stream);
     }
     if (e == hipSuccess && !bt->items.empty())
         e = hipMemcpyAsync(bt->d_items, bt->items.data(), sizeof(WorkItem) * bt->items.size(), hipMemcpyHostToDevice,
@@ -693,7 +707,7 @@ static int batch_create(Batch **out, int npairs, const void *a_cat, const int64_
 
 // Host-side guard: every device buffer a launch touches must exist.
 static int batch_check_ready(const Batch *bt) {
-    if (!bt->d_pairs || !bt->d_items || !bt->d_pool || !bt->d_rc || !bt->d_buf || !bt->d_ctl ||
+    if (!bt->d_pairs || !bt->d_items || !bt->d_flags || !bt->d_pool || !bt->d_rc || !bt->d_buf || !bt->d_ctl ||
         (bt->with_cr && !bt->d_cr) || !bt->stream) {
         set_error("batch is missing a device buffer (internal defect); nothing launched");
         return kErrHip;
@@ -722,9 +736,11 @@ bool next_run_tag(uint64_t &cleared_at, uint32_t *tag) {
 static int batch_reset(Batch *bt) {
     const int rc = batch_check_ready(bt);
     if (rc) return rc;
-    if (next_run_tag(bt->cleared_at, &bt->epoch))   // new tags for this run's strand hand-offs
+    if (next_run_tag(bt->cleared_at, &bt->epoch)) {   // new tags for this run's strand hand-offs
         SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_buf, 0u, (uint64_t)std::max<int64_t>(bt->buf_total, 1) * 2u,
                                    bt->stream));
+        SL_HIP_TRY(launch_fill_u32(bt->d_flags, 0u, bt->items_cap, bt->stream));
+    }
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
     return kOk;
 }
@@ -738,7 +754,8 @@ static hipError_t launch_items(Batch *bt, const WorkItem *items, uint32_t nitems
     const uint32_t table_words = waves >= (unsigned)device_cus() ? bt->table_words : 0u;
     // per-pair pointers are resolved inside the kernel from PairDesc offsets
     return launch_comb_batch(bt->sym_bytes, bt->launch_rows, waves, bt->stream, bt->d_pairs, items, nitems, pool,
-                             bt->d_rc, bt->d_cr, bt->d_buf, bt->epoch, ctl, bt->spin_limit, table_words);
+                             bt->d_rc, bt->d_cr, bt->d_buf, bt->epoch, ctl, bt->spin_limit, table_words,
+                             bt->d_flags);
 }
 
 static int batch_run(Batch *bt) {
@@ -757,6 +774,7 @@ static int batch_run_replayable(Batch *bt) {
     if (rc) return rc;
     if (bt->epoch == 0) (void)next_run_tag(bt->cleared_at, &bt->epoch);
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_buf, 0u, (uint64_t)bt->buf_total * 2u, bt->stream));
+    SL_HIP_TRY(launch_fill_u32(bt->d_flags, 0u, bt->items_cap, bt->stream));
     bt->cleared_at = run_serial_now();
     SL_HIP_TRY(launch_fill_u32((uint32_t *)bt->d_ctl, 0u, (uint64_t)(bt->npairs + 1) * 2u, bt->stream));
     SL_HIP_TRY(launch_items(bt, bt->d_items + bt->items.size(), (uint32_t)bt->order.size(), bt->d_ctl));

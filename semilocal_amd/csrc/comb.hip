@@ -138,7 +138,7 @@ __device__ __forceinline__ uint32_t sym_value(uint32_t code, uint32_t tag_shift)
 template <int R, int MODE, typename Sym>
 __device__ __attribute__((always_inline)) void comb_pass(const PairDesc &d, int pass, const Sym *__restrict__ pool, int32_t *__restrict__ rc,
                           int32_t *__restrict__ cr, uint64_t *__restrict__ buf, uint32_t epoch,
-                          LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W, uint32_t *T) {
+                          LaunchCtl *ctl, uint32_t spin_limit, uint32_t *W, uint32_t *T, uint32_t *wait_flags) {
     constexpr bool ONEHOT = MODE != kCellCode;
     const int lane = threadIdx.x & 63;
     const uint32_t m = (uint32_t)d.m, n = (uint32_t)d.n, size = m + n;
@@ -197,13 +197,28 @@ __device__ __attribute__((always_inline)) void comb_pass(const PairDesc &d, int 
         }
     };
     // Bounded wait until every lane's entry of chunk j carries want_tag.
+    // While a pass waits on its predecessor it raises the predecessor's wait
+    // flag (entry item0 + pass - 1: (tag << 1) | 1), and lowers it when the
+    // wait ends; the predecessor reads its flag once per chunk and runs at
+    // raised wave priority while it is up.  A chain of passes then runs at
+    // about the speed of its average pass instead of its slowest one: waves
+    // of equal priority on a SIMD issue by wave age, so one pass of a chain
+    // can be held back for its whole length.
+    uint32_t *const my_flag = wait_flags && !first ? wait_flags + d.item0 + pass - 1 : nullptr;
+    const uint32_t *const succ_flag = wait_flags && !last ? wait_flags + d.item0 + pass : nullptr;
     auto settle_chunk = [&](uint32_t j, uint64_t &tv) -> bool {
         if (first) return true;
         const uint32_t col = 64u * j + (uint32_t)lane;
         uint32_t spins = 0;
         for (;;) {
             const bool ready = col >= n || (uint32_t)(tv >> 32) == want_tag;
-            if (__builtin_amdgcn_ballot_w64(!ready) == 0) return true;
+            if (__builtin_amdgcn_ballot_w64(!ready) == 0) {
+                if (spins && my_flag && lane == 0)
+                    __hip_atomic_store(my_flag, epoch << 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return true;
+            }
+            if (spins == 0 && my_flag && lane == 0)
+                __hip_atomic_store(my_flag, (epoch << 1) | 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (++spins >= spin_limit || launch_failed(ctl)) {
                 report_timeout(ctl);
                 return false;
@@ -217,11 +232,16 @@ __device__ __attribute__((always_inline)) void comb_pass(const PairDesc &d, int 
     uint32_t bin_nx = pad_col;
     issue_chunk(0, tv_nx, bin_nx);
     uint32_t t = kNoStrand, b_prev = pad_col;
+    uint32_t succ_waiting = 0;   // the successor's wait flag, read half a chunk ahead of its use
     bool ok = true;
     for (uint32_t j = 0; j < nchunks; ++j) {
         if (!settle_chunk(j, tv_nx)) {
             ok = false;
             break;
+        }
+        if (succ_flag && !d.prio) {
+            if (__builtin_amdgcn_readfirstlane(succ_waiting) == ((epoch << 1) | 1u)) __builtin_amdgcn_s_setprio(2);
+            else __builtin_amdgcn_s_setprio(0);
         }
         const uint32_t tin = (uint32_t)tv_nx, bin = bin_nx;
         W[kWinB + lane] = b_prev;
@@ -281,6 +301,8 @@ __device__ __attribute__((always_inline)) void comb_pass(const PairDesc &d, int 
         // read chunk j+1 half-way through chunk j: it is needed when chunk j+1
         // starts, and reading it late shortens the lag behind the previous pass
         if (j + 1u < nchunks) issue_chunk(j + 1u, tv_nx, bin_nx);
+        if (succ_flag && !d.prio)
+            succ_waiting = __hip_atomic_load(succ_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         steps(32, 64);
         wave_lds_sync();
         // slot 1+k was written last by lane 63 (step k): final top strand of column 64j+k-63
@@ -309,13 +331,14 @@ template <int R, typename Sym>
 __device__ __attribute__((always_inline)) void comb_pass_cells(const PairDesc &d, int pass, const Sym *pool,
                                                                int32_t *rc, int32_t *cr, uint64_t *buf,
                                                                uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit,
-                                                               uint32_t table_words, uint32_t *W, uint32_t *T) {
+                                                               uint32_t table_words, uint32_t *W, uint32_t *T,
+                                                               uint32_t *wait_flags) {
     if (d.cell == kCellOneHot)
-        comb_pass<R, kCellOneHot, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+        comb_pass<R, kCellOneHot, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T, wait_flags);
     else if (d.cell == kCellTable && table_words != 0)
-        comb_pass<R, kCellTable, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+        comb_pass<R, kCellTable, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T, wait_flags);
     else
-        comb_pass<R, kCellCode, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T);
+        comb_pass<R, kCellCode, Sym>(d, pass, pool, rc, cr, buf, epoch, ctl, spin_limit, W, T, wait_flags);
 }
 
 }  // namespace
@@ -326,14 +349,15 @@ __device__ __attribute__((always_inline)) void comb_pass_cells(const PairDesc &d
 // (d.rows_per_lane, any of 4, 8, 16), dispatched per ticket like the cell
 // mode.  A pair marked `prio` (a chain of passes on the batch's critical path)
 // runs its passes at raised wave priority, so on a SIMD shared with bulk
-// passes its instructions issue first.
+// passes its instructions issue first.  wait_flags (one word per queue item,
+// or null) carry the wait feedback between consecutive passes of a pair.
 template <typename Sym, int RSET>
 __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restrict__ pairs,
                                                          const WorkItem *__restrict__ items, uint32_t nitems,
                                                          const Sym *__restrict__ pool, int32_t *__restrict__ rc,
                                                          int32_t *__restrict__ cr, uint64_t *__restrict__ buf,
                                                          uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit,
-                                                         uint32_t table_words) {
+                                                         uint32_t table_words, uint32_t *wait_flags) {
     extern __shared__ uint32_t lds[];   // kWavesPerBlock x (kWaveLds + table_words) words
     const int lane = threadIdx.x & 63;
     uint32_t *W = lds + (threadIdx.x >> 6) * (kWaveLds + table_words);   // this wave's private staging area
@@ -354,12 +378,15 @@ __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restr
         }
         if (d.prio) __builtin_amdgcn_s_setprio(3);
         if (RSET == 4 || (RSET == 0 && R == 4))
-            comb_pass_cells<4, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T);
+            comb_pass_cells<4, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T,
+                                    wait_flags);
         else if (RSET == 8 || (RSET == 0 && R == 8))
-            comb_pass_cells<8, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T);
+            comb_pass_cells<8, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T,
+                                    wait_flags);
         else
-            comb_pass_cells<16, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T);
-        if (d.prio) __builtin_amdgcn_s_setprio(0);
+            comb_pass_cells<16, Sym>(d, it.pass, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, W, T,
+                                     wait_flags);
+        __builtin_amdgcn_s_setprio(0);
     }
 }
 
@@ -379,10 +406,11 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStrea
 template <typename Sym, int RSET>
 static void launch_one(unsigned blocks, hipStream_t stream, const PairDesc *pairs, const WorkItem *items,
                        uint32_t nitems, const void *pool, int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch,
-                       LaunchCtl *ctl, uint32_t spin_limit, uint32_t table_words) {
+                       LaunchCtl *ctl, uint32_t spin_limit, uint32_t table_words, uint32_t *wait_flags) {
     const size_t lds_bytes = sizeof(uint32_t) * kWavesPerBlock * (kWaveLds + table_words);
     hipLaunchKernelGGL((comb_batch_kernel<Sym, RSET>), dim3(blocks), dim3(kWavesPerBlock * 64), lds_bytes, stream, pairs,
-                       items, nitems, (const Sym *)pool, rc, cr, buf, epoch, ctl, spin_limit, table_words);
+                       items, nitems, (const Sym *)pool, rc, cr, buf, epoch, ctl, spin_limit, table_words,
+                       wait_flags);
 }
 
 // rows_per_lane: 4, 8 or 16 when every pair of the launch has that tile
@@ -393,12 +421,13 @@ static void launch_one(unsigned blocks, hipStream_t stream, const PairDesc *pair
 hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
                              const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
                              int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch, LaunchCtl *ctl,
-                             uint32_t spin_limit, uint32_t table_words) {
+                             uint32_t spin_limit, uint32_t table_words, uint32_t *wait_flags) {
     if (nitems == 0 || waves == 0) return hipSuccess;
     if (table_words > (uint32_t)(kTableMaxSymbols + 1) * 64u) return hipErrorInvalidValue;
     const unsigned blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
 #define SL_LAUNCH(SYM, RR) \
-    launch_one<SYM, RR>(blocks, stream, pairs, items, nitems, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words)
+    launch_one<SYM, RR>(blocks, stream, pairs, items, nitems, pool, rc, cr, buf, epoch, ctl, spin_limit, table_words, \
+                        wait_flags)
     if (rows_per_lane != 0 && rows_per_lane != 4 && rows_per_lane != 8 && rows_per_lane != 16)
         return hipErrorInvalidValue;
     if (sym_bytes == 1) {

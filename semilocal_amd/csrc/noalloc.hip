@@ -25,7 +25,7 @@ namespace slhip {
 hipError_t launch_comb_batch(int sym_bytes, int rows_per_lane, unsigned waves, hipStream_t stream,
                              const PairDesc *pairs, const WorkItem *items, uint32_t nitems, const void *pool,
                              int32_t *rc, int32_t *cr, uint64_t *buf, uint32_t epoch, LaunchCtl *ctl,
-                             uint32_t spin_limit, uint32_t table_words);
+                             uint32_t spin_limit, uint32_t table_words, uint32_t *wait_flags);
 int model_rows_one_pair(int32_t m, int32_t n, int cus);
 
 namespace {
@@ -166,7 +166,7 @@ int sl_semi_local_comb_noalloc(const void *d_a, int m, const void *d_b, int n, i
     SL_HIP_TRY(launch_comb_batch(sym_bytes, R, waves, st, reinterpret_cast<const PairDesc *>(s + kOffPair),
                                  reinterpret_cast<const WorkItem *>(s + kOffItems), (uint32_t)d.npasses,
                                  reinterpret_cast<const void *>(base), d_row_to_col, d_col_to_row,
-                                 reinterpret_cast<uint64_t *>(s + kOffBuf), 1u, ctl, 1u << 23, 0u));
+                                 reinterpret_cast<uint64_t *>(s + kOffBuf), 1u, ctl, 1u << 23, 0u, nullptr));
     LaunchCtl h{};
     SL_HIP_TRY(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
     SL_HIP_TRY(hipStreamSynchronize(st));

@@ -104,15 +104,31 @@ def test_pair_by_pair_order_is_largest_first():
         assert np.all(np.diff(idx) == 1)
 
 
-def test_simulated_order_starts_with_the_largest_pair_and_interleaves():
+def test_simulated_order_starts_with_the_longest_chain_and_interleaves():
+    # critical-path list scheduling: the first ticket is the first pass of the
+    # pair whose chain (n + 64 steps for its last pass, plus the lag of 256
+    # steps at schedule 2 before each later pass) is the longest
     a, b = _shapes(6, 200)
-    cells = a.astype(np.int64) * b
+    m = np.minimum(a, b).astype(np.int64)
+    n = np.maximum(a, b).astype(np.int64)
+    passes = np.where((a > 0) & (b > 0), (m + 255) // 256, 0)
+    chain = np.where(passes > 0, n + 64 + (passes - 1) * 256, -1)
     pair, pas = S.plan_queue(a, b, 4, 8, 256, 2)
-    assert pair[0] == int(np.argmax(cells)) and pas[0] == 0
+    assert pair[0] == int(np.argmax(chain)) and pas[0] == 0
     # a large pair's passes are spread over the queue, not handed out back to back
-    big = int(np.argmax(cells))
+    big = int(np.argmax(chain))
     idx = np.nonzero(pair == big)[0]
     assert len(idx) > 1 and np.any(np.diff(idx) > 1)
+
+
+def test_a_skinny_pair_of_one_long_pass_is_not_left_for_the_end():
+    # 600 x 40,000 has few cells but one 40,000-step pass: it starts before
+    # the many short pairs of more cells
+    a = np.full(2000, 3000, np.int32)
+    b = np.full(2000, 3000, np.int32)
+    a[1000], b[1000] = 600, 40000
+    pair, pas = S.plan_queue(a, b, 8, 8, 256, 2)
+    assert int(np.nonzero(pair == 1000)[0][0]) == 0
 
 
 def test_bad_arguments_are_rejected():
