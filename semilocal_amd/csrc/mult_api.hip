@@ -259,13 +259,12 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
     if (p->ntiles == 0) return kOk;
     const int nt = p->ntiles;
     const int64_t ntot = p->ntot;
-    const int eb = (int)ceil_div(ntot, 256);
     int cur = 0;
     for (size_t li = 0; li < p->lv.size(); ++li) {
         const Level &L = p->lv[li];
-        hipLaunchKernelGGL(split_count, dim3(nt), dim3(kTile), 0, s, p->d_P[cur], p->d_Qi[cur], p->d_tile_prob,
-                           p->d_probs, L.S, p->d_cntP, p->d_cntQ);
-        hipLaunchKernelGGL(split_scatter, dim3(nt), dim3(kTile), 0, s, p->d_P[cur], p->d_Qi[cur], p->d_tile_prob,
+        hipLaunchKernelGGL(split_count, dim3(nt), dim3(kSplitThreads), 0, s, p->d_P[cur], p->d_Qi[cur],
+                           p->d_tile_prob, p->d_probs, L.S, p->d_cntP, p->d_cntQ);
+        hipLaunchKernelGGL(split_scatter, dim3(nt), dim3(kSplitThreads), 0, s, p->d_P[cur], p->d_Qi[cur], p->d_tile_prob,
                            p->d_probs, L.S, p->d_cntP, p->d_cntQ, p->d_P[cur ^ 1], p->d_Qi[cur ^ 1],
                            p->d_PR + li * ntot, p->d_QC + li * ntot);
         cur ^= 1;
@@ -282,7 +281,7 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
         const SaSeg *segs = p->d_segs + L.seg0;
         const int2 *walks = p->d_walks + L.walk0;
         int32_t *cend = p->d_cend + L.seg0;
-        hipLaunchKernelGGL(merge_map, dim3(eb), dim3(256), 0, s, p->d_R[rc], p->d_PR + k * ntot,
+        hipLaunchKernelGGL(merge_map, dim3(nt), dim3(kSplitThreads), 0, s, p->d_R[rc], p->d_PR + k * ntot,
                            p->d_QC + k * ntot, p->d_tile_prob, p->d_probs, p->d_seg_first + L.sf0, L.S, ntot,
                            p->d_U, p->d_Ui, p->d_R[rc ^ 1], cend);
         rc ^= 1;

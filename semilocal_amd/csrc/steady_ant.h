@@ -14,6 +14,8 @@ constexpr int kWalkBlock = 64;   // lanes (segments) per block of the large-leve
 constexpr int kWalkWindow = 16;  // rows and columns a large-level walk reads ahead per round trip
 constexpr int kWalkGroups = kWalkWindow / 4 + 1;   // aligned 4-entry groups covering one window side
 constexpr int kWalkSlack = 64;   // entries allocated past the end of U and U^-1 for those groups
+constexpr int kSplitPer = 4;     // consecutive elements per thread of the large-level split kernels
+constexpr int kSplitThreads = kTile / kSplitPer;   // their block size (one tile per block)
 constexpr int kMaxNb = 4096;     // count-table blocks per side: subproblems up to kMaxNb * kBlk
 
 // One product of the batch: elements [base, base + n) of every level array.

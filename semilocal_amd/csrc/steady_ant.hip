@@ -115,24 +115,38 @@ __global__ __launch_bounds__(kTile) void store_r(const int32_t *__restrict__ R, 
 // ---------------------------------------------------------------------------
 // Split, large levels (segment size S > kTile; a tile lies in one segment).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kTile) void split_count(const int32_t *__restrict__ P, const int32_t *__restrict__ Qi,
-                                                     const int32_t *__restrict__ tile_prob,
-                                                     const SaProb *__restrict__ probs, int S,
-                                                     int32_t *__restrict__ cntP, int32_t *__restrict__ cntQ) {
-    __shared__ int tmp[kTile / 64 + 1];
+// A tile of kTile elements per block of kSplitThreads threads, kSplitPer
+// consecutive elements per thread (one 16-byte load of P and of Qi each):
+// a quarter of the waves of one element per thread, for kernels whose time
+// is mostly the dispatch of their waves.  The level arrays hold whole tiles,
+// so the loads stay inside them; elements past a problem's end are masked.
+static_assert(kSplitPer == 4, "one int4 load per array and thread");
+
+__global__ __launch_bounds__(kSplitThreads) void split_count(const int32_t *__restrict__ P,
+                                                             const int32_t *__restrict__ Qi,
+                                                             const int32_t *__restrict__ tile_prob,
+                                                             const SaProb *__restrict__ probs, int S,
+                                                             int32_t *__restrict__ cntP, int32_t *__restrict__ cntQ) {
+    __shared__ int red[kSplitThreads / 64];
     const int t = blockIdx.x;
     const SaProb pb = probs[tile_prob[t]];
-    const int64_t x = (int64_t)t * kTile + threadIdx.x;
-    const bool in = x - pb.base < pb.n;
+    const int64_t x0 = (int64_t)t * kTile + kSplitPer * threadIdx.x;
+    const int64_t lx0 = x0 - pb.base;
     const int h = S >> 1;
-    int fp = 0, fq = 0;
-    if (in) {
-        fp = P[x] < h;
-        fq = Qi[x] < h;
-    }
-    int tpq;   // both counts in one scan: each is at most kTile < 2^16
-    (void)block_excl_scan(fp | (fq << 16), tmp, tpq);
+    const int4 p4 = *reinterpret_cast<const int4 *>(P + x0);
+    const int4 q4 = *reinterpret_cast<const int4 *>(Qi + x0);
+    const int pv[kSplitPer] = {p4.x, p4.y, p4.z, p4.w}, qv[kSplitPer] = {q4.x, q4.y, q4.z, q4.w};
+    int v = 0;   // both counts in one word: each is at most kTile < 2^16
+#pragma unroll
+    for (int e = 0; e < kSplitPer; ++e)
+        if (lx0 + e < pb.n) v += (pv[e] < h) | ((qv[e] < h) << 16);
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        int tpq = 0;
+#pragma unroll
+        for (int w = 0; w < kSplitThreads / 64; ++w) tpq += red[w];
         cntP[t] = tpq & 0xFFFF;
         cntQ[t] = tpq >> 16;
     }
@@ -142,26 +156,26 @@ __global__ __launch_bounds__(kTile) void split_count(const int32_t *__restrict__
 // coalesced read of at most S / kTile counts and a block reduction): the
 // tile's exclusive offsets, without a separate scan launch.  Segments start
 // on tile boundaries (problem bases and S are multiples of kTile).
-__global__ __launch_bounds__(kTile) void split_scatter(const int32_t *__restrict__ P, const int32_t *__restrict__ Qi,
-                                                       const int32_t *__restrict__ tile_prob,
-                                                       const SaProb *__restrict__ probs, int S,
-                                                       const int32_t *__restrict__ cntP,
-                                                       const int32_t *__restrict__ cntQ, int32_t *__restrict__ P1,
-                                                       int32_t *__restrict__ Qi1, int32_t *__restrict__ PR1,
-                                                       int32_t *__restrict__ QC1) {
-    __shared__ int tmp[kTile / 64 + 1];
-    __shared__ int red[2 * (kTile / 64)];
+__global__ __launch_bounds__(kSplitThreads) void split_scatter(const int32_t *__restrict__ P,
+                                                               const int32_t *__restrict__ Qi,
+                                                               const int32_t *__restrict__ tile_prob,
+                                                               const SaProb *__restrict__ probs, int S,
+                                                               const int32_t *__restrict__ cntP,
+                                                               const int32_t *__restrict__ cntQ,
+                                                               int32_t *__restrict__ P1, int32_t *__restrict__ Qi1,
+                                                               int32_t *__restrict__ PR1, int32_t *__restrict__ QC1) {
+    __shared__ int tmp[kSplitThreads / 64 + 1];
+    __shared__ int red[2 * (kSplitThreads / 64)];
     const int t = blockIdx.x;
     const SaProb pb = probs[tile_prob[t]];
-    const int64_t x = (int64_t)t * kTile + threadIdx.x;
-    const int64_t lx = x - pb.base;
-    const bool in = lx < pb.n;
+    const int64_t x0 = (int64_t)t * kTile + kSplitPer * threadIdx.x;
+    const int64_t lx0 = x0 - pb.base;
     int offp = 0, offq = 0;
     {
-        const int64_t lx0 = (int64_t)t * kTile - pb.base;
-        const int64_t tile0 = (pb.base + (lx0 / S) * S) / kTile;
+        const int64_t lt0 = (int64_t)t * kTile - pb.base;
+        const int64_t tile0 = (pb.base + (lt0 / S) * S) / kTile;
         int vp = 0, vq = 0;
-        for (int64_t u = tile0 + threadIdx.x; u < t; u += kTile) {
+        for (int64_t u = tile0 + threadIdx.x; u < t; u += kSplitThreads) {
             vp += cntP[u];
             vq += cntQ[u];
         }
@@ -173,33 +187,41 @@ __global__ __launch_bounds__(kTile) void split_scatter(const int32_t *__restrict
         }
         __syncthreads();
 #pragma unroll
-        for (int w = 0; w < kTile / 64; ++w) {
+        for (int w = 0; w < kSplitThreads / 64; ++w) {
             offp += red[2 * w];
             offq += red[2 * w + 1];
         }
     }
     const int h = S >> 1;
-    int fp = 0, fq = 0, p = 0, q = 0;
-    if (in) {
-        p = P[x];
-        q = Qi[x];
-        fp = p < h;
-        fq = q < h;
+    const int4 p4 = *reinterpret_cast<const int4 *>(P + x0);
+    const int4 q4 = *reinterpret_cast<const int4 *>(Qi + x0);
+    const int pv[kSplitPer] = {p4.x, p4.y, p4.z, p4.w}, qv[kSplitPer] = {q4.x, q4.y, q4.z, q4.w};
+    int f[kSplitPer];
+    int mine = 0;   // this thread's flags, both counts in one word (each < 2^16)
+#pragma unroll
+    for (int e = 0; e < kSplitPer; ++e) {
+        f[e] = lx0 + e < pb.n ? (pv[e] < h) | ((qv[e] < h) << 16) : 0;
+        mine += f[e];
     }
-    int tpq;   // both flags in one scan: each count is at most kTile < 2^16
-    const int ex = block_excl_scan(fp | (fq << 16), tmp, tpq);
-    const int exP = (ex & 0xFFFF) + offp;
-    const int exQ = (ex >> 16) + offq;
-    if (in) {
-        const int64_t seg0 = pb.base + (lx / S) * S;
-        const int r = (int)(lx % S);
-        // lo elements keep their order at the front, hi elements from position h
-        const int64_t dp = fp ? seg0 + exP : seg0 + h + (r - exP);
-        const int64_t dq = fq ? seg0 + exQ : seg0 + h + (r - exQ);
-        P1[dp] = fp ? p : p - h;
-        PR1[dp] = r;
-        Qi1[dq] = fq ? q : q - h;
-        QC1[dq] = r;
+    int tpq;
+    int ex = block_excl_scan(mine, tmp, tpq);
+    const int64_t seg0 = pb.base + (lx0 / S) * S;   // a tile, so a thread's elements, lie in one segment
+    const int r0 = (int)(lx0 % S);
+#pragma unroll
+    for (int e = 0; e < kSplitPer; ++e) {
+        if (lx0 + e < pb.n) {
+            const int exP = (ex & 0xFFFF) + offp, exQ = (ex >> 16) + offq;
+            const int fp = f[e] & 1, fq = f[e] >> 16;
+            const int r = r0 + e;
+            // lo elements keep their order at the front, hi elements from position h
+            const int64_t dp = fp ? seg0 + exP : seg0 + h + (r - exP);
+            const int64_t dq = fq ? seg0 + exQ : seg0 + h + (r - exQ);
+            P1[dp] = fp ? pv[e] : pv[e] - h;
+            PR1[dp] = r;
+            Qi1[dq] = fq ? qv[e] : qv[e] - h;
+            QC1[dq] = r;
+        }
+        ex += f[e];
     }
 }
 
@@ -579,48 +601,56 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
 // Children (level S/2) -> parent (level S): U[parent row] = parent col | hi
 // bit, Ui[parent col] = parent row | hi bit, R = U without the bit, and
 // c_end[seg] = 1 + largest column of a lo point (one atomic per block: a
-// block of 256 elements lies in one subproblem).
-__global__ __launch_bounds__(256) void merge_map(const int32_t *__restrict__ Rc, const int32_t *__restrict__ PRc,
-                                                 const int32_t *__restrict__ QCc,
-                                                 const int32_t *__restrict__ tile_prob,
-                                                 const SaProb *__restrict__ probs,
-                                                 const int32_t *__restrict__ seg_first, int S, int64_t ntot,
-                                                 uint32_t *__restrict__ U, uint32_t *__restrict__ Ui,
-                                                 int32_t *__restrict__ R, int32_t *__restrict__ c_end) {
-    __shared__ int wmax[4];
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // ntot is a multiple of 256
-    const int pid = tile_prob[x / kTile];
+// block's 1,024 elements, a tile, lie in one subproblem).  kSplitPer
+// consecutive elements per thread, as in the split kernels.
+__global__ __launch_bounds__(kSplitThreads) void merge_map(const int32_t *__restrict__ Rc,
+                                                           const int32_t *__restrict__ PRc,
+                                                           const int32_t *__restrict__ QCc,
+                                                           const int32_t *__restrict__ tile_prob,
+                                                           const SaProb *__restrict__ probs,
+                                                           const int32_t *__restrict__ seg_first, int S, int64_t ntot,
+                                                           uint32_t *__restrict__ U, uint32_t *__restrict__ Ui,
+                                                           int32_t *__restrict__ R, int32_t *__restrict__ c_end) {
+    __shared__ int wmax[kSplitThreads / 64];
+    (void)ntot;
+    const int t = blockIdx.x;
+    const int pid = tile_prob[t];
     const SaProb pb = probs[pid];
-    const int64_t lx = x - pb.base;
+    const int64_t x0 = (int64_t)t * kTile + kSplitPer * threadIdx.x;
+    const int64_t lx0 = x0 - pb.base;
+    const int h = S >> 1;
+    const int4 r4 = *reinterpret_cast<const int4 *>(Rc + x0);
+    const int4 pr4 = *reinterpret_cast<const int4 *>(PRc + x0);
+    const int rv[kSplitPer] = {r4.x, r4.y, r4.z, r4.w}, prv[kSplitPer] = {pr4.x, pr4.y, pr4.z, pr4.w};
     int cand = 0;
-    if (lx < pb.n) {
-        const int h = S >> 1;
-        const int64_t cid = lx / h;
-        const uint32_t bit = (cid & 1) ? kHiBit : 0u;
-        const int64_t seg0 = pb.base + (lx / S) * S;
-        const int prow = PRc[x];
-        const int pcol = QCc[pb.base + cid * h + Rc[x]];
-        U[seg0 + prow] = (uint32_t)pcol | bit;
-        Ui[seg0 + pcol] = (uint32_t)prow | bit;
-        R[seg0 + prow] = pcol;
-        if (!bit) cand = pcol + 1;
+#pragma unroll
+    for (int e = 0; e < kSplitPer; ++e) {
+        const int64_t lx = lx0 + e;
+        if (lx < pb.n) {
+            const int64_t cid = lx / h;
+            const uint32_t bit = (cid & 1) ? kHiBit : 0u;
+            const int64_t seg0 = pb.base + (lx / S) * S;
+            const int prow = prv[e];
+            const int pcol = QCc[pb.base + cid * h + rv[e]];
+            U[seg0 + prow] = (uint32_t)pcol | bit;
+            Ui[seg0 + pcol] = (uint32_t)prow | bit;
+            R[seg0 + prow] = pcol;
+            if (!bit) cand = max(cand, pcol + 1);
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cand = max(cand, __shfl_xor(cand, off));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = cand;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int m = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-        const int64_t lx0 = (int64_t)blockIdx.x * blockDim.x - pb.base;
-        if (m > 0) atomicMax(&c_end[seg_first[pid] + lx0 / S], m);
+        int m = 0;
+#pragma unroll
+        for (int w = 0; w < kSplitThreads / 64; ++w) m = max(m, wmax[w]);
+        const int64_t lt0 = (int64_t)t * kTile - pb.base;
+        if (m > 0) atomicMax(&c_end[seg_first[pid] + lt0 / S], m);
     }
 }
 
-// Count tables of a split subproblem, row pass: one block per (subproblem,
-// block-row i).  The points of rows [i kBlk, (i+1) kBlk) are binned by
-// column block in LDS; along block-columns j, hi counts get an exclusive
-// prefix and lo counts an inclusive suffix, written transposed:
-// T[seg][j * (nb+1) + i], j in [0, nb].
 __global__ __launch_bounds__(256) void table_rows(const uint32_t *__restrict__ U, const SaSeg *__restrict__ segs,
                                                   const int2 *__restrict__ rows, int32_t *__restrict__ cntHi,
                                                   int32_t *__restrict__ cntLo) {
