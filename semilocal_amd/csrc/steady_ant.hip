@@ -392,8 +392,8 @@ __device__ __forceinline__ uint32_t widen16(uint16_t x) {
 __device__ __forceinline__ int wave_delta_lds(const uint16_t *U16, int s, int r, int c) {
     const int lane = threadIdx.x & 63;
     int acc = 0;
-#pragma unroll
-    for (int k = 0; k < kTile / 64; ++k) {
+    const int rounds = (s + 63) >> 6;   // uniform: the subproblem's own rows only
+    for (int k = 0; k < rounds; ++k) {
         const int rho = lane + 64 * k;
         const uint32_t u = U16[rho < s ? rho : 0];
         const int col = (int)(u & 0x7FFFu);
