@@ -472,6 +472,15 @@ def bench_hyyro(S, W, pairs=None):
     hb = S.HyyroBatch(pairs)
     create_s = time.perf_counter() - t0
     scores = hb.run()
+    default_ms = hb.time(warmup=1, iters=5)
+    # words per lane per length bucket: the autotuner times every offered
+    # form on each bucket's own pairs and keeps the fastest
+    t0 = time.perf_counter()
+    hb.autotune(rounds=3)
+    tune_s = time.perf_counter() - t0
+    buckets = hb.buckets()
+    if not np.array_equal(hb.run(), scores):
+        raise RuntimeError("hyyro: scores changed with the autotuned words per lane")
     ms = hb.time(warmup=1, iters=5)
     st = hb.stats()
     hb.close()
@@ -480,6 +489,9 @@ def bench_hyyro(S, W, pairs=None):
     a = g.integers(0, 4, 20_000).astype(np.uint8)
     b = g.integers(0, 4, 1_000_000).astype(np.uint8)
     one = S.HyyroBatch([(a, b)])
+    one_default_ms = one.time(warmup=1, iters=3)
+    one.autotune(rounds=2)
+    one_wpl = [bk["words_per_lane"] for bk in one.buckets() if bk["pairs"]]
     one_ms = one.time(warmup=1, iters=3)
     one_st = one.stats()
     one.close()
@@ -488,6 +500,9 @@ def bench_hyyro(S, W, pairs=None):
     short = [(g.integers(0, 20, 2000).astype(np.uint8), g.integers(0, 20, 2000).astype(np.uint8))
              for _ in range(4096)]
     hs = S.HyyroBatch(short)
+    short_default_ms = hs.time(warmup=1, iters=5)
+    hs.autotune(rounds=3)
+    short_wpl = [bk["words_per_lane"] for bk in hs.buckets() if bk["pairs"]]
     short_ms = hs.time(warmup=1, iters=5)
     short_llcs = hs.run()
     hs.close()
@@ -499,13 +514,18 @@ def bench_hyyro(S, W, pairs=None):
     return {"workload": "LLCS of the ragged batch's 4096 pairs (lengths log-normal, alphabet 20); one 20,000 x "
                         "1,000,000 pair (alphabet 4); 4096 pairs of 2000 x 2000 (alphabet 20)",
             "batch_create_s_incl_recode_upload_tables": round(create_s, 3),
+            "batch_words_per_lane_by_bucket": [{k: bk[k] for k in ("max_words", "pairs", "words_per_lane", "waves")}
+                                               for bk in buckets],
+            "batch_autotune_s": round(tune_s, 2), "batch_default_wpl_ms": round(default_ms, 3),
             "batch_ms": round(ms, 3), "batch_cells": cells, "batch_gcups": round(cells / (ms * 1e-3) / 1e9, 1),
             "batch_word_rows": st["word_rows"],
             "batch_word_rows_per_s": round(st["word_rows"] / (ms * 1e-3), 1),
             "batch_roofline": valu_roofline(st["word_rows"] * HYYRO_OPS_PER_WORD_ROW, ms, "8 per 64-bit word-row"),
             "batch_llcs_sum": int(scores.sum()),
+            "long_pair_words_per_lane": one_wpl, "long_pair_default_wpl_ms": round(one_default_ms, 2),
             "long_pair_ms": round(one_ms, 2), "long_pair_gcups": round(2e10 / (one_ms * 1e-3) / 1e9, 1),
             "long_pair_word_rows": one_st["word_rows"],
+            "short_pairs_words_per_lane": short_wpl, "short_pairs_default_wpl_ms": round(short_default_ms, 3),
             "short_pairs_4096x2000x2000_ms": round(short_ms, 3),
             "short_pairs_gcups": round(short_cells / (short_ms * 1e-3) / 1e9, 1),
             "short_pairs_semi_local_batch_ms": round(sb_ms, 3),

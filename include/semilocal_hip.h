@@ -399,6 +399,25 @@ int sl_hyyro_time(sl_hyyro *h, int warmup, int iters, double *ms_per_run);
 /* Work: sum over pairs of rows x 64-bit words (after dropping rows whose
  * symbol is absent from the packed string), and the mask-table words. */
 int sl_hyyro_stats(const sl_hyyro *h, int64_t *word_rows, int64_t *table_words);
+/* Length buckets by packed width (64-bit words of the longer string): up to */
+/* 64, 128, 256, 2048 and 16384 words.  Each bucket runs one kernel form,    */
+/* chosen by its words per lane (the reference's analogue is the word type   */
+/* and threads of hyyro_magic_mpi_with_precalc, one per call):              */
+/*   bucket 0: 1 or 2; 1: 1 (2 waves) or 2 (1 wave); 2: 1, 2 or 4 (4, 2, 1   */
+/*   waves); 3 and 4: 1 or 2 (segments of 256 x words-per-lane words).       */
+/* Arrays of SL_HYYRO_BUCKETS entries; any pointer may be null.  waves[b] is */
+/* per pair (buckets 0-2) or per segment (3-4).                              */
+#define SL_HYYRO_BUCKETS 5
+int sl_hyyro_buckets(const sl_hyyro *h, int32_t *max_words, int32_t *pairs, int32_t *words_per_lane,
+                     int32_t *waves);
+/* Forces the words per lane per bucket (0 keeps a bucket's choice); a value */
+/* the bucket does not offer is SL_ERR_ARG.                                  */
+int sl_hyyro_set_words_per_lane(sl_hyyro *h, const int32_t *words_per_lane);
+/* Autotuner: times every offered form on each bucket's own pairs (rounds    */
+/* rounds of two timed launches after a warm-up) and keeps the fastest;      */
+/* words_per_lane (optional) receives the choices.  Results never depend on  */
+/* the choice.                                                               */
+int sl_hyyro_autotune(sl_hyyro *h, int rounds, int32_t *words_per_lane);
 void sl_hyyro_destroy(sl_hyyro *h);
 
 /* ------------------------------------------------------------------------ */

@@ -80,6 +80,9 @@ SIGNATURES = {
     "sl_hyyro_result": ([_vp, _i32p], C.c_int),
     "sl_hyyro_time": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "sl_hyyro_stats": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "sl_hyyro_buckets": ([_vp, _i32p, _i32p, _i32p, _i32p], C.c_int),
+    "sl_hyyro_set_words_per_lane": ([_vp, _i32p], C.c_int),
+    "sl_hyyro_autotune": ([_vp, C.c_int, _i32p], C.c_int),
     "sl_hyyro_destroy": ([_vp], None),
     "sl_amatch_create": ([C.POINTER(_vp), _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "sl_amatch_stats": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)], C.c_int),
@@ -325,6 +328,28 @@ class HyyroBatch:
         wr, tw = C.c_int64(), C.c_int64()
         check(lib().sl_hyyro_stats(self.h, C.byref(wr), C.byref(tw)))
         return {"word_rows": wr.value, "table_words": tw.value}
+
+    HYYRO_BUCKETS = 5
+
+    def buckets(self) -> list:
+        """Per length bucket (packed width up to max_words): pairs, words per
+        lane of the kernel it runs, and waves per pair or segment."""
+        n = self.HYYRO_BUCKETS
+        mw, pr, wpl, wv = (np.zeros(n, np.int32) for _ in range(4))
+        check(lib().sl_hyyro_buckets(self.h, mw, pr, wpl, wv))
+        return [{"max_words": int(mw[i]), "pairs": int(pr[i]), "words_per_lane": int(wpl[i]),
+                 "waves": int(wv[i])} for i in range(n)]
+
+    def set_words_per_lane(self, wpl) -> None:
+        arr = np.ascontiguousarray(wpl, np.int32)
+        if arr.shape != (self.HYYRO_BUCKETS,):
+            raise ValueError("one words-per-lane value per bucket")
+        check(lib().sl_hyyro_set_words_per_lane(self.h, arr))
+
+    def autotune(self, rounds: int = 3) -> list:
+        out = np.zeros(self.HYYRO_BUCKETS, np.int32)
+        check(lib().sl_hyyro_autotune(self.h, rounds, out))
+        return [int(x) for x in out]
 
     def close(self):
         if getattr(self, "h", None):

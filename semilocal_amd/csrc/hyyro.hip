@@ -453,11 +453,14 @@ struct Hyyro {
     hipStream_t stream = nullptr;
     std::vector<hy::HyPair> pairs;
     std::vector<std::vector<int32_t>> cls_list;   // launch class -> pair indices
+    std::vector<int> wpl;                         // launch class -> words per lane of its kernel
     hy::HyPair *d_pairs = nullptr;
     int32_t *d_list = nullptr;          // the one-workgroup classes' lists, concatenated
     std::vector<int64_t> list_off;
-    // segmented classes: (pair, segment) tickets, exchange buffer, control
-    std::vector<std::vector<hy::SegItem>> seg_items;
+    // segmented classes: (pair, segment) tickets for each segment width
+    // (1 or 2 words per lane), exchange buffer (sized for the narrower
+    // segments, which are more), control
+    std::vector<std::vector<hy::SegItem>> seg_items;   // [class - first][R - 1]
     hy::SegItem *d_items = nullptr;
     std::vector<int64_t> item_off;
     int64_t *d_xoff = nullptr;          // per pair: its segments' exchange entries in d_xbuf
@@ -483,26 +486,47 @@ struct Hyyro {
 
 namespace {
 
-// Launch classes by packed width K (words).  Up to 64 words: one wave of
-// the lookahead kernel (hy_rows), the faster form for short pairs; up to 256:
-// one workgroup of the systolic kernel (hy_sys) on 2 or 4 waves.  Wider pairs
-// (up to 16,384 words) are cut into segments of 256 R words on workgroups of
-// their own (hy_seg, R = 1 up to 2,048 words, R = 2 beyond), so a long pair
-// spreads over as many compute units as it has segments.  (A form with 4
-// words per lane would halve the segments but double each lane's step, so it
-// is no faster for any row count; it is not built.)  Class 5 (one 16-wave workgroup of
-// hy_sys with 16 words per lane) holds no pairs with these limits.
+// Launch classes (length buckets) by packed width K (words): up to 64, 128,
+// 256, 2,048 and 16,384 words (2^20 columns).  Each class offers kernels that
+// differ in words per lane (R), and the batch runs one of them per class:
+// the default below, or the one sl_hyyro_autotune measured fastest on the
+// batch's own pairs.
+//  * up to 64 words: one wave of the lookahead kernel (hy_rows), R = 1 or 2;
+//  * up to 128: the systolic kernel (hy_sys) on 2 waves (R = 1), or the
+//    lookahead kernel on one wave (R = 2);
+//  * up to 256: hy_sys on 4 waves (R = 1) or 2 waves (R = 2), or the
+//    lookahead kernel on one wave (R = 4);
+//  * wider pairs are cut into segments of 256 R words on workgroups of their
+//    own (hy_seg, R = 1 or 2), so a long pair spreads over as many compute
+//    units as it has segments.
 int hy_class(int K) {
-    static const int limits[] = {64, 128, 256, 2048, 16384, 16384};
+    static const int limits[] = {64, 128, 256, 2048, 16384};
     int c = 0;
     while (K > limits[c]) ++c;
     return c;
 }
-constexpr int kClasses = 6;
+constexpr int kClasses = 5;
+constexpr int kClassMaxWords[kClasses] = {64, 128, 256, 2048, 16384};
 constexpr int kFirstSegClass = 3;
 constexpr int kSegClasses = 2;   // classes 3 and 4
 bool is_seg_class(int cls) { return cls >= kFirstSegClass && cls < kFirstSegClass + kSegClasses; }
-int seg_R(int cls) { return cls == 3 ? 1 : 2; }
+// words per lane each class offers, default first
+const std::vector<int> &class_wpl_options(int cls) {
+    static const std::vector<int> opts[kClasses] = {{1, 2}, {1, 2}, {1, 2, 4}, {1, 2}, {2, 1}};
+    return opts[cls];
+}
+bool class_offers(int cls, int R) {
+    for (int r : class_wpl_options(cls))
+        if (r == R) return true;
+    return false;
+}
+// waves per pair (one-workgroup classes) or per segment (segmented ones)
+int class_waves(int cls, int R) {
+    if (is_seg_class(cls)) return 4;   // hy::kSegWaves
+    if (cls == 0) return 1;
+    if (cls == 1) return R == 1 ? 2 : 1;
+    return R == 1 ? 4 : (R == 2 ? 2 : 1);
+}
 
 void hy_free(Hyyro *h) {
     if (!h) return;
@@ -581,7 +605,9 @@ int hy_encode_u32(const uint32_t *a, int m, const uint32_t *b, int n, std::vecto
     return kOk;
 }
 
-int hy_launch(Hyyro *h) {
+// Launches the classes in class_mask (bit c: class c) on their streams,
+// forked from and joined back into h->stream.
+int hy_launch(Hyyro *h, unsigned class_mask = ~0u) {
     // segmented pairs add their segments' zero counts: clear the scores, the
     // tickets and the error flags first
     SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_llcs, 0u, (uint64_t)std::max(h->npairs, 1), h->stream));
@@ -593,19 +619,22 @@ int hy_launch(Hyyro *h) {
     const int cus = device_cus();
     // widest class first: it holds the longest pairs
     for (int c = kClasses - 1; c >= 0; --c) {
+        if (!((class_mask >> c) & 1u)) continue;
         hipStream_t cs = h->cls_stream[(size_t)c];
+        const int R = h->wpl[(size_t)c];
         if (is_seg_class(c)) {
             const size_t k = (size_t)(c - kFirstSegClass);
-            const uint32_t nit = (uint32_t)h->seg_items[k].size();
+            const size_t v = (size_t)(R - 1);
+            const uint32_t nit = (uint32_t)h->seg_items[2 * k + v].size();
             if (!nit) continue;
             SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
-            const hy::SegItem *items = h->d_items + h->item_off[k];
+            const hy::SegItem *items = h->d_items + h->item_off[2 * k + v];
             const unsigned blocks = (unsigned)std::min<int64_t>(nit, 2 * cus);
 #define SL_HY_SEG(R, D)                                                                                         \
     hipLaunchKernelGGL((hy::hy_seg<R, D>), dim3(blocks), dim3(64 * hy::kSegWaves), 0, cs, h->d_pairs, items, nit, \
                        h->d_xoff, h->d_rows, h->d_tab, h->d_tab + h->tab_words, h->d_xbuf, epoch, h->d_ctl + k,     \
                        h->spin_limit, h->d_llcs)
-            if (seg_R(c) == 1) SL_HY_SEG(1, 32);
+            if (R == 1) SL_HY_SEG(1, 32);
             else SL_HY_SEG(2, 16);
 #undef SL_HY_SEG
         } else {
@@ -616,11 +645,18 @@ int hy_launch(Hyyro *h) {
 #define SL_HY_LAUNCH(KERNEL, R, D, NW)                                                                       \
     hipLaunchKernelGGL((hy::KERNEL<R, D, NW>), dim3(cnt), dim3(64 * NW), 0, cs, h->d_pairs, lst, h->d_rows, \
                        h->d_tab, h->d_tab + h->tab_words, h->d_llcs)
-            switch (c) {
-                case 0: SL_HY_LAUNCH(hy_rows, 1, 16, 1); break;   // one wave: the lookahead form is faster
-                case 1: SL_HY_LAUNCH(hy_sys, 1, 32, 2); break;
-                case 2: SL_HY_LAUNCH(hy_sys, 1, 32, 4); break;
-                default: SL_HY_LAUNCH(hy_sys, 16, 2, 16); break;
+            // every form covers the class's widest pair: hy_rows 64 R words,
+            // hy_sys 64 NW R words
+            if (c == 0) {
+                if (R == 1) SL_HY_LAUNCH(hy_rows, 1, 16, 1);
+                else SL_HY_LAUNCH(hy_rows, 2, 16, 1);
+            } else if (c == 1) {
+                if (R == 1) SL_HY_LAUNCH(hy_sys, 1, 32, 2);
+                else SL_HY_LAUNCH(hy_rows, 2, 16, 1);
+            } else {
+                if (R == 1) SL_HY_LAUNCH(hy_sys, 1, 32, 4);
+                else if (R == 2) SL_HY_LAUNCH(hy_sys, 2, 16, 2);
+                else SL_HY_LAUNCH(hy_rows, 4, 8, 1);
             }
 #undef SL_HY_LAUNCH
         }
@@ -715,27 +751,35 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     std::vector<hy::SegItem> all_items;
     std::vector<int64_t> xoff((size_t)std::max(npairs, 1), 0);
     int64_t xtotal = 0;
-    h->seg_items.assign(kSegClasses, {});
-    h->item_off.assign(kSegClasses, 0);
+    h->seg_items.assign(2 * kSegClasses, {});
+    h->item_off.assign(2 * kSegClasses, 0);
     for (int c = kFirstSegClass; c < kFirstSegClass + kSegClasses; ++c) {
         auto lst = h->cls_list[(size_t)c];
         std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) {
             return (int64_t)h->pairs[(size_t)x].m * h->pairs[(size_t)x].K >
                    (int64_t)h->pairs[(size_t)y].m * h->pairs[(size_t)y].K;
         });
-        const int segw = 64 * hy::kSegWaves * seg_R(c);
-        auto &items = h->seg_items[(size_t)(c - kFirstSegClass)];
-        for (int pi : lst) {
+        const size_t k = (size_t)(c - kFirstSegClass);
+        for (int pi : lst) {   // exchange entries for the narrower (R = 1) segments, which are more
             const hy::HyPair &P = h->pairs[(size_t)pi];
-            const int nseg = (P.K + segw - 1) / segw;
+            const int nseg = (P.K + 64 * hy::kSegWaves - 1) / (64 * hy::kSegWaves);
             const int64_t ng = (P.m + 63) / 64 + 8;   // groups a segment runs (hy_seg: G1 - G0 + 1)
             xoff[(size_t)pi] = xtotal;
             xtotal += (int64_t)nseg * ng * 2;
-            for (int sg = 0; sg < nseg; ++sg) items.push_back(hy::SegItem{pi, sg});
         }
-        h->item_off[(size_t)(c - kFirstSegClass)] = (int64_t)all_items.size();
-        all_items.insert(all_items.end(), items.begin(), items.end());
+        for (int R = 1; R <= 2; ++R) {
+            const int segw = 64 * hy::kSegWaves * R;
+            auto &items = h->seg_items[2 * k + (size_t)(R - 1)];
+            for (int pi : lst) {
+                const int nseg = (h->pairs[(size_t)pi].K + segw - 1) / segw;
+                for (int sg = 0; sg < nseg; ++sg) items.push_back(hy::SegItem{pi, sg});
+            }
+            h->item_off[2 * k + (size_t)(R - 1)] = (int64_t)all_items.size();
+            all_items.insert(all_items.end(), items.begin(), items.end());
+        }
     }
+    h->wpl.assign(kClasses, 0);
+    for (int c = 0; c < kClasses; ++c) h->wpl[(size_t)c] = class_wpl_options(c)[0];
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     for (auto &ev : h->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
@@ -850,6 +894,78 @@ int sl_hyyro_time(sl_hyyro *hh, int warmup, int iters, double *ms_per_run) {
     SL_HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
     *ms_per_run = ms / iters;
     return hy_check(h);
+}
+
+int sl_hyyro_buckets(const sl_hyyro *hh, int32_t *max_words, int32_t *pairs, int32_t *words_per_lane,
+                     int32_t *waves) {
+    const Hyyro *h = reinterpret_cast<const Hyyro *>(hh);
+    if (!h) {
+        set_error("sl_hyyro_buckets: null handle");
+        return kErrArg;
+    }
+    for (int c = 0; c < kClasses; ++c) {
+        if (max_words) max_words[c] = kClassMaxWords[c];
+        if (pairs) pairs[c] = (int32_t)h->cls_list[(size_t)c].size();
+        if (words_per_lane) words_per_lane[c] = h->wpl[(size_t)c];
+        if (waves) waves[c] = class_waves(c, h->wpl[(size_t)c]);
+    }
+    return kOk;
+}
+
+int sl_hyyro_set_words_per_lane(sl_hyyro *hh, const int32_t *words_per_lane) {
+    Hyyro *h = reinterpret_cast<Hyyro *>(hh);
+    if (!h || !words_per_lane) {
+        set_error("sl_hyyro_set_words_per_lane: null argument");
+        return kErrArg;
+    }
+    for (int c = 0; c < kClasses; ++c)
+        if (words_per_lane[c] != 0 && !class_offers(c, words_per_lane[c])) {
+            set_error("sl_hyyro_set_words_per_lane: bucket " + std::to_string(c) + " offers no kernel with " +
+                      std::to_string(words_per_lane[c]) + " words per lane");
+            return kErrArg;
+        }
+    for (int c = 0; c < kClasses; ++c)
+        if (words_per_lane[c] != 0) h->wpl[(size_t)c] = words_per_lane[c];
+    return kOk;
+}
+
+// Per bucket with pairs: each offered words-per-lane kernel runs that bucket
+// alone, timed with HIP events (one warm-up launch, then two timed launches
+// per round); the fastest minimum over the rounds is kept.
+int sl_hyyro_autotune(sl_hyyro *hh, int rounds, int32_t *words_per_lane) {
+    Hyyro *h = reinterpret_cast<Hyyro *>(hh);
+    if (!h || rounds < 1) {
+        set_error("sl_hyyro_autotune: bad argument");
+        return kErrArg;
+    }
+    for (int c = 0; c < kClasses; ++c) {
+        if (h->cls_list[(size_t)c].empty()) continue;
+        const std::vector<int> &opts = class_wpl_options(c);
+        std::vector<double> best(opts.size(), 1e30);
+        for (int round = 0; round < rounds; ++round)
+            for (size_t o = 0; o < opts.size(); ++o) {
+                h->wpl[(size_t)c] = opts[o];
+                int rc = hy_launch(h, 1u << c);
+                if (rc) return rc;
+                SL_HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+                for (int it = 0; it < 2 && !rc; ++it) rc = hy_launch(h, 1u << c);
+                if (rc) return rc;
+                SL_HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+                SL_HIP_TRY(hipEventSynchronize(h->ev[1]));
+                rc = hy_check(h);
+                if (rc) return rc;
+                float ms = 0;
+                SL_HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+                best[o] = std::min(best[o], (double)ms / 2);
+            }
+        size_t pick = 0;
+        for (size_t o = 1; o < opts.size(); ++o)
+            if (best[o] < best[pick]) pick = o;
+        h->wpl[(size_t)c] = opts[pick];
+    }
+    if (words_per_lane)
+        for (int c = 0; c < kClasses; ++c) words_per_lane[c] = h->wpl[(size_t)c];
+    return kOk;
 }
 
 int sl_hyyro_stats(const sl_hyyro *hh, int64_t *word_rows, int64_t *table_words) {

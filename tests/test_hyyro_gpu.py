@@ -72,3 +72,38 @@ def test_unsupported_inputs_are_rejected(gpu):
         gpu.llcs_hyyro(g.integers(0, 4, 10).astype(np.uint8), np.zeros((1 << 20) + 1, np.uint8))
     assert gpu.llcs_hyyro(np.zeros(0, np.uint8), np.ones(5, np.uint8)) == 0
     assert gpu.llcs_hyyro(np.array([1, 2, 3], np.uint8), np.array([4, 5], np.uint8)) == 0
+
+
+# offered words per lane per bucket (packed width up to 64, 128, 256, 2,048
+# and 16,384 words), default first
+WPL_OPTIONS = [[1, 2], [1, 2], [1, 2, 4], [1, 2], [2, 1]]
+
+
+def test_every_words_per_lane_form_equals_dp(gpu, oracle):
+    g = np.random.default_rng(77)
+    widths = [1, 63, 64, 65, 4095, 4096, 4097, 8191, 8192, 8193, 12000, 16384, 16385, 65536, 65537, 131072,
+              131073, 200000, 524289, 1 << 20]
+    pairs = []
+    for n in widths:
+        m = int(min(600, max(1, 4_000_000 // n)))
+        sigma = int(g.choice([2, 4, 20]))
+        pairs.append((g.integers(0, sigma, m).astype(np.uint8), g.integers(0, sigma, n).astype(np.uint8)))
+    want = [oracle.llcs_dp(a, b) for a, b in pairs]
+    hb = gpu.HyyroBatch(pairs)
+    b0 = hb.buckets()
+    assert [b["max_words"] for b in b0] == [64, 128, 256, 2048, 16384]
+    assert [b["pairs"] for b in b0] == [6, 3, 3, 4, 4]
+    assert [b["words_per_lane"] for b in b0] == [o[0] for o in WPL_OPTIONS]
+    for k in range(3):
+        wpl = [o[min(k, len(o) - 1)] for o in WPL_OPTIONS]
+        hb.set_words_per_lane(wpl)
+        assert [b["words_per_lane"] for b in hb.buckets()] == wpl
+        print(f"hyyro words per lane {wpl}", flush=True)
+        assert hb.run().tolist() == want
+    with pytest.raises(gpu.SemiLocalError):
+        hb.set_words_per_lane([4, 0, 0, 0, 0])                # bucket 0 has no 4-word form
+    chosen = hb.autotune(rounds=1)
+    assert all(c in o for c, o in zip(chosen, WPL_OPTIONS))
+    assert [b["words_per_lane"] for b in hb.buckets()] == chosen
+    assert hb.run().tolist() == want
+    hb.close()
