@@ -570,8 +570,24 @@ def bench_hybrid(S, W, golden):
     ok = ok and W.sha256(h.run()) == golden["similar_pair_1e6"]["perm_sha256"]
     gtot, _, _ = h.time(warmup=1, iters=3)
     h.close()
+    # block split per length bucket: the grid autotuner times every candidate
+    # grid on a pair of the bucket's length and keeps the fastest for it
+    grids = [(4, 4), (8, 8), (16, 16), (8, 16), (16, 8), (32, 32)]
+    by_bucket = []
+    t0 = time.perf_counter()
+    for length in (100_000, 300_000, 1_000_000):
+        x, y = (a, b) if length == 1_000_000 else W.similar_pair(seed=5, length=length)
+        best, ms = S.hybrid_autotune_blocks(x, y, grids, iters=2)
+        by_bucket.append({"length": length, "bucket_log2": int(np.log2(max(len(x), len(y)))),
+                          "grid": list(best), "ms_by_grid": {f"{gm}x{gn}": round(t, 2) for (gm, gn), t in zip(grids, ms)}})
+    split_s = time.perf_counter() - t0
+    best = S.hybrid_bucket_blocks(len(a), len(b))
+    hb = S.Hybrid(a, b, *best)
+    ok = ok and W.sha256(hb.run()) == golden["similar_pair_1e6"]["perm_sha256"]
+    hb.close()
     return {"workload": "similar pair: a 1e6 symbols (alphabet 4), b = a with 5% subs and 1% indels (seed 3); "
-                        "8x8 blocks", "ms": round(tot, 3), "comb_ms": round(comb, 3), "merge_ms": round(merge, 3),
+                        "8x8 blocks", "ms": round(tot, 3),
+            "block_split_by_bucket": by_bucket, "block_split_autotune_s": round(split_s, 1), "comb_ms": round(comb, 3), "merge_ms": round(merge, 3),
             "graph_ms": round(gtot, 3), "merge_levels": levels, "gcups": round(cells / (tot * 1e-3) / 1e9, 2),
             "comb_roofline": valu_roofline(cells * VALU_OPS_PER_CELL, comb, "5 per cell"),
             "graph_gcups": round(cells / (gtot * 1e-3) / 1e9, 2),

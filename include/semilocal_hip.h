@@ -294,6 +294,20 @@ int sl_hybrid_set_graph(sl_hybrid *h, int enable);
  * captured again on the next graph run.  Results never depend on it. */
 int sl_hybrid_autotune(sl_hybrid *h, int iters, int *rows_per_lane, int *waves_per_cu);
 void sl_hybrid_destroy(sl_hybrid *h);
+/* Block split per length bucket.  Replaces the fixed block rule of          */
+/*   hybrid_iterative_wrapper (semi_local.h:740-790) with a measured one:    */
+/* for the pair (a, b), builds the hybrid for every candidate grid           */
+/* cand_m[i] x cand_n[i], times `iters` runs of each after a warm-up (HIP    */
+/* events, the default tile policy) and returns the fastest grid; ms         */
+/* (optional, ncand entries) receives each grid's time.  The choice is kept  */
+/* for the pair's length bucket, floor(log2(max(m, n))), for the process.    */
+/* Results never depend on the grid.                                          */
+int sl_hybrid_autotune_blocks(const void *a, int m, const void *b, int n, int sym_bytes, int ncand,
+                              const int32_t *cand_m, const int32_t *cand_n, int iters, int32_t *best_m,
+                              int32_t *best_n, double *ms);
+/* The grid kept for the length bucket of (m, n); 0 x 0 when that bucket was */
+/* never tuned.                                                               */
+int sl_hybrid_bucket_blocks(int m, int n, int32_t *small_m, int32_t *small_n);
 
 /* ------------------------------------------------------------------------ */
 /* Request loop (serving): continuous batching of comparisons on one GPU.     */

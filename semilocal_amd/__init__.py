@@ -7,12 +7,12 @@ Python binding used by the tests and the benchmark.
 """
 from ._lib import (AMatch, Batch, BitParallel, BraidIndex, Hybrid, HyyroBatch, SemiLocalError, braid_multiply,
                    braid_multiply_batch, braid_multiply_time, comb_batch, device_info, duplicate_search,
-                   hybrid_iterative, lib, llcs_bitparallel, llcs_hyyro, plan_queue, semi_local_comb,
+                   hybrid_autotune_blocks, hybrid_bucket_blocks, hybrid_iterative, lib, llcs_bitparallel, llcs_hyyro, plan_queue, semi_local_comb,
                    staggered_multiply, Server)
 
 __all__ = ["AMatch", "Batch", "BitParallel", "BraidIndex", "Hybrid", "HyyroBatch", "SemiLocalError",
            "braid_multiply", "braid_multiply_batch", "braid_multiply_time", "comb_batch", "device_info",
-           "duplicate_search", "hybrid_iterative", "lib", "llcs_bitparallel", "llcs_hyyro", "plan_queue",
+           "duplicate_search", "hybrid_autotune_blocks", "hybrid_bucket_blocks", "hybrid_iterative", "lib", "llcs_bitparallel", "llcs_hyyro", "plan_queue",
            "semi_local_comb", "staggered_multiply", "Server", "llcs_from_kernel"]
 
 

@@ -23,6 +23,7 @@ SL_QUERY_STRING_SUBSTRING = 1
 SL_QUERY_SUBSTRING_STRING = 2
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 
@@ -116,6 +117,9 @@ SIGNATURES = {
     "sl_hybrid_destroy": ([_vp], None),
     "sl_hybrid_set_graph": ([_vp, C.c_int], C.c_int),
     "sl_hybrid_autotune": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "sl_hybrid_autotune_blocks": ([_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _i32p, _i32p, C.c_int,
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32), _f64p], C.c_int),
+    "sl_hybrid_bucket_blocks": ([C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "sl_server_create": ([C.POINTER(_vp), C.c_int64, C.c_int, C.c_int, C.c_int64], C.c_int),
     "sl_server_submit_compare": ([_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)],
                                  C.c_int),
@@ -745,6 +749,28 @@ def hybrid_iterative(a, b, small_m: int, small_n: int) -> np.ndarray:
     check(lib().sl_hybrid_iterative(_ptr(a_arr) if m else None, m, _ptr(b_arr) if n else None, n, w, small_m,
                                     small_n, _ptr(out)))
     return out[:m + n]
+
+
+def hybrid_autotune_blocks(a, b, grids, iters: int = 2):
+    """Times the hybrid of (a, b) at every candidate block grid (sm, sn) and
+    returns (best grid, list of ms per grid); the choice is kept for the
+    pair's length bucket (see hybrid_bucket_blocks)."""
+    a_arr, b_arr, w = _pair_symbols(a, b)
+    m, n = len(a_arr), len(b_arr)
+    cm = np.ascontiguousarray([g[0] for g in grids], np.int32)
+    cn = np.ascontiguousarray([g[1] for g in grids], np.int32)
+    ms = np.zeros(len(grids), np.float64)
+    bm, bn = C.c_int32(), C.c_int32()
+    check(lib().sl_hybrid_autotune_blocks(_ptr(a_arr) if m else None, m, _ptr(b_arr) if n else None, n, w,
+                                          len(grids), cm, cn, iters, C.byref(bm), C.byref(bn), ms))
+    return (bm.value, bn.value), ms.tolist()
+
+
+def hybrid_bucket_blocks(m: int, n: int):
+    """The block grid kept for the length bucket of (m, n); (0, 0) if none."""
+    bm, bn = C.c_int32(), C.c_int32()
+    check(lib().sl_hybrid_bucket_blocks(m, n, C.byref(bm), C.byref(bn)))
+    return bm.value, bn.value
 
 
 # ---------------------------------------------------------------------------

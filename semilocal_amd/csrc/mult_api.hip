@@ -9,6 +9,7 @@
 //   hybrid_iterative<Input, WithIf> (semi_local.h:577-580).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/semilocal_hip.h"
@@ -985,6 +986,64 @@ int sl_hybrid_iterative(const void *a, int m, const void *b, int n, int sym_byte
     if (!rc) rc = sl_hybrid_download(h, row_to_col);
     sl_hybrid_destroy(h);
     return rc;
+}
+
+// Block split per length bucket.  The reference sizes the hybrid's blocks
+// by a fixed rule (hybrid_iterative_wrapper, semi_local.h:740-790); here the
+// grid is measured.  Bucket of a pair: floor(log2(max(m, n))).
+static std::mutex g_grid_mu;
+static int32_t g_grid[32][2];   // bucket -> (small_m, small_n), 0 x 0: not tuned
+
+static int grid_bucket(int m, int n) {
+    int v = std::max(std::max(m, n), 1), b = 0;
+    while (v >>= 1) ++b;
+    return b;
+}
+
+int sl_hybrid_autotune_blocks(const void *a, int m, const void *b, int n, int sym_bytes, int ncand,
+                              const int32_t *cand_m, const int32_t *cand_n, int iters, int32_t *best_m,
+                              int32_t *best_n, double *ms) {
+    if (ncand < 1 || !cand_m || !cand_n || iters < 1 || !best_m || !best_n) {
+        set_error("sl_hybrid_autotune_blocks: bad argument");
+        return kErrArg;
+    }
+    for (int i = 0; i < ncand; ++i)
+        if (cand_m[i] < 1 || cand_n[i] < 1) {
+            set_error("sl_hybrid_autotune_blocks: a candidate grid has fewer than 1 x 1 blocks");
+            return kErrArg;
+        }
+    double best = 1e300;
+    int pick = -1;
+    for (int i = 0; i < ncand; ++i) {
+        sl_hybrid *h = nullptr;
+        int rc = sl_hybrid_create(&h, a, m, b, n, sym_bytes, cand_m[i], cand_n[i]);
+        double t = 0;
+        if (!rc) rc = sl_hybrid_time(h, 1, iters, &t, nullptr, nullptr);
+        sl_hybrid_destroy(h);
+        if (rc) return rc;
+        if (ms) ms[i] = t;
+        if (t < best) {
+            best = t;
+            pick = i;
+        }
+    }
+    *best_m = cand_m[pick];
+    *best_n = cand_n[pick];
+    std::lock_guard<std::mutex> lk(g_grid_mu);
+    g_grid[grid_bucket(m, n)][0] = cand_m[pick];
+    g_grid[grid_bucket(m, n)][1] = cand_n[pick];
+    return kOk;
+}
+
+int sl_hybrid_bucket_blocks(int m, int n, int32_t *small_m, int32_t *small_n) {
+    if (m < 0 || n < 0 || !small_m || !small_n) {
+        set_error("sl_hybrid_bucket_blocks: bad argument");
+        return kErrArg;
+    }
+    std::lock_guard<std::mutex> lk(g_grid_mu);
+    *small_m = g_grid[grid_bucket(m, n)][0];
+    *small_n = g_grid[grid_bucket(m, n)][1];
+    return kOk;
 }
 
 }  // extern "C"

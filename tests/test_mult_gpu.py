@@ -189,3 +189,23 @@ def test_hybrid_autotune_keeps_results_and_recaptures_the_graph(gpu):
     h.set_graph(False)
     assert np.array_equal(h.run(), want)
     h.close()
+
+
+def test_block_grid_autotuner_picks_a_candidate_per_length_bucket(gpu):
+    # the grid autotuner times each candidate block split of the pair and keeps
+    # the fastest for the pair's length bucket (floor(log2(max(m, n)))); the
+    # hybrid at that grid gives the same permutation as direct combing
+    g = _rng(43)
+    a = g.integers(0, 4, size=40000).astype(np.uint8)
+    b = a.copy()
+    b[g.integers(0, len(b), 2000)] = g.integers(0, 4, 2000).astype(np.uint8)
+    grids = [(1, 1), (2, 2), (4, 4), (2, 8)]
+    assert gpu.hybrid_bucket_blocks(1 << 29, 5) == (0, 0)     # a bucket never tuned
+    best, ms = gpu.hybrid_autotune_blocks(a, b, grids, iters=1)
+    assert best in grids and len(ms) == len(grids) and all(t > 0 for t in ms)
+    assert ms[grids.index(best)] == min(ms)
+    assert gpu.hybrid_bucket_blocks(len(a), len(b)) == best
+    assert gpu.hybrid_bucket_blocks(33000, 20000) == best       # the same bucket (2^15 .. 2^16 - 1)
+    assert np.array_equal(gpu.hybrid_iterative(a, b, *best), gpu.semi_local_comb(a, b))
+    with pytest.raises(gpu.SemiLocalError):
+        gpu.hybrid_autotune_blocks(a, b, [(0, 2)])
