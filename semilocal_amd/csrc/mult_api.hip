@@ -49,6 +49,8 @@ __global__ void seg_search(const uint32_t *, const uint32_t *, const int32_t *, 
                            const int2 *, int, const int32_t *, int4 *);
 __global__ void seg_walk(const uint32_t *, const uint32_t *, const SaSeg *, const int2 *, int, const int32_t *,
                          const int4 *, int32_t *);
+__global__ void seg_walk_wave(const uint32_t *, const uint32_t *, const SaSeg *, const int2 *, int, const int32_t *,
+                              const int4 *, int32_t *);
 __global__ void fill_i32(int32_t *, int32_t, int64_t);
 __global__ void stag_qinv(const int32_t *, const StagJob *, const int32_t *, int32_t *);
 __global__ void stag_count(const int32_t *, const int32_t *, const StagJob *, const int32_t *, int32_t *);
@@ -293,9 +295,8 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
                            p->d_cols + L.col0, L.ncol, p->d_cntHi, p->d_cntLo);
         hipLaunchKernelGGL(seg_search, dim3((unsigned)ceil_div(L.nwalk, 4 * kSearchRun)), dim3(256), 0, s, p->d_U, p->d_Ui,
                            p->d_cntHi, p->d_cntLo, segs, walks, L.nwalk, cend, p->d_start);
-        hipLaunchKernelGGL(seg_walk, dim3((unsigned)(8 * ceil_div(ceil_div(L.nwalk, kWalkBlock), 8))), dim3(kWalkBlock), 0, s,
-                           p->d_U, p->d_Ui, segs,
-                           walks, L.nwalk, cend, p->d_start, p->d_R[rc]);
+        hipLaunchKernelGGL(seg_walk_wave, dim3((unsigned)(8 * ceil_div(ceil_div(L.nwalk, 4), 8))), dim3(256), 0, s,
+                           p->d_U, p->d_Ui, segs, walks, L.nwalk, cend, p->d_start, p->d_R[rc]);
     }
     p->result = rc;
     SL_HIP_TRY(hipGetLastError());

@@ -971,6 +971,200 @@ __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__res
     }
 }
 
+// ---------------------------------------------------------------------------
+// The walk by a whole wave, 64 x 64 positions per round.
+// ---------------------------------------------------------------------------
+// A window is anchored at the walk's position (r0, c0) with d0 = delta(r0,
+// c0).  Position (i, j) is (r0 - i, c0 + j); lane j holds column c0 + j's
+// point and bit i of a 64-bit mask stands for row r0 - 1 - i, the row an up
+// move from (i, j) passes.  Then
+//     delta(i, j) = D_j - popcount(RT_j & (2^i - 1)),
+// D_j = delta(r0, c0 + j) (d0 plus the column terms of columns c0 .. c0+j-1
+// at row r0: one ballot and a masked bit count) and RT_j the row terms of
+// the window rows at column c0 + j.  A row's term changes with the column
+// only at the row's own point, so RT_j is RT_0 with the bits of the rows
+// whose points lie in columns [c0, c0 + j) flipped: a prefix XOR across the
+// lanes of one-hot row bits.
+//
+// delta is nonincreasing in i and nondecreasing in j.  The walk goes up while
+// delta >= 0 and right otherwise, so with tau_j the first i with
+// delta(i, j) < 0 it leaves column j at row b_j = max(b_{j-1}, tau_j)
+// (b_{-1} = 0): a prefix maximum across lanes, and it enters column j at
+// a_j = b_{j-1}.  The position of step k of the window is (k - j, j) for the
+// j with a_j + j <= k <= b_j + j.
+//
+// A good point of column j (delta(i, j) = 0, delta(i - 1, j) = 1 and
+// delta(i, j - 1) = -1, the reference's test) can only be at i* = the first
+// i with delta(i, j) <= 0 (delta moves by at most 1 per row): the D_j-th set
+// bit of RT_j plus 1 (0 when D_j = 0); then tau_j is one past the next set
+// bit from i*.  It is reported when the walk evaluates (i*, j): a_j <= i* and
+// i* + j inside the steps this window evaluates.
+//
+// The window ends the segment at its step budget or where the walk leaves the
+// square (moving right at column s, or up at row 0: positions evaluated and
+// then the walk stops), else the walk continues from where it leaves the
+// window, at the top (row i = 64 of the first column with b_j >= 64) or the
+// right side (column 64 at row b_63).  Lanes outside the square (rows below
+// 0, columns past s - 1) hold the value 0 and only affect positions after
+// the stop there.  Every value here is the sequential walk's (ant_walk):
+// the same positions, budget and good points.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_or0(uint32_t v) {   // lanes without a source read 0
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+
+// Inclusive scans across the wave: within rows of 16 lanes by shifts of 1, 2,
+// 4 and 8, then row 0's total into row 1 (and row 2's into row 3), then rows
+// 0-1's total into rows 2 and 3.
+__device__ __forceinline__ uint64_t wave_xor_scan64(uint64_t x) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    lo ^= dpp_or0<kDppRowShr1, 0xf>(lo);
+    hi ^= dpp_or0<kDppRowShr1, 0xf>(hi);
+    lo ^= dpp_or0<kDppRowShr2, 0xf>(lo);
+    hi ^= dpp_or0<kDppRowShr2, 0xf>(hi);
+    lo ^= dpp_or0<kDppRowShr4, 0xf>(lo);
+    hi ^= dpp_or0<kDppRowShr4, 0xf>(hi);
+    lo ^= dpp_or0<kDppRowShr8, 0xf>(lo);
+    hi ^= dpp_or0<kDppRowShr8, 0xf>(hi);
+    lo ^= dpp_or0<kDppRowBcast15, 0xa>(lo);
+    hi ^= dpp_or0<kDppRowBcast15, 0xa>(hi);
+    lo ^= dpp_or0<kDppRowBcast31, 0xc>(lo);
+    hi ^= dpp_or0<kDppRowBcast31, 0xc>(hi);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int wave_max_scan(int x) {   // x >= 0
+    x = max(x, (int)dpp_or0<kDppRowShr1, 0xf>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowShr2, 0xf>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowShr4, 0xf>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowShr8, 0xf>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowBcast15, 0xa>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowBcast31, 0xc>((uint32_t)x));
+    return x;
+}
+
+__device__ __forceinline__ int ctz64(uint64_t x) {   // 64 when x = 0
+    const int f = __ffsll((unsigned long long)x);
+    return f ? f - 1 : 64;
+}
+
+// Index of the k-th (k >= 1) set bit of x, 64 when x has fewer: halves,
+// quarters, ... chosen by bit counts.
+__device__ __forceinline__ int select64(uint64_t x, int k) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    int c = __popc(lo);
+    int base = k > c ? 32 : 0;
+    uint32_t w = k > c ? hi : lo;
+    k -= k > c ? c : 0;
+#pragma unroll
+    for (int half = 16; half >= 1; half >>= 1) {
+        c = __popc(w & ((1u << half) - 1u));
+        const bool up = k > c;
+        base += up ? half : 0;
+        w = up ? w >> half : w;
+        k -= up ? c : 0;
+    }
+    return (k == 1 && (w & 1u)) ? base : 64;
+}
+
+// The walk from (r0, c0) with d0 = delta(r0, c0) for at most B evaluations
+// (B >= 1), by the whole wave; every argument is uniform.  rowv(r) gives row
+// r's point (col | hi bit 31), colv(c) column c's point (row | hi bit 31) for
+// r, c in [0, s); good points go to put(row, col).
+template <class RowF, class ColF, class PutF>
+__device__ __forceinline__ void wave_walk(int r0, int c0, int d0, int s, int B, RowF rowv, ColF colv, PutF put) {
+    const int lane = threadIdx.x & 63;
+    for (bool more = true; more;) {
+        const int rr = r0 - 1 - lane, cc = c0 + lane;
+        const uint32_t u_ld = rowv(rr >= 0 ? rr : 0), v_ld = colv(cc < s ? cc : s - 1);
+        const uint32_t u_r0 = rowv(r0);
+        const uint32_t v_left = colv(c0 > 0 ? c0 - 1 : 0);
+        const uint32_t u = rr >= 0 ? u_ld : 0u, v = cc < s ? v_ld : 0u;
+        const int ucol = (int)(u & kValMask), vrow = (int)(v & kValMask);
+        const uint64_t RT0 = __ballot((ucol >= c0) != (u >> 31 != 0));
+        const int ih = r0 - 1 - vrow;
+        const uint64_t oh = (cc < s && ih >= 0 && ih < 64) ? (1ull << ih) : 0ull;
+        const uint64_t RT = RT0 ^ wave_xor_scan64(oh) ^ oh;
+        const uint64_t CT = __ballot(cc < s && ((vrow >= r0) != (v >> 31 != 0)));
+        const int D = d0 + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(CT >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)CT, 0u));
+        const int p = select64(RT, D > 1 ? D : 1);
+        const int istar = D < 0 ? 64 : (D == 0 ? 0 : (p < 64 ? p + 1 : 64));
+        const int tnext = istar < 64 ? istar + 1 + ctz64(RT >> istar) : 64;
+        const int tau = D < 0 ? 0 : min(tnext, 64);
+        const int b = wave_max_scan(tau);
+        const int a = (int)dpp_or0<kDppWaveShr1, 0xf>((uint32_t)b);
+        // events, in step order
+        const int jt = ctz64(__ballot(b >= 64));   // first column left at the top (64: none)
+        const int b63 = __builtin_amdgcn_readlane(b, 63);
+        const int X = jt < 64 ? 64 + jt : b63 + 64;   // step at which the walk leaves the window
+        int kstop = 1 << 30;                          // step of a stop (evaluated, then the walk ends)
+        const int js = s - c0;
+        if (js >= 0 && js < 64) {
+            const int bjs = __builtin_amdgcn_readlane(b, js);
+            if (bjs < 64 && jt > js) kstop = bjs + js;
+        }
+        if (r0 < 64) {
+            const uint64_t rm = __ballot(b > r0);
+            if (rm) kstop = min(kstop, r0 + ctz64(rm));
+        }
+        const bool ends = kstop + 1 <= B || B <= X;
+        const int lim = kstop + 1 <= B ? kstop + 1 : min(B, X);   // steps [0, lim) are evaluated here
+        // the good point of column j, at (i*, j)
+        const uint32_t vl_sh = dpp_or0<kDppWaveShr1, 0xf>(v);
+        const uint32_t vl = lane == 0 ? v_left : vl_sh;
+        const bool rt_r0 = ((int)(u_r0 & kValMask) >= cc) != (u_r0 >> 31 != 0);
+        const bool ct_l = ((int)(vl & kValMask) >= r0 - istar) != (vl >> 31 != 0);
+        if (istar < 64 && a <= istar && istar + lane < lim && cc >= 1 && (istar > 0 || rt_r0) && ct_l)
+            put(r0 - istar, cc - 1);
+        more = !ends;
+        // continue from the exit: the top of column jt, or the right side at row b_63
+        int ie, je, dl;
+        if (jt < 64) {
+            ie = 64;
+            je = jt;
+            dl = __builtin_amdgcn_readlane(D - __popcll(RT), jt);
+        } else {
+            ie = b63;
+            je = 64;
+            const int ct = ((vrow >= r0 - b63) != (v >> 31 != 0)) ? 1 : 0;
+            dl = __builtin_amdgcn_readlane(D - __popcll(RT & ((1ull << b63) - 1ull)) + ct, 63);
+        }
+        r0 = __builtin_amdgcn_readfirstlane(r0 - ie);
+        c0 = __builtin_amdgcn_readfirstlane(c0 + je);
+        d0 = __builtin_amdgcn_readfirstlane(dl);
+        B = __builtin_amdgcn_readfirstlane(B - X);
+    }
+}
+
+// Walks, large levels: one wave per segment (wave_walk over the level's U and
+// U^-1).  Blocks of 4 waves; the hardware deals blocks to the 8 XCDs
+// round-robin, so block b is mapped to segments in contiguous ranges per XCD
+// and each XCD's L2 holds the rows and columns of its part of the level.
+__global__ __launch_bounds__(256) void seg_walk_wave(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
+                                                     const SaSeg *__restrict__ segs, const int2 *__restrict__ walks,
+                                                     int nwalks, const int32_t *__restrict__ c_end,
+                                                     const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
+    const int per_xcd = (int)gridDim.x / 8;   // the grid is a multiple of 8 blocks
+    const int blk = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    const int w = __builtin_amdgcn_readfirstlane(blk * 4 + (int)(threadIdx.x >> 6));
+    if (w >= nwalks) return;
+    const int4 st = seg_start[w];
+    const int r0 = __builtin_amdgcn_readfirstlane(st.x);
+    if (r0 < 0) return;
+    const int2 job = walks[w];
+    const SaSeg sg = segs[job.x];
+    const int s = __builtin_amdgcn_readfirstlane(sg.s);
+    const int t_end = s - 1 + c_end[job.x];
+    const uint32_t *Us = U + sg.x0, *Uis = Ui + sg.x0;
+    int32_t *Rs = R + sg.x0;
+    wave_walk(r0, __builtin_amdgcn_readfirstlane(st.y), __builtin_amdgcn_readfirstlane(st.z), s,
+              __builtin_amdgcn_readfirstlane(min(kSegSteps, t_end - st.w + 1)), [&](int x) { return Us[x]; },
+              [&](int x) { return Uis[x]; }, [&](int rr, int cc) { Rs[rr] = cc; });
+}
+
 // Walks: one lane per segment, reading the level's U / U^-1 a window of
 // kWalkWindow rows and columns per round trip (ant_walk_window).  Blocks of
 // kWalkBlock lanes spread the few thousand segments of a level over many
