@@ -47,8 +47,6 @@ __global__ void table_rows(const uint32_t *, const SaSeg *, const int2 *, int32_
 __global__ void table_cols(const SaSeg *, const int2 *, int, int32_t *, int32_t *);
 __global__ void seg_search(const uint32_t *, const uint32_t *, const int32_t *, const int32_t *, const SaSeg *,
                            const int2 *, int, const int32_t *, int4 *);
-__global__ void seg_walk(const uint32_t *, const uint32_t *, const SaSeg *, const int2 *, int, const int32_t *,
-                         const int4 *, int32_t *);
 __global__ void seg_walk_wave(const uint32_t *, const uint32_t *, const SaSeg *, const int2 *, int, const int32_t *,
                               const int4 *, int32_t *);
 __global__ void fill_i32(int32_t *, int32_t, int64_t);
@@ -235,7 +233,7 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
     }
     PLAN_TRY(dalloc(&p->d_PR, nt * std::max<size_t>(nl, 1)));
     PLAN_TRY(dalloc(&p->d_QC, nt * std::max<size_t>(nl, 1)));
-    // slack past the last tile: the walk reads its window in aligned 16-byte
+    // slack past the last tile: the segment searches read aligned 16-byte
     // groups that can reach a few entries past a subproblem's end
     PLAN_TRY(dalloc(&p->d_U, nt + kWalkSlack));
     PLAN_TRY(dalloc(&p->d_Ui, nt + kWalkSlack));

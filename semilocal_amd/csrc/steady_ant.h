@@ -8,12 +8,9 @@ namespace sa {
 
 constexpr int kTile = 1024;      // elements per block of the split / small-solver kernels
 constexpr int kBlk = 1024;       // block size of the per-subproblem count tables
-constexpr int kSegSteps = 128;   // walk steps per parallel walk segment (large levels)
-constexpr int kSearchRun = 4;    // consecutive walk segments whose starts one wave finds (large levels)
-constexpr int kWalkBlock = 64;   // lanes (segments) per block of the large-level walk kernel
-constexpr int kWalkWindow = 16;  // rows and columns a large-level walk reads ahead per round trip
-constexpr int kWalkGroups = kWalkWindow / 4 + 1;   // aligned 4-entry groups covering one window side
-constexpr int kWalkSlack = 64;   // entries allocated past the end of U and U^-1 for those groups
+constexpr int kSegSteps = 512;   // walk steps per parallel walk segment (large levels)
+constexpr int kSearchRun = 2;    // consecutive walk segments whose starts one wave finds (large levels)
+constexpr int kWalkSlack = 64;   // entries allocated past the end of U and U^-1 (the searches' 16-byte groups)
 constexpr int kSplitPer = 4;     // consecutive elements per thread of the large-level split kernels
 constexpr int kSplitThreads = kTile / kSplitPer;   // their block size (one tile per block)
 constexpr int kMaxNb = 4096;     // count-table blocks per side: subproblems up to kMaxNb * kBlk

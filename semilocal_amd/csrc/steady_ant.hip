@@ -31,7 +31,7 @@
 // per-subproblem table of block counts plus two short scans, down to a range
 // of about a thousand candidates, which one pass of prefix sums over the
 // diagonal's own rows and columns settles.  Each segment is then walked by
-// one lane from LDS.  The product is U with the good points
+// one wave, 64 x 64 positions per round (wave_walk).  The product is U with the good points
 // written over it, which is how the reference merges r_hi into r_lo and adds
 // the good points.
 //
@@ -284,92 +284,187 @@ __device__ __forceinline__ void ant_walk(int r, int c, int d, int s, int steps, 
     }
 }
 
-// The same walk with its reads taken K steps at a time.  A walk moves one
-// row up or one column right per step, so its next K steps read only rows
-// r-1 .. r-K and columns c .. c+K-1: the lane reads that window in one batch
-// of independent loads into its own LDS slots (lw[k * 64], interleaved with
-// the other lanes of its wave, so every access is conflict-free), then takes
-// up to K steps from LDS.  One memory round trip per K steps instead of one
-// per step; the values are the same as ant_walk's, so the walk is too.
+// ---------------------------------------------------------------------------
+// The walk by a whole wave, 64 x 64 positions per round.
+// ---------------------------------------------------------------------------
+// A window is anchored at the walk's position (r0, c0) with d0 = delta(r0,
+// c0).  Position (i, j) is (r0 - i, c0 + j); lane j holds column c0 + j's
+// point and bit i of a 64-bit mask stands for row r0 - 1 - i, the row an up
+// move from (i, j) passes.  Then
+//     delta(i, j) = D_j - popcount(RT_j & (2^i - 1)),
+// D_j = delta(r0, c0 + j) (d0 plus the column terms of columns c0 .. c0+j-1
+// at row r0: one ballot and a masked bit count) and RT_j the row terms of
+// the window rows at column c0 + j.  A row's term changes with the column
+// only at the row's own point, so RT_j is RT_0 with the bits of the rows
+// whose points lie in columns [c0, c0 + j) flipped: a prefix XOR across the
+// lanes of one-hot row bits.
 //
-// ant_walk_window over the level arrays themselves (rows U, columns Ui of
-// the subproblem), with each window side read as kWalkGroups aligned 16-byte
-// groups: rows from a_r = (r - K) rounded down to a multiple of 4 (at least
-// 0), columns from a_c = c rounded down.  Those cover rows r-K .. r-1 and
-// columns c .. c+K-1, and reach at most 19 entries past the subproblem's end
-// (its tile padding, the next subproblem or the arrays' slack; such entries
-// are never used).  A step indexes the window by its row and column.
+// delta is nonincreasing in i and nondecreasing in j.  The walk goes up while
+// delta >= 0 and right otherwise, so with tau_j the first i with
+// delta(i, j) < 0 it leaves column j at row b_j = max(b_{j-1}, tau_j)
+// (b_{-1} = 0): a prefix maximum across lanes, and it enters column j at
+// a_j = b_{j-1}.  The position of step k of the window is (k - j, j) for the
+// j with a_j + j <= k <= b_j + j.
 //
-// Two changes keep a step's dependent chain short.  A walk that moves right
-// next needs column c+1, one that moves up row r-2; every other value is the
-// same as the step before.  So both candidates are read from LDS one step
-// ahead, while the current step is computed, and the move picks one: no LDS
-// read waits on the step's direction.  And every lane takes K steps per
-// window, with a lane that has stopped (or used its steps) frozen by a mask,
-// so the step loop has a uniform trip count and no per-lane exit.
-template <class PutF>
-__device__ __forceinline__ void ant_walk_window4(int r, int c, int d, int s, int steps, const uint32_t *U,
-                                                 const uint32_t *Ui, PutF put, uint32_t *lw) {
-    constexpr int K = kWalkWindow, G = kWalkGroups;
-    uint32_t ur = U[r];
-    uint32_t left = c > 0 ? Ui[c - 1] : 0u;
-    int t = 0;
-    bool alive = steps > 0;
-    while (__ballot(alive) != 0) {   // uniform
-        const int ar = r - K > 0 ? (r - K) & ~3 : 0;
-        const int ac = c & ~3;
-        uint4 wr[G], wc[G];
+// A good point of column j (delta(i, j) = 0, delta(i - 1, j) = 1 and
+// delta(i, j - 1) = -1, the reference's test) can only be at i* = the first
+// i with delta(i, j) <= 0 (delta moves by at most 1 per row): the D_j-th set
+// bit of RT_j plus 1 (0 when D_j = 0); then tau_j is one past the next set
+// bit from i*.  It is reported when the walk evaluates (i*, j): a_j <= i* and
+// i* + j inside the steps this window evaluates.
+//
+// The window ends the segment at its step budget or where the walk leaves the
+// square (moving right at column s, or up at row 0: positions evaluated and
+// then the walk stops), else the walk continues from where it leaves the
+// window, at the top (row i = 64 of the first column with b_j >= 64) or the
+// right side (column 64 at row b_63).  Lanes outside the square (rows below
+// 0, columns past s - 1) hold the value 0 and only affect positions after
+// the stop there.  Every value here is the sequential walk's (ant_walk):
+// the same positions, budget and good points.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_or0(uint32_t v) {   // lanes without a source read 0
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+
+// Inclusive scans across the wave: within rows of 16 lanes by shifts of 1, 2,
+// 4 and 8, then row 0's total into row 1 (and row 2's into row 3), then rows
+// 0-1's total into rows 2 and 3.
+__device__ __forceinline__ uint64_t wave_xor_scan64(uint64_t x) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    lo ^= dpp_or0<kDppRowShr1, 0xf>(lo);
+    hi ^= dpp_or0<kDppRowShr1, 0xf>(hi);
+    lo ^= dpp_or0<kDppRowShr2, 0xf>(lo);
+    hi ^= dpp_or0<kDppRowShr2, 0xf>(hi);
+    lo ^= dpp_or0<kDppRowShr4, 0xf>(lo);
+    hi ^= dpp_or0<kDppRowShr4, 0xf>(hi);
+    lo ^= dpp_or0<kDppRowShr8, 0xf>(lo);
+    hi ^= dpp_or0<kDppRowShr8, 0xf>(hi);
+    lo ^= dpp_or0<kDppRowBcast15, 0xa>(lo);
+    hi ^= dpp_or0<kDppRowBcast15, 0xa>(hi);
+    lo ^= dpp_or0<kDppRowBcast31, 0xc>(lo);
+    hi ^= dpp_or0<kDppRowBcast31, 0xc>(hi);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int wave_max_scan(int x) {   // x >= 0
+    x = max(x, (int)dpp_or0<kDppRowShr1, 0xf>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowShr2, 0xf>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowShr4, 0xf>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowShr8, 0xf>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowBcast15, 0xa>((uint32_t)x));
+    x = max(x, (int)dpp_or0<kDppRowBcast31, 0xc>((uint32_t)x));
+    return x;
+}
+
+__device__ __forceinline__ int ctz64(uint64_t x) {   // 64 when x = 0
+    const int f = __ffsll((unsigned long long)x);
+    return f ? f - 1 : 64;
+}
+
+// Index of the k-th (k >= 1) set bit of x, 64 when x has fewer: halves,
+// quarters, ... chosen by bit counts.
+__device__ __forceinline__ int select64(uint64_t x, int k) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    int c = __popc(lo);
+    int base = k > c ? 32 : 0;
+    uint32_t w = k > c ? hi : lo;
+    k -= k > c ? c : 0;
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            wr[g] = *reinterpret_cast<const uint4 *>(U + ar + 4 * g);
-            wc[g] = *reinterpret_cast<const uint4 *>(Ui + ac + 4 * g);
+    for (int half = 16; half >= 1; half >>= 1) {
+        c = __popc(w & ((1u << half) - 1u));
+        const bool up = k > c;
+        base += up ? half : 0;
+        w = up ? w >> half : w;
+        k -= up ? c : 0;
+    }
+    return (k == 1 && (w & 1u)) ? base : 64;
+}
+
+// The walk from (r0, c0) with d0 = delta(r0, c0) for at most B evaluations
+// (B >= 1), by the whole wave; every argument is uniform.  rowv(r) gives row
+// r's point (col | hi bit 31), colv(c) column c's point (row | hi bit 31) for
+// r, c in [0, s); good points go to put(row, col).  Returns where the walk
+// stands after its B evaluations, (r, c, delta(r, c)) -- the next segment's
+// start -- or r = -1 where it left the square before.
+template <class RowF, class ColF, class PutF>
+__device__ __forceinline__ int3 wave_walk(int r0, int c0, int d0, int s, int B, RowF rowv, ColF colv, PutF put) {
+    const int lane = threadIdx.x & 63;
+    int3 end = make_int3(-1, 0, 0);
+    for (bool more = true; more;) {
+        const int rr = r0 - 1 - lane, cc = c0 + lane;
+        const uint32_t u_ld = rowv(rr >= 0 ? rr : 0), v_ld = colv(cc < s ? cc : s - 1);
+        const uint32_t u_r0 = rowv(r0);
+        const uint32_t v_left = colv(c0 > 0 ? c0 - 1 : 0);
+        const uint32_t u = rr >= 0 ? u_ld : 0u, v = cc < s ? v_ld : 0u;
+        const int ucol = (int)(u & kValMask), vrow = (int)(v & kValMask);
+        const uint64_t RT0 = __ballot((ucol >= c0) != (u >> 31 != 0));
+        const int ih = r0 - 1 - vrow;
+        const uint64_t oh = (cc < s && ih >= 0 && ih < 64) ? (1ull << ih) : 0ull;
+        const uint64_t RT = RT0 ^ wave_xor_scan64(oh) ^ oh;
+        const uint64_t CT = __ballot(cc < s && ((vrow >= r0) != (v >> 31 != 0)));
+        const int D = d0 + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(CT >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)CT, 0u));
+        const int p = select64(RT, D > 1 ? D : 1);
+        const int istar = D < 0 ? 64 : (D == 0 ? 0 : (p < 64 ? p + 1 : 64));
+        const int tnext = istar < 64 ? istar + 1 + ctz64(RT >> istar) : 64;
+        const int tau = D < 0 ? 0 : min(tnext, 64);
+        const int b = wave_max_scan(tau);
+        const int a = (int)dpp_or0<kDppWaveShr1, 0xf>((uint32_t)b);
+        // events, in step order
+        const int jt = ctz64(__ballot(b >= 64));   // first column left at the top (64: none)
+        const int b63 = __builtin_amdgcn_readlane(b, 63);
+        const int X = jt < 64 ? 64 + jt : b63 + 64;   // step at which the walk leaves the window
+        int kstop = 1 << 30;                          // step of a stop (evaluated, then the walk ends)
+        const int js = s - c0;
+        if (js >= 0 && js < 64) {
+            const int bjs = __builtin_amdgcn_readlane(b, js);
+            if (bjs < 64 && jt > js) kstop = bjs + js;
         }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            lw[(4 * g + 0) * 64] = wr[g].x;
-            lw[(4 * g + 1) * 64] = wr[g].y;
-            lw[(4 * g + 2) * 64] = wr[g].z;
-            lw[(4 * g + 3) * 64] = wr[g].w;
-            lw[(4 * G + 4 * g + 0) * 64] = wc[g].x;
-            lw[(4 * G + 4 * g + 1) * 64] = wc[g].y;
-            lw[(4 * G + 4 * g + 2) * 64] = wc[g].z;
-            lw[(4 * G + 4 * g + 3) * 64] = wc[g].w;
+        if (r0 < 64) {
+            const uint64_t rm = __ballot(b > r0);
+            if (rm) kstop = min(kstop, r0 + ctz64(rm));
         }
-        // slots: row r-1 at ir (read as slot 0 when r = 0, where it is never
-        // used), column c at 4G + ic; within K steps ic + 1 < 4G
-        int ir = r - 1 - ar, ic = c - ac;
-        uint32_t up = lw[(ir > 0 ? ir : 0) * 64];
-        uint32_t nxt = lw[(4 * G + ic) * 64];
-#pragma unroll 2
-        for (int k = 0; k < K; ++k) {
-            const uint32_t up2 = lw[(ir > 1 ? ir - 1 : 0) * 64];   // row r-2
-            const uint32_t nxt2 = lw[(4 * G + ic + 1) * 64];      // column c+1
-            const int wcol = ((int)(left & kValMask) >= r) ^ (int)(left >> 31);
-            const int wrow = ((int)(ur & kValMask) >= c) ^ (int)(ur >> 31);
-            if (alive && c > 0 && d - wcol < 0 && d + wrow > 0) put(r, c - 1);
-            const int rt = (int)((uint32_t)d >> 31);   // 1: move right (delta < 0), 0: up
-            const int stop = (rt & (c >= s)) | ((rt ^ 1) & (r == 0));
-            alive = alive && !stop;
-            const int live = -(int)alive;                  // all ones when this lane moves
-            const uint32_t rm = 0u - (uint32_t)rt;
-            const uint32_t v = (nxt & rm) | (up & ~rm);    // the point the move passes
-            const int thr = c + ((r - c) & (int)rm);
-            const int dd = ((int)(v & kValMask) >= thr) ^ (int)(v >> 31);
-            const int up1 = rt ^ 1;
-            d += ((dd ^ -up1) + up1) & live;
-            const uint32_t mr = rm & (uint32_t)live, mu = ~rm & (uint32_t)live;   // moved right / up
-            left = (v & mr) | (left & ~mr);
-            ur = (v & mu) | (ur & ~mu);
-            nxt = (nxt2 & mr) | (nxt & ~mr);
-            up = (up2 & mu) | (up & ~mu);
-            r -= up1 & live;
-            c += rt & live;
-            ir -= up1 & live;
-            ic += rt & live;
-            ++t;
-            alive = alive && t < steps;
+        const bool stops = kstop + 1 <= B;
+        const int lim = stops ? kstop + 1 : min(B, X);   // steps [0, lim) are evaluated here
+        // the good point of column j, at (i*, j)
+        const uint32_t vl_sh = dpp_or0<kDppWaveShr1, 0xf>(v);
+        const uint32_t vl = lane == 0 ? v_left : vl_sh;
+        const bool rt_r0 = ((int)(u_r0 & kValMask) >= cc) != (u_r0 >> 31 != 0);
+        const bool ct_l = ((int)(vl & kValMask) >= r0 - istar) != (vl >> 31 != 0);
+        if (istar < 64 && a <= istar && istar + lane < lim && cc >= 1 && (istar > 0 || rt_r0) && ct_l)
+            put(r0 - istar, cc - 1);
+        if (stops) break;
+        if (B < X) {   // the budget ends inside the window: the position of step B
+            const int j = ctz64(__ballot(a + lane <= B && B <= b + lane));
+            const int i = B - j;   // <= 63: step B precedes the window's exit
+            const int dj = __builtin_amdgcn_readlane(D - __popcll(RT & ((1ull << i) - 1ull)), j);
+            end = make_int3(r0 - i, c0 + j, dj);
+            break;
+        }
+        // continue from the exit: the top of column jt, or the right side at row b_63
+        int ie, je, dl;
+        if (jt < 64) {
+            ie = 64;
+            je = jt;
+            dl = __builtin_amdgcn_readlane(D - __popcll(RT), jt);
+        } else {
+            ie = b63;
+            je = 64;
+            const int ct = ((vrow >= r0 - b63) != (v >> 31 != 0)) ? 1 : 0;
+            dl = __builtin_amdgcn_readlane(D - __popcll(RT & ((1ull << b63) - 1ull)) + ct, 63);
+        }
+        r0 = __builtin_amdgcn_readfirstlane(r0 - ie);
+        c0 = __builtin_amdgcn_readfirstlane(c0 + je);
+        d0 = __builtin_amdgcn_readfirstlane(dl);
+        B = __builtin_amdgcn_readfirstlane(B - X);
+        if (B == 0) {   // the budget ends exactly where the walk leaves the window
+            end = make_int3(r0, c0, d0);
+            more = false;
         }
     }
+    return end;
 }
 
 // ---------------------------------------------------------------------------
@@ -377,10 +472,8 @@ __device__ __forceinline__ void ant_walk_window4(int r, int c, int d, int s, int
 // inside its tile, from the split down to size 1 and back up, in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kSmallLevels = 10;   // log2(kTile)
-constexpr int kSmallSeg = 32;      // walk steps per segment inside the small solver
-static_assert(kSmallSeg <= 64, "a run's row sum takes one row per lane");
-constexpr int kSmallBoxS = 16;     // from this subproblem size on, walks are cut into searched segments
-constexpr int kSmallMaxSeg = 64;   // segments per level and tile: max(kTile / kSmallBoxS, 2 kTile / kSmallSeg)
+constexpr int kSmallSeg = 128;     // walk steps per segment inside the small solver (one wave each)
+constexpr int kSmallBoxS = 16;     // from this subproblem size on, walks are walked by whole waves
 constexpr uint32_t kHi16 = 0x8000u;
 
 // LDS points of the small solver: value | hi bit 15 (values < kTile).
@@ -456,14 +549,13 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
     __shared__ uint16_t sU[kTile], sUi[kTile];
     __shared__ int16_t segP[kTile], segQ[kTile];
     __shared__ int sCend[kTile / 2];
-    __shared__ int4 sStart[kSmallMaxSeg];
     __shared__ int tmp[kTile / 64 + 1];
     const int t = blockIdx.x;
     const SaProb pb = probs[tile_prob[t]];
     const int64_t base = (int64_t)t * kTile;
     const int len = (int)min((int64_t)kTile, pb.n - (base - pb.base));
     const int x = threadIdx.x;
-    const int lane = x & 63, wid = x >> 6;
+    const int wid = x >> 6;
     const bool in = x < len;
     if (in) {
         sP[0][x] = (int16_t)P0[base + x];
@@ -536,58 +628,50 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
                 }
             }
         } else {
-            // segments of kSmallSeg steps.  Their start points are found on
-            // anti-diagonals, each wave taking a run of consecutive segments;
-            // then every segment is walked by its own lane, so the walks run
-            // side by side on the vector ALU.  A run's first segment (or one
-            // that starts another subproblem) takes D at the diagonal's lower
-            // end by an LDS scan and one window pass over the diagonal; the
-            // next segment of the same walk crosses within kSmallSeg columns
-            // of this crossing c, and D there at c is this D(c) less the row
-            // terms of the kSmallSeg rows between the two diagonals.
-            const int nsps = max(1, (2 * S) / kSmallSeg);
+            // segments of kSmallSeg evaluations, each walked by one wave
+            // (wave_walk from LDS).  A wave takes a run of consecutive
+            // segments: one that continues the walk its previous segment
+            // ended starts where that one ended; a subproblem's first segment
+            // starts at the walk's origin (s - 1, 0); any other is found on its
+            // anti-diagonal t0: D at the diagonal's lower end by an LDS scan,
+            // then one window pass over the diagonal.
+            const int nsps = (2 * S + kSmallSeg - 1) / kSmallSeg;   // covers the walk's <= 2s evaluations
             const int nsub = (len + S - 1) / S;
-            const int nseg = nsub * nsps;   // <= kSmallMaxSeg
+            const int nseg = nsub * nsps;
             const int per = (nseg + kTile / 64 - 1) / (kTile / 64);
-            int prev = -1, pc = 0, pD = 0;   // last segment of this run with a crossing: index, c, D(c)
-            for (int w = wid * per; w < min(nseg, (wid + 1) * per); ++w) {
+            int3 cont = make_int3(-1, 0, 0);   // where this wave's previous segment ended (-1: none)
+            for (int w = wid * per; w < min(nseg, (wid + 1) * per); ++w) {   // uniform over the wave
                 const int sub = w / nsps, k = w - sub * nsps;
                 const int s0 = sub * S, s = min(S, len - s0);
                 const int t_end = s - 1 + sCend[sub];
                 const int t0 = k * kSmallSeg;
-                int4 st = make_int4(-1, 0, 0, 0);
-                if (s > h && t0 <= t_end) {   // uniform over the wave
-                    const uint16_t *U16 = sU + s0, *Ui16 = sUi + s0;
+                if (s <= h || t0 > t_end) {   // not split at this level, or the walk ended before t0
+                    cont.x = -1;
+                    continue;
+                }
+                const uint16_t *U16 = sU + s0, *Ui16 = sUi + s0;
+                int r, c, d;
+                if (k == 0) {
+                    r = s - 1;
+                    c = 0;
+                    d = (U16[s - 1] & kHi16) ? 0 : -1;
+                } else if (w > wid * per && cont.x >= 0) {
+                    r = cont.x;
+                    c = cont.y;
+                    d = cont.z;
+                } else {
                     const int off = s - t0;
                     const int lo = max(0, t0 - s + 1), hi = min(t0, s);
-                    int2 cd;
-                    if (prev == w - 1 && k > 0 && pc >= lo) {
-                        // rows [pc + off, pc + off + kSmallSeg) lie in [1, s - 1]
-                        const int acc = lane < kSmallSeg ? row_term16(U16[pc + off + lane], pc) : 0;
-                        const int Da = pD - __builtin_amdgcn_readfirstlane(wave_sum(acc));
-                        cd = wave_window_lds(U16, Ui16, off, pc, min(pc + kSmallSeg, hi), Da);
-                    } else {
-                        cd = wave_window_lds(U16, Ui16, off, lo, hi, wave_delta_lds(U16, s, lo + off, lo));
-                    }
-                    const int r = cd.x + off - 1;
-                    st = make_int4(r, cd.x, cd.y - row_term16(U16[r], cd.x),
-                                   nsps == 1 ? t_end + 1 : min(kSmallSeg, t_end - t0 + 1));
-                    prev = w;
-                    pc = cd.x;
-                    pD = cd.y;
+                    const int2 cd = wave_window_lds(U16, Ui16, off, lo, hi, wave_delta_lds(U16, s, lo + off, lo));
+                    c = cd.x;
+                    r = cd.x + off - 1;
+                    d = cd.y - row_term16(U16[r], cd.x);
                 }
-                if (lane == 0) sStart[w] = st;
-            }
-            __syncthreads();
-            if (x < nseg) {
-                const int4 st = sStart[x];
-                if (st.x >= 0) {
-                    const int sub = x / nsps, s0 = sub * S, s = min(S, len - s0);
-                    const uint16_t *U16 = sU + s0, *Ui16 = sUi + s0;
-                    int16_t *R = sR + s0;
-                    ant_walk(st.x, st.y, st.z, s, st.w, [&](int rr) { return widen16(U16[rr]); },
-                             [&](int cc) { return widen16(Ui16[cc]); }, [&](int rr, int cc) { R[rr] = (int16_t)cc; });
-                }
+                int16_t *R = sR + s0;
+                cont = wave_walk(__builtin_amdgcn_readfirstlane(r), __builtin_amdgcn_readfirstlane(c),
+                                 __builtin_amdgcn_readfirstlane(d), s, min(kSmallSeg, t_end - t0 + 1),
+                                 [&](int rr) { return widen16(U16[rr]); }, [&](int cc) { return widen16(Ui16[cc]); },
+                                 [&](int rr, int cc) { R[rr] = (int16_t)cc; });
             }
         }
         __syncthreads();
@@ -801,7 +885,7 @@ __device__ int wave_delta(const uint32_t *__restrict__ U, const uint32_t *__rest
 // rounded down to a multiple of 4, so it settles b - a <= 64 PER - 4.
 constexpr int kWinPer = 16;                   // positions per lane after a binary search
 constexpr int kWinSpan = 64 * kWinPer - 4;    // b - a bound for that pass
-constexpr int kIncPer = 8;                    // positions per lane for a run's next segment
+constexpr int kIncPer = 12;                   // positions per lane for a run's next segment
 static_assert(64 * kIncPer - 4 >= kSegSteps, "a run's next crossing lies within kSegSteps columns");
 static_assert(kSegSteps % 64 == 0, "row sums take kSegSteps / 64 rows per lane");
 
@@ -971,174 +1055,6 @@ __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__res
     }
 }
 
-// ---------------------------------------------------------------------------
-// The walk by a whole wave, 64 x 64 positions per round.
-// ---------------------------------------------------------------------------
-// A window is anchored at the walk's position (r0, c0) with d0 = delta(r0,
-// c0).  Position (i, j) is (r0 - i, c0 + j); lane j holds column c0 + j's
-// point and bit i of a 64-bit mask stands for row r0 - 1 - i, the row an up
-// move from (i, j) passes.  Then
-//     delta(i, j) = D_j - popcount(RT_j & (2^i - 1)),
-// D_j = delta(r0, c0 + j) (d0 plus the column terms of columns c0 .. c0+j-1
-// at row r0: one ballot and a masked bit count) and RT_j the row terms of
-// the window rows at column c0 + j.  A row's term changes with the column
-// only at the row's own point, so RT_j is RT_0 with the bits of the rows
-// whose points lie in columns [c0, c0 + j) flipped: a prefix XOR across the
-// lanes of one-hot row bits.
-//
-// delta is nonincreasing in i and nondecreasing in j.  The walk goes up while
-// delta >= 0 and right otherwise, so with tau_j the first i with
-// delta(i, j) < 0 it leaves column j at row b_j = max(b_{j-1}, tau_j)
-// (b_{-1} = 0): a prefix maximum across lanes, and it enters column j at
-// a_j = b_{j-1}.  The position of step k of the window is (k - j, j) for the
-// j with a_j + j <= k <= b_j + j.
-//
-// A good point of column j (delta(i, j) = 0, delta(i - 1, j) = 1 and
-// delta(i, j - 1) = -1, the reference's test) can only be at i* = the first
-// i with delta(i, j) <= 0 (delta moves by at most 1 per row): the D_j-th set
-// bit of RT_j plus 1 (0 when D_j = 0); then tau_j is one past the next set
-// bit from i*.  It is reported when the walk evaluates (i*, j): a_j <= i* and
-// i* + j inside the steps this window evaluates.
-//
-// The window ends the segment at its step budget or where the walk leaves the
-// square (moving right at column s, or up at row 0: positions evaluated and
-// then the walk stops), else the walk continues from where it leaves the
-// window, at the top (row i = 64 of the first column with b_j >= 64) or the
-// right side (column 64 at row b_63).  Lanes outside the square (rows below
-// 0, columns past s - 1) hold the value 0 and only affect positions after
-// the stop there.  Every value here is the sequential walk's (ant_walk):
-// the same positions, budget and good points.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ uint32_t dpp_or0(uint32_t v) {   // lanes without a source read 0
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
-}
-constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
-constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
-
-// Inclusive scans across the wave: within rows of 16 lanes by shifts of 1, 2,
-// 4 and 8, then row 0's total into row 1 (and row 2's into row 3), then rows
-// 0-1's total into rows 2 and 3.
-__device__ __forceinline__ uint64_t wave_xor_scan64(uint64_t x) {
-    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    lo ^= dpp_or0<kDppRowShr1, 0xf>(lo);
-    hi ^= dpp_or0<kDppRowShr1, 0xf>(hi);
-    lo ^= dpp_or0<kDppRowShr2, 0xf>(lo);
-    hi ^= dpp_or0<kDppRowShr2, 0xf>(hi);
-    lo ^= dpp_or0<kDppRowShr4, 0xf>(lo);
-    hi ^= dpp_or0<kDppRowShr4, 0xf>(hi);
-    lo ^= dpp_or0<kDppRowShr8, 0xf>(lo);
-    hi ^= dpp_or0<kDppRowShr8, 0xf>(hi);
-    lo ^= dpp_or0<kDppRowBcast15, 0xa>(lo);
-    hi ^= dpp_or0<kDppRowBcast15, 0xa>(hi);
-    lo ^= dpp_or0<kDppRowBcast31, 0xc>(lo);
-    hi ^= dpp_or0<kDppRowBcast31, 0xc>(hi);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ int wave_max_scan(int x) {   // x >= 0
-    x = max(x, (int)dpp_or0<kDppRowShr1, 0xf>((uint32_t)x));
-    x = max(x, (int)dpp_or0<kDppRowShr2, 0xf>((uint32_t)x));
-    x = max(x, (int)dpp_or0<kDppRowShr4, 0xf>((uint32_t)x));
-    x = max(x, (int)dpp_or0<kDppRowShr8, 0xf>((uint32_t)x));
-    x = max(x, (int)dpp_or0<kDppRowBcast15, 0xa>((uint32_t)x));
-    x = max(x, (int)dpp_or0<kDppRowBcast31, 0xc>((uint32_t)x));
-    return x;
-}
-
-__device__ __forceinline__ int ctz64(uint64_t x) {   // 64 when x = 0
-    const int f = __ffsll((unsigned long long)x);
-    return f ? f - 1 : 64;
-}
-
-// Index of the k-th (k >= 1) set bit of x, 64 when x has fewer: halves,
-// quarters, ... chosen by bit counts.
-__device__ __forceinline__ int select64(uint64_t x, int k) {
-    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    int c = __popc(lo);
-    int base = k > c ? 32 : 0;
-    uint32_t w = k > c ? hi : lo;
-    k -= k > c ? c : 0;
-#pragma unroll
-    for (int half = 16; half >= 1; half >>= 1) {
-        c = __popc(w & ((1u << half) - 1u));
-        const bool up = k > c;
-        base += up ? half : 0;
-        w = up ? w >> half : w;
-        k -= up ? c : 0;
-    }
-    return (k == 1 && (w & 1u)) ? base : 64;
-}
-
-// The walk from (r0, c0) with d0 = delta(r0, c0) for at most B evaluations
-// (B >= 1), by the whole wave; every argument is uniform.  rowv(r) gives row
-// r's point (col | hi bit 31), colv(c) column c's point (row | hi bit 31) for
-// r, c in [0, s); good points go to put(row, col).
-template <class RowF, class ColF, class PutF>
-__device__ __forceinline__ void wave_walk(int r0, int c0, int d0, int s, int B, RowF rowv, ColF colv, PutF put) {
-    const int lane = threadIdx.x & 63;
-    for (bool more = true; more;) {
-        const int rr = r0 - 1 - lane, cc = c0 + lane;
-        const uint32_t u_ld = rowv(rr >= 0 ? rr : 0), v_ld = colv(cc < s ? cc : s - 1);
-        const uint32_t u_r0 = rowv(r0);
-        const uint32_t v_left = colv(c0 > 0 ? c0 - 1 : 0);
-        const uint32_t u = rr >= 0 ? u_ld : 0u, v = cc < s ? v_ld : 0u;
-        const int ucol = (int)(u & kValMask), vrow = (int)(v & kValMask);
-        const uint64_t RT0 = __ballot((ucol >= c0) != (u >> 31 != 0));
-        const int ih = r0 - 1 - vrow;
-        const uint64_t oh = (cc < s && ih >= 0 && ih < 64) ? (1ull << ih) : 0ull;
-        const uint64_t RT = RT0 ^ wave_xor_scan64(oh) ^ oh;
-        const uint64_t CT = __ballot(cc < s && ((vrow >= r0) != (v >> 31 != 0)));
-        const int D = d0 + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(CT >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)CT, 0u));
-        const int p = select64(RT, D > 1 ? D : 1);
-        const int istar = D < 0 ? 64 : (D == 0 ? 0 : (p < 64 ? p + 1 : 64));
-        const int tnext = istar < 64 ? istar + 1 + ctz64(RT >> istar) : 64;
-        const int tau = D < 0 ? 0 : min(tnext, 64);
-        const int b = wave_max_scan(tau);
-        const int a = (int)dpp_or0<kDppWaveShr1, 0xf>((uint32_t)b);
-        // events, in step order
-        const int jt = ctz64(__ballot(b >= 64));   // first column left at the top (64: none)
-        const int b63 = __builtin_amdgcn_readlane(b, 63);
-        const int X = jt < 64 ? 64 + jt : b63 + 64;   // step at which the walk leaves the window
-        int kstop = 1 << 30;                          // step of a stop (evaluated, then the walk ends)
-        const int js = s - c0;
-        if (js >= 0 && js < 64) {
-            const int bjs = __builtin_amdgcn_readlane(b, js);
-            if (bjs < 64 && jt > js) kstop = bjs + js;
-        }
-        if (r0 < 64) {
-            const uint64_t rm = __ballot(b > r0);
-            if (rm) kstop = min(kstop, r0 + ctz64(rm));
-        }
-        const bool ends = kstop + 1 <= B || B <= X;
-        const int lim = kstop + 1 <= B ? kstop + 1 : min(B, X);   // steps [0, lim) are evaluated here
-        // the good point of column j, at (i*, j)
-        const uint32_t vl_sh = dpp_or0<kDppWaveShr1, 0xf>(v);
-        const uint32_t vl = lane == 0 ? v_left : vl_sh;
-        const bool rt_r0 = ((int)(u_r0 & kValMask) >= cc) != (u_r0 >> 31 != 0);
-        const bool ct_l = ((int)(vl & kValMask) >= r0 - istar) != (vl >> 31 != 0);
-        if (istar < 64 && a <= istar && istar + lane < lim && cc >= 1 && (istar > 0 || rt_r0) && ct_l)
-            put(r0 - istar, cc - 1);
-        more = !ends;
-        // continue from the exit: the top of column jt, or the right side at row b_63
-        int ie, je, dl;
-        if (jt < 64) {
-            ie = 64;
-            je = jt;
-            dl = __builtin_amdgcn_readlane(D - __popcll(RT), jt);
-        } else {
-            ie = b63;
-            je = 64;
-            const int ct = ((vrow >= r0 - b63) != (v >> 31 != 0)) ? 1 : 0;
-            dl = __builtin_amdgcn_readlane(D - __popcll(RT & ((1ull << b63) - 1ull)) + ct, 63);
-        }
-        r0 = __builtin_amdgcn_readfirstlane(r0 - ie);
-        c0 = __builtin_amdgcn_readfirstlane(c0 + je);
-        d0 = __builtin_amdgcn_readfirstlane(dl);
-        B = __builtin_amdgcn_readfirstlane(B - X);
-    }
-}
-
 // Walks, large levels: one wave per segment (wave_walk over the level's U and
 // U^-1).  Blocks of 4 waves; the hardware deals blocks to the 8 XCDs
 // round-robin, so block b is mapped to segments in contiguous ranges per XCD
@@ -1160,39 +1076,9 @@ __global__ __launch_bounds__(256) void seg_walk_wave(const uint32_t *__restrict_
     const int t_end = s - 1 + c_end[job.x];
     const uint32_t *Us = U + sg.x0, *Uis = Ui + sg.x0;
     int32_t *Rs = R + sg.x0;
-    wave_walk(r0, __builtin_amdgcn_readfirstlane(st.y), __builtin_amdgcn_readfirstlane(st.z), s,
+    (void)wave_walk(r0, __builtin_amdgcn_readfirstlane(st.y), __builtin_amdgcn_readfirstlane(st.z), s,
               __builtin_amdgcn_readfirstlane(min(kSegSteps, t_end - st.w + 1)), [&](int x) { return Us[x]; },
               [&](int x) { return Uis[x]; }, [&](int rr, int cc) { Rs[rr] = cc; });
-}
-
-// Walks: one lane per segment, reading the level's U / U^-1 a window of
-// kWalkWindow rows and columns per round trip (ant_walk_window).  Blocks of
-// kWalkBlock lanes spread the few thousand segments of a level over many
-// CUs, and are mapped to XCDs in contiguous ranges of segments (the hardware
-// deals blocks to the 8 XCDs round-robin), so each XCD's L2 holds the rows
-// and columns of its own part of the diagonal range.
-__global__ __launch_bounds__(kWalkBlock) void seg_walk(const uint32_t *__restrict__ U,
-                                                       const uint32_t *__restrict__ Ui,
-                                                       const SaSeg *__restrict__ segs,
-                                                       const int2 *__restrict__ walks, int nwalks,
-                                                       const int32_t *__restrict__ c_end,
-                                                       const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
-    __shared__ uint32_t win[kWalkBlock * 8 * kWalkGroups];
-    const int per_xcd = (int)gridDim.x / 8;   // the grid is a multiple of 8 blocks
-    const int b = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);   // XCD-contiguous block order
-    const int w = b * kWalkBlock + (int)threadIdx.x;
-    if (w >= nwalks) return;
-    const int4 st = seg_start[w];
-    if (st.x < 0) return;
-    const int2 job = walks[w];
-    const SaSeg sg = segs[job.x];
-    const uint32_t *Us = U + sg.x0, *Uis = Ui + sg.x0;
-    int32_t *Rs = R + sg.x0;
-    const int s = sg.s;
-    const int t_end = s - 1 + c_end[job.x];
-    uint32_t *lw = win + (threadIdx.x >> 6) * (64 * 8 * kWalkGroups) + (threadIdx.x & 63);
-    ant_walk_window4(st.x, st.y, st.z, s, min(kSegSteps, t_end - st.w + 1), Us, Uis,
-                     [&](int rr, int cc) { Rs[rr] = cc; }, lw);
 }
 
 __global__ void fill_i32(int32_t *__restrict__ p, int32_t v, int64_t count) {
