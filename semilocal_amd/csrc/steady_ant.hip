@@ -472,8 +472,8 @@ __device__ __forceinline__ int3 wave_walk(int r0, int c0, int d0, int s, int B, 
 // inside its tile, from the split down to size 1 and back up, in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kSmallLevels = 10;   // log2(kTile)
-constexpr int kSmallSeg = 128;     // walk steps per segment inside the small solver (one wave each)
-constexpr int kSmallBoxS = 16;     // from this subproblem size on, walks are walked by whole waves
+constexpr int kSmallSeg = 512;     // walk steps per segment inside the small solver (one wave each)
+constexpr int kSmallBoxS = 32;     // from this subproblem size on, walks are walked by whole waves
 constexpr uint32_t kHi16 = 0x8000u;
 
 // LDS points of the small solver: value | hi bit 15 (values < kTile).
