@@ -287,10 +287,12 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
                            p->d_U, p->d_Ui, p->d_R[rc ^ 1], cend);
         rc ^= 1;
         if (L.nwalk == 0) continue;
-        hipLaunchKernelGGL(table_rows, dim3((unsigned)L.nrow), dim3(256), 0, s, p->d_U, segs, p->d_rows + L.row0,
-                           p->d_cntHi, p->d_cntLo);
-        hipLaunchKernelGGL(table_cols, dim3((unsigned)ceil_div(L.ncol, 4)), dim3(256), 0, s, segs,
-                           p->d_cols + L.col0, L.ncol, p->d_cntHi, p->d_cntLo);
+        if (L.S > kDirectNb * kBlk) {   // smaller subproblems are searched by direct scans
+            hipLaunchKernelGGL(table_rows, dim3((unsigned)L.nrow), dim3(256), 0, s, p->d_U, segs, p->d_rows + L.row0,
+                               p->d_cntHi, p->d_cntLo);
+            hipLaunchKernelGGL(table_cols, dim3((unsigned)ceil_div(L.ncol, 4)), dim3(256), 0, s, segs,
+                               p->d_cols + L.col0, L.ncol, p->d_cntHi, p->d_cntLo);
+        }
         hipLaunchKernelGGL(seg_search, dim3((unsigned)ceil_div(L.nwalk, 4 * kSearchRun)), dim3(256), 0, s, p->d_U, p->d_Ui,
                            p->d_cntHi, p->d_cntLo, segs, walks, L.nwalk, cend, p->d_start);
         hipLaunchKernelGGL(seg_walk_wave, dim3((unsigned)(8 * ceil_div(ceil_div(L.nwalk, 4), 8))), dim3(256), 0, s,

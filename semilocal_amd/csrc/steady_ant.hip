@@ -169,11 +169,12 @@ __global__ __launch_bounds__(kSplitThreads) void split_scatter(const int32_t *__
     const int t = blockIdx.x;
     const SaProb pb = probs[tile_prob[t]];
     const int64_t x0 = (int64_t)t * kTile + kSplitPer * threadIdx.x;
-    const int64_t lx0 = x0 - pb.base;
+    const int lx0 = (int)(x0 - pb.base);   // local index (problems are below 2^31 elements)
+    const int lg = __builtin_ctz((unsigned)S);   // S is a power of two: shifts, not divisions
     int offp = 0, offq = 0;
     {
-        const int64_t lt0 = (int64_t)t * kTile - pb.base;
-        const int64_t tile0 = (pb.base + (lt0 / S) * S) / kTile;
+        const int lt0 = (int)((int64_t)t * kTile - pb.base);
+        const int64_t tile0 = (pb.base + ((lt0 >> lg) << lg)) / kTile;   // base is a multiple of kTile
         int vp = 0, vq = 0;
         for (int64_t u = tile0 + threadIdx.x; u < t; u += kSplitThreads) {
             vp += cntP[u];
@@ -205,8 +206,8 @@ __global__ __launch_bounds__(kSplitThreads) void split_scatter(const int32_t *__
     }
     int tpq;
     int ex = block_excl_scan(mine, tmp, tpq);
-    const int64_t seg0 = pb.base + (lx0 / S) * S;   // a tile, so a thread's elements, lie in one segment
-    const int r0 = (int)(lx0 % S);
+    const int64_t seg0 = pb.base + ((lx0 >> lg) << lg);   // a tile, so a thread's elements, lie in one segment
+    const int r0 = lx0 & (S - 1);
 #pragma unroll
     for (int e = 0; e < kSplitPer; ++e) {
         if (lx0 + e < pb.n) {
@@ -701,21 +702,21 @@ __global__ __launch_bounds__(kSplitThreads) void merge_map(const int32_t *__rest
     const int pid = tile_prob[t];
     const SaProb pb = probs[pid];
     const int64_t x0 = (int64_t)t * kTile + kSplitPer * threadIdx.x;
-    const int64_t lx0 = x0 - pb.base;
-    const int h = S >> 1;
+    const int lx0 = (int)(x0 - pb.base);   // local index (problems are below 2^31 elements)
+    const int lg = __builtin_ctz((unsigned)S);   // S is a power of two: shifts, not divisions
     const int4 r4 = *reinterpret_cast<const int4 *>(Rc + x0);
     const int4 pr4 = *reinterpret_cast<const int4 *>(PRc + x0);
     const int rv[kSplitPer] = {r4.x, r4.y, r4.z, r4.w}, prv[kSplitPer] = {pr4.x, pr4.y, pr4.z, pr4.w};
     int cand = 0;
 #pragma unroll
     for (int e = 0; e < kSplitPer; ++e) {
-        const int64_t lx = lx0 + e;
+        const int lx = lx0 + e;
         if (lx < pb.n) {
-            const int64_t cid = lx / h;
+            const int cid = lx >> (lg - 1);   // child: lx / h
             const uint32_t bit = (cid & 1) ? kHiBit : 0u;
-            const int64_t seg0 = pb.base + (lx / S) * S;
+            const int64_t seg0 = pb.base + ((lx >> lg) << lg);
             const int prow = prv[e];
-            const int pcol = QCc[pb.base + cid * h + rv[e]];
+            const int pcol = QCc[pb.base + ((int64_t)cid << (lg - 1)) + rv[e]];
             U[seg0 + prow] = (uint32_t)pcol | bit;
             Ui[seg0 + pcol] = (uint32_t)prow | bit;
             R[seg0 + prow] = pcol;
@@ -730,8 +731,7 @@ __global__ __launch_bounds__(kSplitThreads) void merge_map(const int32_t *__rest
         int m = 0;
 #pragma unroll
         for (int w = 0; w < kSplitThreads / 64; ++w) m = max(m, wmax[w]);
-        const int64_t lt0 = (int64_t)t * kTile - pb.base;
-        if (m > 0) atomicMax(&c_end[seg_first[pid] + lt0 / S], m);
+        if (m > 0) atomicMax(&c_end[seg_first[pid] + ((int)((int64_t)t * kTile - pb.base) >> lg)], m);
     }
 }
 
@@ -873,6 +873,31 @@ __device__ int wave_delta(const uint32_t *__restrict__ U, const uint32_t *__rest
         }
     }
     return __builtin_amdgcn_readfirstlane(base + wave_sum(acc));
+}
+
+// delta(r, c) of a subproblem of at most kDirectNb * kBlk elements by one scan
+// of all its rows (uniform result): aligned 16-byte groups from the
+// subproblem's 1024-aligned start, entries at or past s masked (a group that
+// starts below s ends at most 3 entries past it, inside the level arrays).
+__device__ int wave_delta_direct(const uint32_t *__restrict__ Ux, int s, int r, int c) {
+    const int lane = threadIdx.x & 63;
+    constexpr int kG = kDirectNb * kBlk / 256;   // 16-byte groups per lane
+    const uint4 *U4 = reinterpret_cast<const uint4 *>(Ux);
+    uint4 u[kG];
+#pragma unroll
+    for (int k = 0; k < kG; ++k) u[k] = k * 256 + lane * 4 < s ? U4[k * 64 + lane] : make_uint4(0u, 0u, 0u, 0u);
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < kG; ++k) {
+        const uint32_t uu[4] = {u[k].x, u[k].y, u[k].z, u[k].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int rho = k * 256 + lane * 4 + e;
+            const int col = (int)(uu[e] & kValMask);
+            if (rho < s) acc += (uu[e] >> 31) ? (rho < r && col < c ? 1 : 0) : (rho >= r && col >= c ? -1 : 0);
+        }
+    }
+    return __builtin_amdgcn_readfirstlane(wave_sum(acc));
 }
 
 // Along an anti-diagonal, with R(c) = c + off, D(c) = delta(R(c), c) moves by
@@ -1018,26 +1043,47 @@ __global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__res
                 for (int k = 0; k < kSegSteps / 64; ++k) acc += row_term(Ux[pc + off + lane + 64 * k], pc);
                 const int Da = pD - __builtin_amdgcn_readfirstlane(wave_sum(acc));
                 cd = wave_window_search<kIncPer>(Ux, Ui + sg.x0, off, pc, min(pc + kSegSteps, hi), Da);
+            } else if (sg.nb <= kDirectNb) {
+                // a small subproblem: halvings by direct scans, no tables
+                int a = lo, b = hi;
+                while (b - a > kWinSpan) {
+                    const int mid = (a + b) >> 1;
+                    if (wave_delta_direct(Ux, s, mid + off, mid) >= 0) b = mid;
+                    else a = mid + 1;
+                }
+                cd = wave_window_search<kWinPer>(Ux, Ui + sg.x0, off, a, b, wave_delta_direct(Ux, s, a + off, a));
             } else {
                 // delta is nondecreasing in r and in c, so the count tables'
                 // corners around (R, c) bound D(c): delta(i kBlk, j kBlk) <=
                 // D(c) <= delta((i+1) kBlk, (j+1) kBlk).  Most halvings are
                 // decided by the two table entries; only where the bounds
                 // straddle 0 does a probe read the strips.
+                // Each round probes 64 positions strictly inside [a, b),
+                // one per lane, in one memory round trip: the last probe
+                // whose upper bound is negative and the first whose lower
+                // bound is not narrow the range (D is nondecreasing along the
+                // diagonal); where every probe straddles 0, one exact probe
+                // halves it.
                 const int W = sg.nb + 1;
                 const int32_t *th = cntHi + sg.tab_off2, *tl = cntLo + sg.tab_off2;
                 int a = lo, b = hi;
                 while (b - a > kWinSpan) {
-                    const int mid = (a + b) >> 1;
-                    const int i = (mid + off) / kBlk, j = mid / kBlk;
+                    const int p = a + (int)(((int64_t)(b - a) * (lane + 1)) / 65);   // a < p < b
+                    const int i = (p + off) / kBlk, j = p / kBlk;
                     const int i1 = min(i + 1, sg.nb), j1 = min(j + 1, sg.nb);
                     const int64_t t0i = (int64_t)j * W + i, t1i = (int64_t)j1 * W + i1;
-                    bool ge;
-                    if (th[t0i] - tl[t0i] >= 0) ge = true;
-                    else if (th[t1i] - tl[t1i] < 0) ge = false;
-                    else ge = wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0;
-                    if (ge) b = mid;
-                    else a = mid + 1;
+                    const int lower = th[t0i] - tl[t0i], upper = th[t1i] - tl[t1i];
+                    const uint64_t neg = __ballot(upper < 0), nonneg = __ballot(lower >= 0);
+                    const int na = neg ? __builtin_amdgcn_readlane(p, 63 - __clzll((long long)neg)) + 1 : a;
+                    const int nb = nonneg ? __builtin_amdgcn_readlane(p, ctz64(nonneg)) : b;
+                    if (na == a && nb == b) {
+                        const int mid = (a + b) >> 1;
+                        if (wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0) b = mid;
+                        else a = mid + 1;
+                    } else {
+                        a = na;
+                        b = nb;
+                    }
                 }
                 const int Da = wave_delta(U, Ui, cntHi, cntLo, sg, a + off, a);
                 cd = wave_window_search<kWinPer>(Ux, Ui + sg.x0, off, a, b, Da);
