@@ -45,10 +45,8 @@ __global__ void merge_map(const int32_t *, const int32_t *, const int32_t *, con
                           const int32_t *, int, int64_t, uint32_t *, uint32_t *, int32_t *, int32_t *);
 __global__ void table_rows(const uint32_t *, const SaSeg *, const int2 *, int32_t *, int32_t *);
 __global__ void table_cols(const SaSeg *, const int2 *, int, int32_t *, int32_t *);
-__global__ void seg_search(const uint32_t *, const uint32_t *, const int32_t *, const int32_t *, const SaSeg *,
-                           const int2 *, int, const int32_t *, int4 *);
-__global__ void seg_walk_wave(const uint32_t *, const uint32_t *, const SaSeg *, const int2 *, int, const int32_t *,
-                              const int4 *, int32_t *);
+__global__ void seg_search_walk(const uint32_t *, const uint32_t *, const int32_t *, const int32_t *, const SaSeg *,
+                                const int2 *, int, const int32_t *, int32_t *);
 __global__ void fill_i32(int32_t *, int32_t, int64_t);
 __global__ void stag_qinv(const int32_t *, const StagJob *, const int32_t *, int32_t *);
 __global__ void stag_count(const int32_t *, const int32_t *, const StagJob *, const int32_t *, int32_t *);
@@ -105,7 +103,7 @@ struct MulPlan {
     int ntiles = 0;
     int S_top = 1, S0 = 1;
     std::vector<Level> lv;   // large levels, lv[i].S = S_top >> i
-    int max_seg = 0, max_walk = 0;
+    int max_seg = 0;
     int64_t max_tab = 0;
     // device
     SaProb *d_probs = nullptr;
@@ -119,7 +117,6 @@ struct MulPlan {
     uint32_t *d_U = nullptr, *d_Ui = nullptr;
     int32_t *d_cntP = nullptr, *d_cntQ = nullptr, *d_cend = nullptr;
     int32_t *d_cntHi = nullptr, *d_cntLo = nullptr;
-    int4 *d_start = nullptr;
     int32_t *d_map_row = nullptr, *d_map_col = nullptr;   // staggered: core -> block rows / cols
     int result = 0;                                        // index of the R buffer holding the product
     int64_t elements_levels = 0;                           // sum over levels of the elements processed
@@ -129,7 +126,7 @@ static void plan_free(MulPlan *p) {
     if (!p) return;
     void *bufs[] = {p->d_probs, p->d_tile_prob, p->d_segs, p->d_walks, p->d_rows, p->d_cols, p->d_seg_first,
                     p->d_P[0], p->d_P[1], p->d_Qi[0], p->d_Qi[1], p->d_PR, p->d_QC, p->d_R[0], p->d_R[1],
-                    p->d_U, p->d_Ui, p->d_cntP, p->d_cntQ, p->d_cend, p->d_cntHi, p->d_cntLo, p->d_start, p->d_map_row, p->d_map_col};
+                    p->d_U, p->d_Ui, p->d_cntP, p->d_cntQ, p->d_cend, p->d_cntHi, p->d_cntLo, p->d_map_row, p->d_map_col};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -211,7 +208,6 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
             }
         }
         p->max_seg = std::max(p->max_seg, L.nseg);
-        p->max_walk = std::max(p->max_walk, L.nwalk);
         p->max_tab = std::max(p->max_tab, L.tab_total);
         p->lv.push_back(L);
     }
@@ -242,7 +238,6 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
     PLAN_TRY(dalloc(&p->d_cend, std::max<size_t>(segs.size(), 1)));   // one entry per segment of every level
     PLAN_TRY(dalloc(&p->d_cntHi, (size_t)p->max_tab));
     PLAN_TRY(dalloc(&p->d_cntLo, (size_t)p->max_tab));
-    PLAN_TRY(dalloc(&p->d_start, (size_t)p->max_walk));
     if (staggered) {
         PLAN_TRY(dalloc(&p->d_map_row, nt));
         PLAN_TRY(dalloc(&p->d_map_col, nt));
@@ -293,10 +288,8 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
             hipLaunchKernelGGL(table_cols, dim3((unsigned)ceil_div(L.ncol, 4)), dim3(256), 0, s, segs,
                                p->d_cols + L.col0, L.ncol, p->d_cntHi, p->d_cntLo);
         }
-        hipLaunchKernelGGL(seg_search, dim3((unsigned)ceil_div(L.nwalk, 4 * kSearchRun)), dim3(256), 0, s, p->d_U, p->d_Ui,
-                           p->d_cntHi, p->d_cntLo, segs, walks, L.nwalk, cend, p->d_start);
-        hipLaunchKernelGGL(seg_walk_wave, dim3((unsigned)(8 * ceil_div(ceil_div(L.nwalk, 4), 8))), dim3(256), 0, s,
-                           p->d_U, p->d_Ui, segs, walks, L.nwalk, cend, p->d_start, p->d_R[rc]);
+        hipLaunchKernelGGL(seg_search_walk, dim3((unsigned)(8 * ceil_div(ceil_div(L.nwalk, 4), 8))), dim3(256), 0, s,
+                           p->d_U, p->d_Ui, p->d_cntHi, p->d_cntLo, segs, walks, L.nwalk, cend, p->d_R[rc]);
     }
     p->result = rc;
     SL_HIP_TRY(hipGetLastError());

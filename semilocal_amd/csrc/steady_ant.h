@@ -8,8 +8,7 @@ namespace sa {
 
 constexpr int kTile = 1024;      // elements per block of the split / small-solver kernels
 constexpr int kBlk = 1024;       // block size of the per-subproblem count tables
-constexpr int kSegSteps = 512;   // walk steps per parallel walk segment (large levels)
-constexpr int kSearchRun = 2;    // consecutive walk segments whose starts one wave finds (large levels)
+constexpr int kSegSteps = 1024;  // walk steps per segment of the large levels (one wave each)
 constexpr int kWalkSlack = 64;   // entries allocated past the end of U and U^-1 (the searches' 16-byte groups)
 constexpr int kSplitPer = 4;     // consecutive elements per thread of the large-level split kernels
 constexpr int kSplitThreads = kTile / kSplitPer;   // their block size (one tile per block)

@@ -910,9 +910,6 @@ __device__ int wave_delta_direct(const uint32_t *__restrict__ Ux, int s, int r, 
 // rounded down to a multiple of 4, so it settles b - a <= 64 PER - 4.
 constexpr int kWinPer = 16;                   // positions per lane after a binary search
 constexpr int kWinSpan = 64 * kWinPer - 4;    // b - a bound for that pass
-constexpr int kIncPer = 12;                   // positions per lane for a run's next segment
-static_assert(64 * kIncPer - 4 >= kSegSteps, "a run's next crossing lies within kSegSteps columns");
-static_assert(kSegSteps % 64 == 0, "row sums take kSegSteps / 64 rows per lane");
 
 // Lane terms of one window.  Ui is read as aligned 16-byte groups from w0;
 // U from R(w0) rounded down to a multiple of 4 (M = R(w0) mod 4, so entry
@@ -1003,128 +1000,87 @@ __device__ __forceinline__ int row_term(uint32_t u, int c) {
     return ((int)(u & kValMask) >= c) ^ (int)(u >> 31);
 }
 
-// Start points of the walk segments: one wave per run of kSearchRun entries
-// (subproblem, k) of the list; walk segment k covers steps [k kSegSteps,
-// (k+1) kSegSteps).  Output (r, c, delta(r, c), t0), or r = -1 when the walk
-// ends before step t0.
-// A run's first entry (or one that does not continue its predecessor's walk)
-// binary-searches D(c) = delta(c + s - t0, c) until at most kWinSpan
-// candidates remain, then takes one window pass.  The next segment of the
-// same walk crosses diagonal t0 + kSegSteps within kSegSteps columns to the
-// right of this crossing c (a step moves one row or one column), and
-// D there at c is this D(c) less the row terms of the kSegSteps rows between
-// the two diagonals: one short sum instead of a search.  delta(r, c) one row
-// above the crossing is D(c) less row r's term.
-__global__ void seg_search(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
-                           const int32_t *__restrict__ cntHi, const int32_t *__restrict__ cntLo,
-                           const SaSeg *__restrict__ segs, const int2 *__restrict__ walks, int nwalks,
-                           const int32_t *__restrict__ c_end, int4 *__restrict__ seg_start) {
-    const int wv = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+// The walk's crossing of anti-diagonal t0 of a split subproblem (t0 <= the
+// walk's last step): the first c in [lo, hi] with D(c) = delta(c + off, c)
+// >= 0, off = s - t0, and D(c).  Small subproblems (nb <= kDirectNb): exact
+// halvings by direct scans.  Larger ones: delta is nondecreasing in r and in
+// c, so the count tables' corners around (R, c) bound D(c): delta(i kBlk,
+// j kBlk) <= D(c) <= delta((i+1) kBlk, (j+1) kBlk).  Each round probes 64
+// positions strictly inside [a, b), one per lane, in one memory round trip:
+// the last probe whose upper bound is negative and the first whose lower
+// bound is not narrow the range; where every probe straddles 0, one exact
+// probe halves it.  At most kWinSpan candidates are left to one window pass.
+__device__ int2 diag_crossing(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
+                              const int32_t *__restrict__ cntHi, const int32_t *__restrict__ cntLo, const SaSeg &sg,
+                              int t0) {
     const int lane = threadIdx.x & 63;
-    int prev_seg = -1, prev_k = -1, pc = 0, pD = 0;   // last crossing of this run: c and D(c)
-    for (int j = 0; j < kSearchRun; ++j) {
-        const int w = wv * kSearchRun + j;
-        if (w >= nwalks) break;   // uniform
-        const int2 job = walks[w];
-        const SaSeg sg = segs[job.x];
-        const int s = sg.s;
-        const int t_end = s - 1 + c_end[job.x];
-        const int t0 = job.y * kSegSteps;
-        int4 out = make_int4(-1, -1, 0, 0);
-        if (t0 <= t_end) {
-            const int off = s - t0;
-            const int lo = max(0, t0 - s + 1), hi = min(t0, s);
-            const uint32_t *Ux = U + sg.x0;
-            int2 cd;
-            if (job.x == prev_seg && job.y == prev_k + 1 && pc >= lo) {
-                // rows [pc + off, pc + off + kSegSteps) lie in [1, s - 1]
-                int acc = 0;
-#pragma unroll
-                for (int k = 0; k < kSegSteps / 64; ++k) acc += row_term(Ux[pc + off + lane + 64 * k], pc);
-                const int Da = pD - __builtin_amdgcn_readfirstlane(wave_sum(acc));
-                cd = wave_window_search<kIncPer>(Ux, Ui + sg.x0, off, pc, min(pc + kSegSteps, hi), Da);
-            } else if (sg.nb <= kDirectNb) {
-                // a small subproblem: halvings by direct scans, no tables
-                int a = lo, b = hi;
-                while (b - a > kWinSpan) {
-                    const int mid = (a + b) >> 1;
-                    if (wave_delta_direct(Ux, s, mid + off, mid) >= 0) b = mid;
-                    else a = mid + 1;
-                }
-                cd = wave_window_search<kWinPer>(Ux, Ui + sg.x0, off, a, b, wave_delta_direct(Ux, s, a + off, a));
-            } else {
-                // delta is nondecreasing in r and in c, so the count tables'
-                // corners around (R, c) bound D(c): delta(i kBlk, j kBlk) <=
-                // D(c) <= delta((i+1) kBlk, (j+1) kBlk).  Most halvings are
-                // decided by the two table entries; only where the bounds
-                // straddle 0 does a probe read the strips.
-                // Each round probes 64 positions strictly inside [a, b),
-                // one per lane, in one memory round trip: the last probe
-                // whose upper bound is negative and the first whose lower
-                // bound is not narrow the range (D is nondecreasing along the
-                // diagonal); where every probe straddles 0, one exact probe
-                // halves it.
-                const int W = sg.nb + 1;
-                const int32_t *th = cntHi + sg.tab_off2, *tl = cntLo + sg.tab_off2;
-                int a = lo, b = hi;
-                while (b - a > kWinSpan) {
-                    const int p = a + (int)(((int64_t)(b - a) * (lane + 1)) / 65);   // a < p < b
-                    const int i = (p + off) / kBlk, j = p / kBlk;
-                    const int i1 = min(i + 1, sg.nb), j1 = min(j + 1, sg.nb);
-                    const int64_t t0i = (int64_t)j * W + i, t1i = (int64_t)j1 * W + i1;
-                    const int lower = th[t0i] - tl[t0i], upper = th[t1i] - tl[t1i];
-                    const uint64_t neg = __ballot(upper < 0), nonneg = __ballot(lower >= 0);
-                    const int na = neg ? __builtin_amdgcn_readlane(p, 63 - __clzll((long long)neg)) + 1 : a;
-                    const int nb = nonneg ? __builtin_amdgcn_readlane(p, ctz64(nonneg)) : b;
-                    if (na == a && nb == b) {
-                        const int mid = (a + b) >> 1;
-                        if (wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0) b = mid;
-                        else a = mid + 1;
-                    } else {
-                        a = na;
-                        b = nb;
-                    }
-                }
-                const int Da = wave_delta(U, Ui, cntHi, cntLo, sg, a + off, a);
-                cd = wave_window_search<kWinPer>(Ux, Ui + sg.x0, off, a, b, Da);
-            }
-            const int c = cd.x, r = c + off - 1;
-            out = make_int4(r, c, cd.y - row_term(Ux[r], c), t0);
-            prev_seg = job.x;
-            prev_k = job.y;
-            pc = c;
-            pD = cd.y;
-        } else {
-            prev_seg = -1;
+    const int s = sg.s, off = s - t0;
+    const int lo = max(0, t0 - s + 1), hi = min(t0, s);
+    const uint32_t *Ux = U + sg.x0;
+    if (sg.nb <= kDirectNb) {
+        int a = lo, b = hi;
+        while (b - a > kWinSpan) {
+            const int mid = (a + b) >> 1;
+            if (wave_delta_direct(Ux, s, mid + off, mid) >= 0) b = mid;
+            else a = mid + 1;
         }
-        if (lane == 0) seg_start[w] = out;
+        return wave_window_search<kWinPer>(Ux, Ui + sg.x0, off, a, b, wave_delta_direct(Ux, s, a + off, a));
     }
+    const int W = sg.nb + 1;
+    const int32_t *th = cntHi + sg.tab_off2, *tl = cntLo + sg.tab_off2;
+    int a = lo, b = hi;
+    while (b - a > kWinSpan) {
+        const int p = a + (int)(((int64_t)(b - a) * (lane + 1)) / 65);   // a < p < b
+        const int i = (p + off) / kBlk, j = p / kBlk;
+        const int i1 = min(i + 1, sg.nb), j1 = min(j + 1, sg.nb);
+        const int64_t t0i = (int64_t)j * W + i, t1i = (int64_t)j1 * W + i1;
+        const int lower = th[t0i] - tl[t0i], upper = th[t1i] - tl[t1i];
+        const uint64_t neg = __ballot(upper < 0), nonneg = __ballot(lower >= 0);
+        const int na = neg ? __builtin_amdgcn_readlane(p, 63 - __clzll((long long)neg)) + 1 : a;
+        const int nb = nonneg ? __builtin_amdgcn_readlane(p, ctz64(nonneg)) : b;
+        if (na == a && nb == b) {
+            const int mid = (a + b) >> 1;
+            if (wave_delta(U, Ui, cntHi, cntLo, sg, mid + off, mid) >= 0) b = mid;
+            else a = mid + 1;
+        } else {
+            a = na;
+            b = nb;
+        }
+    }
+    return wave_window_search<kWinPer>(Ux, Ui + sg.x0, off, a, b, wave_delta(U, Ui, cntHi, cntLo, sg, a + off, a));
 }
 
-// Walks, large levels: one wave per segment (wave_walk over the level's U and
-// U^-1).  Blocks of 4 waves; the hardware deals blocks to the 8 XCDs
-// round-robin, so block b is mapped to segments in contiguous ranges per XCD
-// and each XCD's L2 holds the rows and columns of its part of the level.
-__global__ __launch_bounds__(256) void seg_walk_wave(const uint32_t *__restrict__ U, const uint32_t *__restrict__ Ui,
-                                                     const SaSeg *__restrict__ segs, const int2 *__restrict__ walks,
-                                                     int nwalks, const int32_t *__restrict__ c_end,
-                                                     const int4 *__restrict__ seg_start, int32_t *__restrict__ R) {
+// Searches and walks of a large level, one wave per segment of kSegSteps
+// evaluations: the wave finds where the walk crosses the segment's first
+// anti-diagonal (diag_crossing; delta(r, c) one row above the crossing is
+// D(c) less row r's term) and walks the segment from there (wave_walk).
+// Blocks of 4 waves; the hardware deals blocks to the 8 XCDs round-robin,
+// so block b is mapped to segments in contiguous ranges per XCD and each
+// XCD's L2 holds the rows and columns of its part of the level.
+__global__ __launch_bounds__(256) void seg_search_walk(const uint32_t *__restrict__ U,
+                                                       const uint32_t *__restrict__ Ui,
+                                                       const int32_t *__restrict__ cntHi,
+                                                       const int32_t *__restrict__ cntLo,
+                                                       const SaSeg *__restrict__ segs,
+                                                       const int2 *__restrict__ walks, int nwalks,
+                                                       const int32_t *__restrict__ c_end, int32_t *__restrict__ R) {
     const int per_xcd = (int)gridDim.x / 8;   // the grid is a multiple of 8 blocks
     const int blk = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     const int w = __builtin_amdgcn_readfirstlane(blk * 4 + (int)(threadIdx.x >> 6));
     if (w >= nwalks) return;
-    const int4 st = seg_start[w];
-    const int r0 = __builtin_amdgcn_readfirstlane(st.x);
-    if (r0 < 0) return;
     const int2 job = walks[w];
     const SaSeg sg = segs[job.x];
     const int s = __builtin_amdgcn_readfirstlane(sg.s);
-    const int t_end = s - 1 + c_end[job.x];
+    const int t_end = __builtin_amdgcn_readfirstlane(s - 1 + c_end[job.x]);
+    const int t0 = job.y * kSegSteps;
+    if (t0 > t_end) return;
     const uint32_t *Us = U + sg.x0, *Uis = Ui + sg.x0;
+    const int2 cd = diag_crossing(U, Ui, cntHi, cntLo, sg, t0);
+    const int c = cd.x, r = c + (s - t0) - 1;
     int32_t *Rs = R + sg.x0;
-    (void)wave_walk(r0, __builtin_amdgcn_readfirstlane(st.y), __builtin_amdgcn_readfirstlane(st.z), s,
-              __builtin_amdgcn_readfirstlane(min(kSegSteps, t_end - st.w + 1)), [&](int x) { return Us[x]; },
-              [&](int x) { return Uis[x]; }, [&](int rr, int cc) { Rs[rr] = cc; });
+    (void)wave_walk(__builtin_amdgcn_readfirstlane(r), __builtin_amdgcn_readfirstlane(c),
+                    __builtin_amdgcn_readfirstlane(cd.y - row_term(Us[r], c)), s, min(kSegSteps, t_end - t0 + 1),
+                    [&](int x) { return Us[x]; }, [&](int x) { return Uis[x]; }, [&](int rr, int cc) { Rs[rr] = cc; });
 }
 
 __global__ void fill_i32(int32_t *__restrict__ p, int32_t v, int64_t count) {
