@@ -687,7 +687,10 @@ __global__ __launch_bounds__(kTile, 8) void solve_small(const int32_t *__restric
 // bit, Ui[parent col] = parent row | hi bit, R = U without the bit, and
 // c_end[seg] = 1 + largest column of a lo point (one atomic per block: a
 // block's 1,024 elements, a tile, lie in one subproblem).  kSplitPer
-// consecutive elements per thread, as in the split kernels.
+// elements per thread, kSplitThreads apart: a store instruction then covers
+// a run of consecutive child rows, whose parent rows increase (each child's
+// rows come from one stable partition), so the U and R stores of a wave
+// fill whole cache lines.
 __global__ __launch_bounds__(kSplitThreads) void merge_map(const int32_t *__restrict__ Rc,
                                                            const int32_t *__restrict__ PRc,
                                                            const int32_t *__restrict__ QCc,
@@ -701,16 +704,19 @@ __global__ __launch_bounds__(kSplitThreads) void merge_map(const int32_t *__rest
     const int t = blockIdx.x;
     const int pid = tile_prob[t];
     const SaProb pb = probs[pid];
-    const int64_t x0 = (int64_t)t * kTile + kSplitPer * threadIdx.x;
+    const int64_t x0 = (int64_t)t * kTile + threadIdx.x;
     const int lx0 = (int)(x0 - pb.base);   // local index (problems are below 2^31 elements)
     const int lg = __builtin_ctz((unsigned)S);   // S is a power of two: shifts, not divisions
-    const int4 r4 = *reinterpret_cast<const int4 *>(Rc + x0);
-    const int4 pr4 = *reinterpret_cast<const int4 *>(PRc + x0);
-    const int rv[kSplitPer] = {r4.x, r4.y, r4.z, r4.w}, prv[kSplitPer] = {pr4.x, pr4.y, pr4.z, pr4.w};
+    int rv[kSplitPer], prv[kSplitPer];
+#pragma unroll
+    for (int e = 0; e < kSplitPer; ++e) {   // whole tiles: in bounds past a problem's end too
+        rv[e] = Rc[x0 + e * kSplitThreads];
+        prv[e] = PRc[x0 + e * kSplitThreads];
+    }
     int cand = 0;
 #pragma unroll
     for (int e = 0; e < kSplitPer; ++e) {
-        const int lx = lx0 + e;
+        const int lx = lx0 + e * kSplitThreads;
         if (lx < pb.n) {
             const int cid = lx >> (lg - 1);   // child: lx / h
             const uint32_t bit = (cid & 1) ? kHiBit : 0u;
