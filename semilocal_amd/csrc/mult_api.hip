@@ -47,6 +47,9 @@ __global__ void table_rows(const uint32_t *, const SaSeg *, const int2 *, int32_
 __global__ void table_cols(const SaSeg *, const int2 *, int, int32_t *, int32_t *);
 __global__ void seg_search_walk(const uint32_t *, const uint32_t *, const int32_t *, const int32_t *, const SaSeg *,
                                 const int2 *, int, const int32_t *, int32_t *);
+__global__ void split_count2(const int32_t *, const int32_t *, const int32_t *, const SaProb *, int, int32_t *);
+__global__ void split_scatter2(const int32_t *, const int32_t *, const int32_t *, const SaProb *, int, const int32_t *,
+                               int32_t *, int32_t *, int32_t *, int32_t *, int32_t *, int32_t *);
 __global__ void fill_i32(int32_t *, int32_t, int64_t);
 __global__ void stag_qinv(const int32_t *, const StagJob *, const int32_t *, int32_t *);
 __global__ void stag_count(const int32_t *, const int32_t *, const StagJob *, const int32_t *, int32_t *);
@@ -115,7 +118,7 @@ struct MulPlan {
     int32_t *d_PR = nullptr, *d_QC = nullptr;
     int32_t *d_R[2] = {nullptr, nullptr};
     uint32_t *d_U = nullptr, *d_Ui = nullptr;
-    int32_t *d_cntP = nullptr, *d_cntQ = nullptr, *d_cend = nullptr;
+    int32_t *d_cntP = nullptr, *d_cntQ = nullptr, *d_cnt8 = nullptr, *d_cend = nullptr;
     int32_t *d_cntHi = nullptr, *d_cntLo = nullptr;
     int32_t *d_map_row = nullptr, *d_map_col = nullptr;   // staggered: core -> block rows / cols
     int result = 0;                                        // index of the R buffer holding the product
@@ -126,7 +129,7 @@ static void plan_free(MulPlan *p) {
     if (!p) return;
     void *bufs[] = {p->d_probs, p->d_tile_prob, p->d_segs, p->d_walks, p->d_rows, p->d_cols, p->d_seg_first,
                     p->d_P[0], p->d_P[1], p->d_Qi[0], p->d_Qi[1], p->d_PR, p->d_QC, p->d_R[0], p->d_R[1],
-                    p->d_U, p->d_Ui, p->d_cntP, p->d_cntQ, p->d_cend, p->d_cntHi, p->d_cntLo, p->d_map_row, p->d_map_col};
+                    p->d_U, p->d_Ui, p->d_cntP, p->d_cntQ, p->d_cnt8, p->d_cend, p->d_cntHi, p->d_cntLo, p->d_map_row, p->d_map_col};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -235,6 +238,7 @@ static int plan_create(MulPlan **out, const std::vector<int32_t> &sizes, bool st
     PLAN_TRY(dalloc(&p->d_Ui, nt + kWalkSlack));
     PLAN_TRY(dalloc(&p->d_cntP, (size_t)p->ntiles));
     PLAN_TRY(dalloc(&p->d_cntQ, (size_t)p->ntiles));
+    PLAN_TRY(dalloc(&p->d_cnt8, 8 * (size_t)p->ntiles));
     PLAN_TRY(dalloc(&p->d_cend, std::max<size_t>(segs.size(), 1)));   // one entry per segment of every level
     PLAN_TRY(dalloc(&p->d_cntHi, (size_t)p->max_tab));
     PLAN_TRY(dalloc(&p->d_cntLo, (size_t)p->max_tab));
@@ -256,7 +260,18 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
     const int nt = p->ntiles;
     const int64_t ntot = p->ntot;
     int cur = 0;
-    for (size_t li = 0; li < p->lv.size(); ++li) {
+    size_t li = 0;
+    for (; li + 1 < p->lv.size(); li += 2) {   // two split levels per pass (S and S/2)
+        const Level &L = p->lv[li];
+        hipLaunchKernelGGL(split_count2, dim3(nt), dim3(kSplitThreads), 0, s, p->d_P[cur], p->d_Qi[cur],
+                           p->d_tile_prob, p->d_probs, L.S, p->d_cnt8);
+        hipLaunchKernelGGL(split_scatter2, dim3(nt), dim3(kSplitThreads), 0, s, p->d_P[cur], p->d_Qi[cur],
+                           p->d_tile_prob, p->d_probs, L.S, p->d_cnt8, p->d_P[cur ^ 1], p->d_Qi[cur ^ 1],
+                           p->d_PR + li * ntot, p->d_QC + li * ntot, p->d_PR + (li + 1) * ntot,
+                           p->d_QC + (li + 1) * ntot);
+        cur ^= 1;
+    }
+    for (; li < p->lv.size(); ++li) {
         const Level &L = p->lv[li];
         hipLaunchKernelGGL(split_count, dim3(nt), dim3(kSplitThreads), 0, s, p->d_P[cur], p->d_Qi[cur],
                            p->d_tile_prob, p->d_probs, L.S, p->d_cntP, p->d_cntQ);

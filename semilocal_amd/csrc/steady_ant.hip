@@ -226,6 +226,156 @@ __global__ __launch_bounds__(kSplitThreads) void split_scatter(const int32_t *__
     }
 }
 
+// Two split levels in one pass (S and S/2, S >= 4 kTile).  Both partitions
+// are stable, so the elements of an S-segment end, at level S/4, ordered by
+// the top two bits of their value within the segment (key k = v / (S/4))
+// and then by their order at level S; at level S/2 by the top bit (k >> 1).
+// An element's position at level S/2 is seg0 + (k >> 1) h + its rank among
+// the segment's elements with the same top bit, at S/4 seg0 + k (S/4) + its
+// rank among those with the same key: earlier tiles' counts of the key(s)
+// plus the in-tile prefix.  PR at S/2 is the element's row in its
+// S-segment, PR at S/4 its row in its S/2-segment (its rank by the top bit);
+// P at S/4 is its value less k (S/4).  The same for Q's columns.  Counts:
+// per tile, 8 words (P's keys 0-3, Q's keys 0-3).
+__global__ __launch_bounds__(kSplitThreads) void split_count2(const int32_t *__restrict__ P,
+                                                              const int32_t *__restrict__ Qi,
+                                                              const int32_t *__restrict__ tile_prob,
+                                                              const SaProb *__restrict__ probs, int S,
+                                                              int32_t *__restrict__ cnt8) {
+    __shared__ int red[4][kSplitThreads / 64];
+    const int t = blockIdx.x;
+    const SaProb pb = probs[tile_prob[t]];
+    const int64_t x0 = (int64_t)t * kTile + kSplitPer * threadIdx.x;
+    const int lx0 = (int)(x0 - pb.base);
+    const int sh = __builtin_ctz((unsigned)S) - 2;   // key = value >> sh
+    const int4 p4 = *reinterpret_cast<const int4 *>(P + x0);
+    const int4 q4 = *reinterpret_cast<const int4 *>(Qi + x0);
+    const int pv[kSplitPer] = {p4.x, p4.y, p4.z, p4.w}, qv[kSplitPer] = {q4.x, q4.y, q4.z, q4.w};
+    int w[4] = {0, 0, 0, 0};   // key pairs packed 16 bits each (a tile holds at most kTile < 2^16 per key)
+#pragma unroll
+    for (int e = 0; e < kSplitPer; ++e)
+        if (lx0 + e < pb.n) {
+            const int kp = pv[e] >> sh, kq = qv[e] >> sh;
+            w[kp >> 1] += 1 << (16 * (kp & 1));
+            w[2 + (kq >> 1)] += 1 << (16 * (kq & 1));
+        }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int v = wave_sum(w[i]);
+        if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        int v = 0;
+#pragma unroll
+        for (int j = 0; j < kSplitThreads / 64; ++j) v += red[threadIdx.x][j];
+        cnt8[(int64_t)t * 8 + 2 * threadIdx.x] = v & 0xFFFF;
+        cnt8[(int64_t)t * 8 + 2 * threadIdx.x + 1] = v >> 16;
+    }
+}
+
+__global__ __launch_bounds__(kSplitThreads) void split_scatter2(const int32_t *__restrict__ P,
+                                                                const int32_t *__restrict__ Qi,
+                                                                const int32_t *__restrict__ tile_prob,
+                                                                const SaProb *__restrict__ probs, int S,
+                                                                const int32_t *__restrict__ cnt8,
+                                                                int32_t *__restrict__ P2, int32_t *__restrict__ Qi2,
+                                                                int32_t *__restrict__ PR1, int32_t *__restrict__ QC1,
+                                                                int32_t *__restrict__ PR2,
+                                                                int32_t *__restrict__ QC2) {
+    __shared__ int tmp[kSplitThreads / 64 + 1];
+    __shared__ int red[8][kSplitThreads / 64];
+    __shared__ int offs[8];
+    const int t = blockIdx.x;
+    const SaProb pb = probs[tile_prob[t]];
+    const int64_t x0 = (int64_t)t * kTile + kSplitPer * threadIdx.x;
+    const int lx0 = (int)(x0 - pb.base);
+    const int lg = __builtin_ctz((unsigned)S), sh = lg - 2;
+    const int h = S >> 1, q = S >> 2;
+    {   // counts of the segment's earlier tiles, per key (the segment starts on a tile)
+        const int lt0 = (int)((int64_t)t * kTile - pb.base);
+        const int64_t tile0 = (pb.base + ((lt0 >> lg) << lg)) / kTile;
+        int v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t u = tile0 + threadIdx.x; u < t; u += kSplitThreads) {
+            const int4 a = *reinterpret_cast<const int4 *>(cnt8 + u * 8);
+            const int4 b = *reinterpret_cast<const int4 *>(cnt8 + u * 8 + 4);
+            v[0] += a.x; v[1] += a.y; v[2] += a.z; v[3] += a.w;
+            v[4] += b.x; v[5] += b.y; v[6] += b.z; v[7] += b.w;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int x = wave_sum(v[i]);
+            if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            int x = 0;
+#pragma unroll
+            for (int j = 0; j < kSplitThreads / 64; ++j) x += red[threadIdx.x][j];
+            offs[threadIdx.x] = x;
+        }
+        __syncthreads();
+    }
+    const int4 p4 = *reinterpret_cast<const int4 *>(P + x0);
+    const int4 q4 = *reinterpret_cast<const int4 *>(Qi + x0);
+    const int pv[kSplitPer] = {p4.x, p4.y, p4.z, p4.w}, qv[kSplitPer] = {q4.x, q4.y, q4.z, q4.w};
+    int kp[kSplitPer], kq[kSplitPer];
+    int w[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < kSplitPer; ++e) {
+        const bool in = lx0 + e < pb.n;
+        kp[e] = pv[e] >> sh;
+        kq[e] = qv[e] >> sh;
+        if (in) {
+            w[kp[e] >> 1] += 1 << (16 * (kp[e] & 1));
+            w[2 + (kq[e] >> 1)] += 1 << (16 * (kq[e] & 1));
+        }
+    }
+    int ex[4];   // this thread's in-tile exclusive prefixes, packed as w
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int tot;
+        ex[i] = block_excl_scan(w[i], tmp, tot);
+    }
+    const int64_t seg0 = pb.base + ((lx0 >> lg) << lg);   // a tile lies in one segment
+    const int r0 = lx0 & (S - 1);
+#pragma unroll
+    for (int e = 0; e < kSplitPer; ++e) {
+        if (lx0 + e < pb.n) {
+            int cp[4], cq[4];   // ranks so far by key
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                cp[k] = offs[k] + ((ex[k >> 1] >> (16 * (k & 1))) & 0xFFFF);
+                cq[k] = offs[4 + k] + ((ex[2 + (k >> 1)] >> (16 * (k & 1))) & 0xFFFF);
+            }
+            const int r = r0 + e;
+            // cp[k]: the segment's elements before this one with key k, so
+            // its rank by the top bit is the sum over the two keys of its pair
+            const int k = kp[e], k1 = k >> 1;
+            const int rank1p = cp[2 * k1] + cp[2 * k1 + 1];
+            const int rank2p = cp[k];
+            const int64_t pos1p = seg0 + (int64_t)k1 * h + rank1p;
+            const int64_t pos2p = seg0 + (int64_t)k * q + rank2p;
+            PR1[pos1p] = r;
+            PR2[pos2p] = rank1p;
+            P2[pos2p] = pv[e] - k * q;
+            const int kk = kq[e], kk1 = kk >> 1;
+            const int rank1q = cq[2 * kk1] + cq[2 * kk1 + 1];
+            const int rank2q = cq[kk];
+            const int64_t pos1q = seg0 + (int64_t)kk1 * h + rank1q;
+            const int64_t pos2q = seg0 + (int64_t)kk * q + rank2q;
+            QC1[pos1q] = r;
+            QC2[pos2q] = rank1q;
+            Qi2[pos2q] = qv[e] - kk * q;
+        }
+        // this element counted for the next ones
+        if (lx0 + e < pb.n) {
+            ex[kp[e] >> 1] += 1 << (16 * (kp[e] & 1));
+            ex[2 + (kq[e] >> 1)] += 1 << (16 * (kq[e] & 1));
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The ant passage walk, shared by the small solver and the large levels.
 // ---------------------------------------------------------------------------
