@@ -44,6 +44,7 @@ __global__ void solve_small(const int32_t *, const int32_t *, const int32_t *, c
 __global__ void merge_map(const int32_t *, const int32_t *, const int32_t *, const int32_t *, const SaProb *,
                           const int32_t *, int, int64_t, uint32_t *, uint32_t *, int32_t *, int32_t *);
 __global__ void table_rows(const uint32_t *, const SaSeg *, const int2 *, int32_t *, int32_t *);
+__global__ void table_small(const uint32_t *, const SaSeg *, int32_t *, int32_t *);
 __global__ void table_cols(const SaSeg *, const int2 *, int, int32_t *, int32_t *);
 __global__ void seg_search_walk(const uint32_t *, const uint32_t *, const int32_t *, const int32_t *, const SaSeg *,
                                 const int2 *, int, const int32_t *, int32_t *);
@@ -297,7 +298,10 @@ static int plan_launch(MulPlan *p, hipStream_t s) {
                            p->d_U, p->d_Ui, p->d_R[rc ^ 1], cend);
         rc ^= 1;
         if (L.nwalk == 0) continue;
-        if (L.S > kDirectNb * kBlk) {   // smaller subproblems are searched by direct scans
+        if (L.S > kDirectNb * kBlk && L.S <= kSmallTabNb * kBlk) {   // small tables: one launch
+            hipLaunchKernelGGL(table_small, dim3((unsigned)L.nseg), dim3(256), 0, s, p->d_U, segs, p->d_cntHi,
+                               p->d_cntLo);
+        } else if (L.S > kDirectNb * kBlk) {   // smaller subproblems are searched by direct scans
             hipLaunchKernelGGL(table_rows, dim3((unsigned)L.nrow), dim3(256), 0, s, p->d_U, segs, p->d_rows + L.row0,
                                p->d_cntHi, p->d_cntLo);
             hipLaunchKernelGGL(table_cols, dim3((unsigned)ceil_div(L.ncol, 4)), dim3(256), 0, s, segs,

@@ -13,6 +13,7 @@ constexpr int kWalkSlack = 64;   // entries allocated past the end of U and U^-1
 constexpr int kSplitPer = 4;     // consecutive elements per thread of the large-level split kernels
 constexpr int kSplitThreads = kTile / kSplitPer;   // their block size (one tile per block)
 constexpr int kMaxNb = 4096;     // count-table blocks per side: subproblems up to kMaxNb * kBlk
+constexpr int kSmallTabNb = 8;   // count tables of at most this many blocks per side: built by one block
 constexpr int kDirectNb = 2;     // subproblems of at most kDirectNb * kBlk elements: no count tables, the
                                  // segment searches scan the subproblem
 

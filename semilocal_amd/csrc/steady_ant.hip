@@ -933,6 +933,66 @@ __global__ __launch_bounds__(256) void table_rows(const uint32_t *__restrict__ U
     }
 }
 
+// Both passes of the count tables in one block per subproblem, for
+// subproblems of at most kSmallTabNb blocks per side: the hi and lo counts of
+// every (block-row, block-column) cell histogrammed in LDS, then cntHi[j][i]
+// = #hi with row-block < i and col-block < j (a 2D prefix) and cntLo[j][i] =
+// #lo with row-block >= i and col-block >= j (a 2D suffix), written whole.
+// One launch instead of table_rows + table_cols.
+__global__ __launch_bounds__(256) void table_small(const uint32_t *__restrict__ U, const SaSeg *__restrict__ segs,
+                                                   int32_t *__restrict__ cntHi, int32_t *__restrict__ cntLo) {
+    constexpr int T = kSmallTabNb + 1;
+    __shared__ int hh[T * T], hl[T * T];   // [i][j]
+    const SaSeg sg = segs[blockIdx.x];
+    const int nb = sg.nb, W = nb + 1;
+    if (nb == 0) return;   // not split at this level: no walk, no table
+    for (int k = threadIdx.x; k < T * T; k += 256) hh[k] = hl[k] = 0;
+    __syncthreads();
+    for (int rr = threadIdx.x; rr < sg.s; rr += 256) {
+        const uint32_t u = U[sg.x0 + rr];
+        const int i = rr / kBlk, j = (int)(u & kValMask) / kBlk;
+        atomicAdd((u & kHiBit) ? &hh[i * T + j] : &hl[i * T + j], 1);
+    }
+    __syncthreads();
+    // hi: prefix along j (exclusive, shifted one column right), then along i;
+    // lo: suffix along j, then along i.  Row/column nb of the hi counts and
+    // the lo sums past nb are empty.
+    if (threadIdx.x < W) {   // one thread per block-row i
+        const int i = threadIdx.x;
+        int ch = 0, cl = 0;
+        for (int j = 0; j < W; ++j) {
+            const int h = i < nb && j < nb ? hh[i * T + j] : 0;
+            hh[i * T + j] = ch;
+            ch += h;
+        }
+        for (int j = nb; j >= 0; --j) {
+            cl += i < nb && j < nb ? hl[i * T + j] : 0;
+            hl[i * T + j] = cl;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < W) {   // one thread per block-column j
+        const int j = threadIdx.x;
+        int ch = 0, cl = 0;
+        for (int i = 0; i < W; ++i) {
+            const int h = hh[i * T + j];
+            hh[i * T + j] = ch;
+            ch += h;
+        }
+        for (int i = nb; i >= 0; --i) {
+            cl += hl[i * T + j];
+            hl[i * T + j] = cl;
+        }
+    }
+    __syncthreads();
+    int32_t *ch = cntHi + sg.tab_off2, *cl = cntLo + sg.tab_off2;
+    for (int k = threadIdx.x; k < W * W; k += 256) {
+        const int j = k / W, i = k - j * W;
+        ch[k] = hh[i * T + j];
+        cl[k] = hl[i * T + j];
+    }
+}
+
 // Column pass, one wave per (segment, block-column j): along block-rows i of
 // T[j], hi gets an exclusive prefix and lo an inclusive suffix, in place.
 // Entry i = nb (not written by the row pass) is defined here.  Afterwards
