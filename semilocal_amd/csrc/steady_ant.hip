@@ -15,8 +15,9 @@
 //
 // Top-down (split): P's rows are stably partitioned by col < h (vertical
 // slices) and Q's columns by row < h (horizontal slices): segmented prefix
-// sums.  Each level keeps, per element, the parent row (PR) and parent column
-// (QC) it came from.
+// sums, two levels per pass (by the top two bits of the value within the
+// segment).  Each level keeps, per element, the parent row (PR) and parent
+// column (QC) it came from.
 //
 // Bottom-up (merge): the children's results are mapped back to the parent
 // (U = union of r_lo and r_hi with a label per row), then the ant passage is
@@ -25,15 +26,15 @@
 // the reference's walk is a monotone staircase: at (r, c) it moves right if
 // delta < 0 and up otherwise, and reports a "good point" at (r, c-1) when
 // delta(r, c-1) < 0 < delta(r+1, c).  The walk is cut into segments of
-// kSegSteps evaluations.  Where it crosses anti-diagonal t (t = steps taken)
-// is the first c on that diagonal with delta(r(c)+1, c) >= 0, r(c) = s-1-t+c,
-// found by binary search (the predicate is monotone along a diagonal) from a
-// per-subproblem table of block counts plus two short scans, down to a range
-// of about a thousand candidates, which one pass of prefix sums over the
-// diagonal's own rows and columns settles.  Each segment is then walked by
-// one wave, 64 x 64 positions per round (wave_walk).  The product is U with the good points
-// written over it, which is how the reference merges r_hi into r_lo and adds
-// the good points.
+// kSegSteps evaluations, one wave each.  Where it crosses anti-diagonal t
+// (t = steps taken) is the first c on that diagonal with
+// delta(r(c)+1, c) >= 0, r(c) = s-1-t+c (the predicate is monotone along a
+// diagonal): 64 probes per round bounded by a per-subproblem table of block
+// counts narrow the range to about a thousand candidates, which one pass of
+// prefix sums over the diagonal's own rows and columns settles.  The wave
+// then walks its segment, 64 x 64 positions per round (wave_walk).  The
+// product is U with the good points written over it, which is how the
+// reference merges r_hi into r_lo and adds the good points.
 //
 // Subproblems of at most kTile elements are solved entirely inside one block
 // (split, base case and merges) in LDS.
