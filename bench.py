@@ -554,9 +554,10 @@ def cpu_baseline_hyyro(pairs, target_s: float = 5.0):
             "sample": f"{done} of the {len(pairs)} pairs (seeded random order), {cells:.3e} cells, {dt:.1f} s"}
 
 
-def bench_hybrid(S, W, golden):
+def bench_hybrid(S, W, golden, grid_autotune: bool = True):
     """Config 3: 8x8 blocks combed in one launch, merged by 6 levels of batched
-    steady-ant multiplications."""
+    steady-ant multiplications; then (grid_autotune) the block split per
+    length bucket."""
     a, b = W.similar_pair()
     h = S.Hybrid(a, b, 8, 8)
     t0 = time.perf_counter()
@@ -570,6 +571,15 @@ def bench_hybrid(S, W, golden):
     ok = ok and W.sha256(h.run()) == golden["similar_pair_1e6"]["perm_sha256"]
     gtot, _, _ = h.time(warmup=1, iters=3)
     h.close()
+    out = {"workload": "similar pair: a 1e6 symbols (alphabet 4), b = a with 5% subs and 1% indels (seed 3); "
+                       "8x8 blocks", "ms": round(tot, 3), "comb_ms": round(comb, 3), "merge_ms": round(merge, 3),
+           "graph_ms": round(gtot, 3), "merge_levels": levels, "gcups": round(cells / (tot * 1e-3) / 1e9, 2),
+           "comb_roofline": valu_roofline(cells * VALU_OPS_PER_CELL, comb, "5 per cell"),
+           "graph_gcups": round(cells / (gtot * 1e-3) / 1e9, 2),
+           "comb_tiles": {"rows_per_lane": tiles[0], "waves_per_cu": tiles[1], "autotune_s": round(tune_s, 1)}}
+    if not grid_autotune:
+        out["parity"] = "bit-exact vs the reference hybrid's fixture (eager and graph)" if ok else "MISMATCH"
+        return out
     # block split per length bucket: the grid autotuner times every candidate
     # grid on a pair of the bucket's length and keeps the fastest for it
     grids = [(4, 4), (8, 8), (16, 16), (8, 16), (16, 8), (32, 32)]
@@ -585,14 +595,11 @@ def bench_hybrid(S, W, golden):
     hb = S.Hybrid(a, b, *best)
     ok = ok and W.sha256(hb.run()) == golden["similar_pair_1e6"]["perm_sha256"]
     hb.close()
-    return {"workload": "similar pair: a 1e6 symbols (alphabet 4), b = a with 5% subs and 1% indels (seed 3); "
-                        "8x8 blocks", "ms": round(tot, 3),
-            "block_split_by_bucket": by_bucket, "block_split_autotune_s": round(split_s, 1), "comb_ms": round(comb, 3), "merge_ms": round(merge, 3),
-            "graph_ms": round(gtot, 3), "merge_levels": levels, "gcups": round(cells / (tot * 1e-3) / 1e9, 2),
-            "comb_roofline": valu_roofline(cells * VALU_OPS_PER_CELL, comb, "5 per cell"),
-            "graph_gcups": round(cells / (gtot * 1e-3) / 1e9, 2),
-            "comb_tiles": {"rows_per_lane": tiles[0], "waves_per_cu": tiles[1], "autotune_s": round(tune_s, 1)},
-            "parity": "bit-exact vs the reference hybrid's fixture (eager and graph)" if ok else "MISMATCH"}
+    out["block_split_by_bucket"] = by_bucket
+    out["block_split_autotune_s"] = round(split_s, 1)
+    out["parity"] = ("bit-exact vs the reference hybrid's fixture (eager, graph and the autotuned grid)" if ok
+                     else "MISMATCH")
+    return out
 
 
 def bench_multiply(S):
@@ -705,6 +712,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--serve-pool-gb", type=float, default=16.0)
     ap.add_argument("--serve-slo-ms", type=float, default=20.0, help="p99 latency SLO of the serving rate sweep")
+    ap.add_argument("--no-grid-autotune", action="store_true",
+                    help="--only hybrid: skip the block-split autotuner (the profiled runs stay config 3's own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -721,7 +730,7 @@ def main():
         golden = json.load(open(os.path.join(ROOT, "tests", "golden", "configs.json")))
         fn = {"plain": lambda: bench_plain(semilocal_amd, W, golden),
               "binary": lambda: bench_binary(semilocal_amd, W, golden),
-              "hybrid": lambda: bench_hybrid(semilocal_amd, W, golden),
+              "hybrid": lambda: bench_hybrid(semilocal_amd, W, golden, not args.no_grid_autotune),
               "multiply": lambda: bench_multiply(semilocal_amd),
               "index": lambda: bench_index(semilocal_amd, W),
               "amatch": lambda: bench_amatch(semilocal_amd),
