@@ -10,22 +10,19 @@
 // BitParalellLCS.h:337-433, prefix_lcs/bitwise/transposition_network_binary_
 // alphabet.h:743-916: hyyro_magic and its anti-diagonal form).
 //
-// The n-bit addition is the only coupling between words; two kernels handle
-// it.
-//  * hy_rows (pairs up to 64 words: one wave): lane l holds word l and a row
-//    is two passes -- with carry-in 0 each lane learns whether it generates
-//    a carry (g) and whether it is all ones after the add (q); two ballots
-//    G, X = G | Q and one 64-bit add give every lane its carry-in (wave64
-//    carry-lookahead: carry into lane l = bit l of (X + G) ^ X ^ G), and the
-//    lane redoes its word with it.
-//  * hy_sys (wider pairs, up to 2^20 columns): systolic -- lane L holds R
-//    words and processes row t - L at step t, taking its carry from lane
-//    L - 1 of the previous step by one DPP move; carries cross waves through
-//    LDS once per 64 steps (the waves run one 64-step group apart).
-// Masks come from a table built on the device (symbols of the packed string:
-// at most 256 per pair) and are prefetched D rows ahead; rows whose symbol
-// does not occur in the packed string are dropped on the host (their p is 0
-// and they leave V unchanged).
+// The n-bit addition is the only coupling between words.  One systolic
+// kernel (hy_seg) handles every width: lane L holds R words (R = 1 or 2) and
+// processes row t - L at step t, so its carry-in is lane L - 1's carry-out
+// of the previous step.  The carries of a wave live in one 64-bit lane mask
+// in scalar registers: a lane's addition is a chain of v_addc_co_u32 taking
+// its carry from that mask, and the next step's mask is the carry-out mask
+// shifted up by one lane.  Carries cross waves of a workgroup through LDS
+// once per 64-step group, and cross workgroups (segments of wide pairs)
+// through tagged global entries.  Masks come from a table built on the
+// device (symbols of the packed string: at most 256 per pair) -- staged in
+// LDS per segment when it fits -- and are read ahead of their step; rows
+// whose symbol does not occur in the packed string are dropped on the host
+// (their p is 0 and they leave V unchanged).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,251 +40,72 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStrea
 namespace hy {
 
 constexpr int kMaxSymbols = 256;
-constexpr int kMaxColumns = 16 * 64 * 64 * 16;   // 16 waves x 64 lanes x 16 words: 2^20
+constexpr int kMaxColumns = 1 << 20;   // the widest class: 16,384 words
 
 struct HyPair {
-    int64_t rows_off;   // row symbols (int32 codes into the pair's table) in the row pool
-    int64_t tab_off;    // the pair's mask table (sigma x K words) in the table pool
+    int64_t rows_off;   // row symbols (codes into the pair's table) in the host's row list
+    int64_t tab_off;    // the pair's mask table ((sigma + 1) rows of ks words) in the table pool
     int64_t cols_off;   // column symbols (uint8 codes) in the column pool
+    int64_t roff_off;   // row 0's entry in the row-code pool
     int32_t m, n, K, sigma;
     int32_t out;        // caller index
-    int32_t pad_;
+    int32_t ks;         // table row stride in words: K rounded up to whole segments
 };
 
+// Row-code pool of a pair: entry i (i in [-kRowPadLo, m +
+// kRowPadHi)) is row i's symbol code, and rows outside [0, m) hold sigma,
+// the table's all-zero row.
+constexpr int kRowPadLo = 64, kRowPadHi = 256;
+
+// Mask table of a pair: row s < sigma, word w < K holds bit j = [b[64 w + j]
+// == s]; row sigma and the words past K are zero.
 __global__ void hy_table(const HyPair *__restrict__ pairs, const uint8_t *__restrict__ cols,
                          uint64_t *__restrict__ tab) {
     const HyPair P = pairs[blockIdx.x];
     const uint8_t *b = cols + P.cols_off;
-    const int64_t total = (int64_t)P.sigma * P.K;
+    const int64_t total = (int64_t)(P.sigma + 1) * P.ks;
     for (int64_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int s = (int)(idx / P.K), w = (int)(idx % P.K);
-        const int j0 = 64 * w, j1 = min(P.n, j0 + 64);
+        const int s = (int)(idx / P.ks), w = (int)(idx % P.ks);
         uint64_t mask = 0;
-        for (int j = j0; j < j1; ++j) mask |= (uint64_t)(b[j] == s ? 1 : 0) << (j - j0);
+        if (s < P.sigma && w < P.K) {
+            const int j0 = 64 * w, j1 = min(P.n, j0 + 64);
+            for (int j = j0; j < j1; ++j) mask |= (uint64_t)(b[j] == s ? 1 : 0) << (j - j0);
+        }
         tab[P.tab_off + idx] = mask;
     }
 }
 
-// 64-bit add with carry in and out
-__device__ __forceinline__ uint64_t add_c(uint64_t x, uint64_t y, int &c) {
-    const uint64_t t = x + y;
-    const int o1 = t < x ? 1 : 0;
-    const uint64_t t2 = t + (uint64_t)c;
-    const int o2 = t2 < t ? 1 : 0;
-    c = o1 | o2;
-    return t2;
-}
-
-template <int R, int D, int NW>
-__global__ __launch_bounds__(64 * NW) void hy_rows(const HyPair *__restrict__ pairs, const int32_t *__restrict__ list,
-                                                   const int32_t *__restrict__ rows,
-                                                   const uint64_t *__restrict__ tab, const uint64_t *__restrict__ zero,
-                                                   int32_t *__restrict__ llcs) {
-    static_assert(64 % D == 0, "the row ring must divide the 64-row symbol chunks");
-    static_assert(NW == 1, "one wave per pair: the carry-in of lane 0 is 0");
-    const HyPair P = pairs[list[blockIdx.x]];
-    const int lane = threadIdx.x & 63;
-    const int w0 = lane * R;   // first word of this lane
-    const uint64_t *T = tab + P.tab_off;
-    const int32_t *A = rows + P.rows_off;
-    const int K = P.K, m = P.m;
-    uint64_t V[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) V[r] = ~0ull;
-    // row symbols, 64 per lane-parallel load: rows [c, c + 64) and [c + 64, c + 128)
-    // loads are unconditional (see hy_sys): symbols of rows past m are read
-    // from a clamped index and never used
-    int scur = A[lane < m ? lane : 0];
-    int snxt = A[64 + lane < m ? 64 + lane : 0];
-    // mask ring: masks of rows i .. i + D - 1
-    uint64_t Mq[D][R];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const int sd = __builtin_amdgcn_readlane(scur, d);
-#pragma unroll
-        for (int r = 0; r < R; ++r) Mq[d][r] = *(d < m && w0 + r < K ? T + (int64_t)sd * K + w0 + r : zero);
-    }
-    for (int i0 = 0; i0 < m; i0 += D) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const int i = i0 + d;
-            if (i < m) {   // uniform
-                uint64_t p[R];
-                int c = 0, all_ones = 1;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    p[r] = Mq[d][r] & V[r];
-                    const uint64_t t = add_c(V[r], p[r], c);
-                    all_ones &= t == ~0ull ? 1 : 0;
-                }
-                const uint64_t G = __ballot(c);
-                const uint64_t X = G | __ballot(all_ones);
-                const uint64_t C = (X + G) ^ X ^ G;
-                c = (int)((C >> lane) & 1ull);
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const uint64_t t = add_c(V[r], p[r], c);
-                    V[r] = (V[r] ^ p[r]) | t;
-                }
-                // refill this ring slot with row i + D (its symbol from the
-                // lane buffers: (i + D) - c lies in [D, 64 + D) where c is the
-                // 64-aligned chunk holding i)
-                const int nx = i + D;
-                if (nx < m) {
-                    const int off = nx & 63;
-                    const int sx = (nx >> 6) == (i >> 6) ? __builtin_amdgcn_readlane(scur, off)
-                                                         : __builtin_amdgcn_readlane(snxt, off);
-#pragma unroll
-                    for (int r = 0; r < R; ++r) Mq[d][r] = *(w0 + r < K ? T + (int64_t)sx * K + w0 + r : zero);
-                }
-            }
-        }
-        if (((i0 + D) & 63) == 0) {   // next chunk of 64 rows: shift the symbol buffers
-            scur = snxt;
-            const int nr = i0 + D + 64 + lane;
-            snxt = A[nr < m ? nr : 0];
-        }
-    }
-    // LLCS = zero bits of V (padding bits beyond n stay one)
-    int z = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) z += 64 - __popcll(V[r]);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
-    if (lane == 0) llcs[P.out] = z;
-}
-
-// Systolic form for wide pairs.  Lane L (0 .. 64 NW - 1) owns words
-// [L R, L R + R) and processes row i = t - L at step t, so its carry-in is
-// the carry-out of lane L - 1 at step t - 1 (same row): one DPP wave_shr:1
-// per step inside a wave.  Across waves the carries of lane 63 are collected
-// per group of 64 steps into a 64-bit word in LDS; wave w runs the steps
-// [64 (g - w), 64 (g - w) + 64) in group g (one group behind wave w - 1), so
-// everything it reads was written in groups g - 1 and g - 2 and one barrier
-// per group suffices.  Masks of the row a lane meets at step t + D are
-// gathered at step t into ring slot t mod D (uniform across lanes), their
-// symbols D steps earlier still.
-constexpr int kDppWaveShr1 = 0x138;
-
-template <int R, int D, int NW>
-__global__ __launch_bounds__(64 * NW) void hy_sys(const HyPair *__restrict__ pairs, const int32_t *__restrict__ list,
-                                                  const int32_t *__restrict__ rows, const uint64_t *__restrict__ tab,
-                                                  const uint64_t *__restrict__ zero, int32_t *__restrict__ llcs) {
-    static_assert(64 % D == 0, "the ring must divide a group of 64 steps");
-    __shared__ uint64_t hand[3][NW];   // lane 63's carry-outs per group (slot g mod 3)
-    __shared__ int zsum[NW];
-    const HyPair P = pairs[list[blockIdx.x]];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int L = wave * 64 + lane;
-    const int w0 = L * R;
-    const uint64_t *T = tab + P.tab_off;
-    const int32_t *A = rows + P.rows_off;
-    const int K = P.K, m = P.m;
-    uint64_t V[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) V[r] = ~0ull;
-    uint64_t Mq[D][R];
-    int Sq[D];
-    // step t of this wave is 64 (g - wave) + j; prime the rings for its first
-    // 2D steps (t = -64 wave + 0 .. 2D - 1): rows below 0 have no masks
-    const int t_first = -64 * wave;
-    // Every load below is unconditional and its value is used as loaded:
-    // masks of rows outside [0, m) or words past K come from `zero` (a word
-    // holding 0 after the table pool), symbols of such rows from a clamped
-    // index (they are never used).  Guarded loads would become branches, and
-    // masking a loaded value makes the compiler wait for the load at once;
-    // either empties the prefetch rings every step.
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const int i = t_first + d - L;
-        const bool ok = i >= 0 && i < m;
-        const int s = A[ok ? i : 0];
-#pragma unroll
-        for (int r = 0; r < R; ++r) Mq[d][r] = *(ok && w0 + r < K ? T + (int64_t)s * K + w0 + r : zero);
-        const int i2 = i + D;
-        Sq[d] = A[i2 >= 0 && i2 < m ? i2 : 0];
-    }
-    const int ngroups = (NW - 1) + (m + 64 * NW - 1 + 63) / 64 + 1;
-    int cprev = 0;   // this lane's carry-out at the previous step
-    for (int g = 0; g < ngroups; ++g) {
-        const int t0 = 64 * (g - wave);
-        uint64_t in = 0;
-        if (wave > 0) {
-            const uint64_t prev1 = g >= 1 ? hand[(g - 1) % 3][wave - 1] : 0ull;
-            const uint64_t prev2 = g >= 2 ? hand[(g - 2) % 3][wave - 1] : 0ull;
-            in = (prev1 << 1) | (prev2 >> 63);
-        }
-        uint64_t outbits = 0;
-        for (int j0 = 0; j0 < 64; j0 += D) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const int j = j0 + d;
-                const int t = t0 + j;
-                const int i = t - L;
-                const int from_left = __builtin_amdgcn_update_dpp(0, cprev, kDppWaveShr1, 0xf, 0xf, false);
-                int c = lane == 0 ? (int)((in >> j) & 1ull) : from_left;
-                if (i >= 0 && i < m) {
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const uint64_t p = Mq[d][r] & V[r];
-                        const uint64_t t1 = V[r] + p;
-                        const int o1 = t1 < V[r] ? 1 : 0;
-                        const uint64_t t2 = t1 + (uint64_t)c;
-                        const int o2 = t2 < t1 ? 1 : 0;
-                        V[r] = (V[r] ^ p) | t2;
-                        c = o1 | o2;
-                    }
-                } else {
-                    c = 0;
-                }
-                cprev = c;
-                outbits |= (uint64_t)(__builtin_amdgcn_readlane(c, 63) & 1) << j;
-                // refill: masks for step t + D (symbol already in Sq[d]),
-                // symbol for step t + 2D
-                const int in_ = i + D;
-                const bool ok = in_ >= 0 && in_ < m;
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-                    Mq[d][r] = *(ok && w0 + r < K ? T + (int64_t)Sq[d] * K + w0 + r : zero);
-                const int i2 = i + 2 * D;
-                Sq[d] = A[i2 >= 0 && i2 < m ? i2 : 0];
-            }
-        }
-        if (lane == 0) hand[g % 3][wave] = outbits;
-        __syncthreads();
-    }
-    int z = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) z += 64 - __popcll(V[r]);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
-    if (lane == 0) zsum[wave] = z;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int tot = 0;
-        for (int j = 0; j < NW; ++j) tot += zsum[j];
-        llcs[P.out] = tot;
-    }
-}
-
-// Segmented systolic form for pairs wider than 256 words: the packed string
-// is cut into segments of 256 R words, each a 4-wave workgroup running the
-// systolic scheme of hy_sys with its global wave index Wg = 4 seg + wave (so
-// lanes, rows and the one-group skew continue across segments).  The carry
-// into a segment's lane 0 is the previous segment's lane-63 output: its last
-// wave publishes, per 64-step group, the 64 carry bits as two tagged 64-bit
-// entries (high word: (epoch << 16) | group, low word: 32 carry bits),
-// written and read with agent-scope atomics like the combing kernel's strand
-// hand-off, so a reader that sees the tag has the bits.  Workgroups are
-// persistent and take (pair, segment) tickets in order: a segment waits only
-// for the segment before it, whose ticket a running workgroup already holds.
-// The wait is bounded; running out sets ctl->error and the run reports a
-// timeout.
+// The systolic kernel.  A pair's packed string is cut into segments of
+// 64 NW R words, each a workgroup of NW waves with the global wave index
+// Wg = NW seg + wave (so lanes, rows and the one-group skew continue across
+// segments); pairs up to 2,048 words are one segment.  Wave Wg runs step t =
+// 64 (g - Wg) + j as step j of its group g, so everything it reads from the
+// wave before it was written in groups g - 1 and g - 2, and one barrier per
+// group suffices.  The carry into a segment's lane 0 is the previous
+// segment's lane-63 output: its last wave publishes, per 64-step group, the
+// 64 carry bits as two tagged 64-bit entries (high word: (epoch << 16) |
+// group, low word: 32 carry bits), written and read with agent-scope atomics
+// like the combing kernel's strand hand-off, so a reader that sees the tag
+// has the bits.  Workgroups are persistent and take (pair, segment) tickets
+// in order: a segment waits only for the segment before it, whose ticket a
+// running workgroup already holds.  The wait is bounded; running out sets
+// ctl->error and the run reports a timeout.
+//
+// The step is branch-free and has no range tests.  The carries between
+// lanes live in one 64-bit lane mask: bit L of `cm` is lane L's carry-in, so
+// a lane's R-word addition is a chain of v_addc_co_u32 that takes its first
+// carry from `cm` and leaves its carry-out mask in a scalar register pair;
+// the next step's mask is that one shifted up by a lane, with lane 0's
+// carry-in from the neighbour's bits (scalar ops, no DPP).  Rows outside
+// [0, m) meet the table's all-zero row through the padded row-code pool, and
+// words past K the table's zero padding, so their p is 0 and their carries
+// stay 0.  A wave runs only the groups in which it meets rows of [0, m); in
+// the others its carry bits are 0.
 struct SegItem {
     int32_t pair, seg;
 };
 
-constexpr int kSegWaves = 4;
+constexpr int kSegWaves = 4;   // waves per segment of the segmented (widest) classes
 
 __device__ __forceinline__ uint64_t seg_ld(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -296,18 +114,92 @@ __device__ __forceinline__ void seg_st(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int R, int D>
-__global__ __launch_bounds__(64 * kSegWaves) void hy_seg(const HyPair *__restrict__ pairs,
+// s = v + p + carry-in over a lane's R words (word 0 lowest), the carry-in
+// of every lane taken from the lane mask `cin`; returns the carry-out mask.
+// One v_addc_co_u32 per 32-bit half, the carry passed in scalar registers.
+template <int R>
+__device__ __forceinline__ uint64_t add_words(const uint64_t (&v)[R], const uint64_t (&p)[R], uint64_t cin,
+                                              uint64_t (&s)[R]);
+template <>
+__device__ __forceinline__ uint64_t add_words<1>(const uint64_t (&v)[1], const uint64_t (&p)[1], uint64_t cin,
+                                                 uint64_t (&s)[1]) {
+    uint32_t lo, hi;
+    uint64_t c1, co;
+    asm("v_addc_co_u32_e64 %0, %2, %4, %5, %8\n\t"
+        "v_addc_co_u32_e64 %1, %3, %6, %7, %2"
+        : "=&v"(lo), "=v"(hi), "=&s"(c1), "=s"(co)
+        : "v"((uint32_t)v[0]), "v"((uint32_t)p[0]), "v"((uint32_t)(v[0] >> 32)), "v"((uint32_t)(p[0] >> 32)),
+          "s"(cin));
+    s[0] = ((uint64_t)hi << 32) | lo;
+    return co;
+}
+template <>
+__device__ __forceinline__ uint64_t add_words<2>(const uint64_t (&v)[2], const uint64_t (&p)[2], uint64_t cin,
+                                                 uint64_t (&s)[2]) {
+    uint32_t l0, h0, l1, h1;
+    uint64_t c1, c2, c3, co;
+    asm("v_addc_co_u32_e64 %0, %4, %8, %9, %16\n\t"
+        "v_addc_co_u32_e64 %1, %5, %10, %11, %4\n\t"
+        "v_addc_co_u32_e64 %2, %6, %12, %13, %5\n\t"
+        "v_addc_co_u32_e64 %3, %7, %14, %15, %6"
+        : "=&v"(l0), "=&v"(h0), "=&v"(l1), "=v"(h1), "=&s"(c1), "=&s"(c2), "=&s"(c3), "=s"(co)
+        : "v"((uint32_t)v[0]), "v"((uint32_t)p[0]), "v"((uint32_t)(v[0] >> 32)), "v"((uint32_t)(p[0] >> 32)),
+          "v"((uint32_t)v[1]), "v"((uint32_t)p[1]), "v"((uint32_t)(v[1] >> 32)), "v"((uint32_t)(p[1] >> 32)),
+          "s"(cin));
+    s[0] = ((uint64_t)h0 << 32) | l0;
+    s[1] = ((uint64_t)h1 << 32) | l1;
+    return co;
+}
+
+// A wave-uniform 64-bit value in scalar registers (free where it is there
+// already): the compiler cannot tell that a value carried through the
+// group loop, or an inline-asm scalar output, is uniform.
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(x >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+}
+
+// V = (V ^ p) | s, one v_bitop3_b32 per half (truth table 0xDE: (a ^ c) | b)
+__device__ __forceinline__ uint64_t hy_update(uint64_t v, uint64_t s, uint64_t p) {
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)v, (uint32_t)s, (uint32_t)p, 0xDE);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(v >> 32), (uint32_t)(s >> 32), (uint32_t)(p >> 32), 0xDE);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The R mask words at `q` (global or LDS): one load (R = 2: 16 aligned
+// bytes).
+template <int R, typename Ptr>
+__device__ __forceinline__ void load_masks(Ptr q, uint64_t (&mq)[R]) {
+    if (R == 1) {
+        mq[0] = *(const uint64_t *)q;
+    } else {
+        const ulonglong2 x = *(const ulonglong2 *)q;
+        mq[0] = x.x;
+        mq[1] = x.y;
+    }
+}
+
+// DM: mask ring depth (steps a lane's masks are read ahead); DS: code ring
+// depth (steps the row codes are read ahead of their masks).  LT: the
+// segment's table slice ((sigma + 1) rows of 256 R words) is copied into LDS
+// first and the masks are gathered from there.  A step's gather reads one
+// word of a different row in every lane (the lanes are on different rows):
+// from the global table that is 64 cache lines per wave and step, which the
+// vector memory pipeline serves at about one line per cycle; from LDS it is
+// one conflict-free read.
+template <int R, int DM, int DS, bool LT, int NW>
+__global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pairs,
                                                          const SegItem *__restrict__ items, uint32_t nitems,
                                                          const int64_t *__restrict__ xoff,
-                                                         const int32_t *__restrict__ rows,
-                                                         const uint64_t *__restrict__ tab,
-                                                         const uint64_t *__restrict__ zero, uint64_t *__restrict__ xbuf,
+                                                         const uint32_t *__restrict__ roff,
+                                                         const uint64_t *__restrict__ tab, uint64_t *__restrict__ xbuf,
                                                          uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit,
                                                          int32_t *__restrict__ llcs) {
-    static_assert(64 % D == 0, "the ring must divide a group of 64 steps");
-    constexpr int NW = kSegWaves;
+    static_assert(R == 1 || R == 2, "one or two words per lane");
+    static_assert(64 % DM == 0 && 64 % DS == 0, "the rings must divide a group of 64 steps");
+    constexpr int SW = 64 * NW * R;   // words per segment
     __shared__ uint64_t hand[3][NW];
+    extern __shared__ uint64_t stab[];   // LT: the segment's table slice, row s at s * SW
     __shared__ uint32_t ticket;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // The ticket is read into a scalar register, so the compiler sees a
@@ -331,37 +223,51 @@ __global__ __launch_bounds__(64 * kSegWaves) void hy_seg(const HyPair *__restric
         const int Wg = NW * seg + wave;
         const int L = 64 * Wg + lane;   // global lane of the pair
         const int w0 = L * R;
-        const uint64_t *T = tab + P.tab_off;
-        const int32_t *A = rows + P.rows_off;
+        if (LT) {
+            // the slice: rows 0 .. sigma (sigma: zeros), the segment's words
+            const uint64_t *src = tab + P.tab_off + (int64_t)seg * SW;
+            for (int idx = threadIdx.x; idx < (P.sigma + 1) * SW; idx += 64 * NW)
+                stab[idx] = src[(int64_t)(idx / SW) * P.ks + idx % SW];
+            __syncthreads();
+        }
+        // the lane's masks of row code c: global table word c * ks + w0, or
+        // LDS slice word c * SW + (64 wave + lane) R
+        const char *tb = (const char *)(tab + P.tab_off) + (size_t)w0 * 8u;
+        const uint32_t ks8 = (uint32_t)P.ks * 8u;
+        const uint64_t *sl = stab + (64 * wave + lane) * R;
+        auto fetch = [&](uint32_t code, uint64_t (&mq)[R]) {
+            if (LT) load_masks<R>(sl + code * SW, mq);
+            else load_masks<R>(tb + code * ks8, mq);
+        };
+        // rl[t]: code of the row this lane meets at step t (row t - L)
+        const uint32_t *rl = roff + P.roff_off - L;
         // groups this workgroup runs: wave Wg meets rows in groups [2 Wg, 2 Wg + c]
         const int c = (m + 63) / 64 + 1;
         const int G0 = 2 * NW * seg, G1 = 2 * NW * seg + 2 * (NW - 1) + c;
         const int ng = G1 - G0 + 1;   // the same for every segment of the pair
+        // the groups in which this wave meets rows of [0, m): lane 0 meets row
+        // 0 in group 2 Wg, lane 63 row m - 1 by group 2 Wg + (m + 62) / 64
+        const int ga = 2 * Wg, gb = 2 * Wg + (m + 62) / 64;
         uint64_t *xmine = xbuf + xoff[it.pair] + (int64_t)seg * ng * 2;
         const uint64_t *xprevseg = xbuf + xoff[it.pair] + (int64_t)(seg - 1) * ng * 2;
         uint64_t V[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) V[r] = ~0ull;
-        uint64_t Mq[D][R];
-        int Sq[D];
-        const int t_first = 64 * (G0 - Wg);
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const int i = t_first + d - L;
-            const bool ok = i >= 0 && i < m;
-            const int sy = A[ok ? i : 0];
-#pragma unroll
-            for (int r = 0; r < R; ++r) Mq[d][r] = *(ok && w0 + r < K ? T + (int64_t)sy * K + w0 + r : zero);
-            const int i2 = i + D;
-            Sq[d] = A[i2 >= 0 && i2 < m ? i2 : 0];
-        }
-        // the previous segment's carry word of group gp (zero outside the
-        // groups it ran: rows there lie outside [0, m))
-        auto seg_word = [&](int gp) -> uint64_t {
-            if (seg == 0 || gp < G0 - 2 * NW || gp > G1 - 2 * NW) return 0ull;
-            const uint64_t *e = xprevseg + (int64_t)(gp - (G0 - 2 * NW)) * 2;
+        // The previous segment's carry word of group gp sits in two tagged
+        // entries (none outside the groups that segment ran: rows there lie
+        // outside [0, m) and the word is 0).  Wave 0 issues the loads of
+        // group g's word half-way through its own group g and checks them
+        // at the start of group g + 1, so the wait is off the step chain
+        // and the ring's loads are not drained; a word that is not there
+        // yet is re-read with a bounded wait.
+        const bool ext = wave == 0 && seg > 0;
+        auto seg_entry = [&](int gp) -> const uint64_t * {
+            if (!ext || gp < G0 - 2 * NW || gp > G1 - 2 * NW) return nullptr;
+            return xprevseg + (int64_t)(gp - (G0 - 2 * NW)) * 2;
+        };
+        auto seg_settle = [&](const uint64_t *e, int gp, uint64_t lo, uint64_t hi) -> uint64_t {
+            if (!e) return 0ull;
             const uint32_t want = (epoch << 16) | ((uint32_t)gp & 0xFFFFu);
-            uint64_t lo = seg_ld(e), hi = seg_ld(e + 1);
             uint32_t spins = 0;
             while ((uint32_t)(lo >> 32) != want || (uint32_t)(hi >> 32) != want) {
                 if (++spins >= spin_limit ||
@@ -375,64 +281,88 @@ __global__ __launch_bounds__(64 * kSegWaves) void hy_seg(const HyPair *__restric
             }
             return ((hi & 0xFFFFFFFFull) << 32) | (lo & 0xFFFFFFFFull);
         };
-        uint64_t xprev = wave == 0 ? seg_word(G0 - 2) : 0ull;
-        int cprev = 0;
+        // a valid address for the loads of waves and groups without a word
+        const uint64_t *const dummy = xbuf;
+        uint64_t xprev = 0;
+        const uint64_t *pf_e = nullptr;   // the word in flight: entry and values
+        uint64_t pf_lo = 0, pf_hi = 0;
+        if (ext) {
+            const uint64_t *e = seg_entry(ga - 2);
+            xprev = uniform64(seg_settle(e, ga - 2, seg_ld(e ? e : dummy), seg_ld(e ? e + 1 : dummy)));
+            pf_e = seg_entry(ga - 1);
+            pf_lo = seg_ld(pf_e ? pf_e : dummy);
+            pf_hi = seg_ld(pf_e ? pf_e + 1 : dummy);
+        }
+        // At step j: Mq[j mod DM] holds the step's masks and So[j mod DS]
+        // the code of step j + DM.  Primed after the hand-off word's loads
+        // above, so that a wait for those (at the first group) leaves the
+        // rings' loads in flight.
+        uint64_t Mq[DM][R];
+        uint32_t So[DS];
+        {
+            const int ta = 64 * (ga - Wg);   // the wave's first step with a row of [0, m)
+#pragma unroll
+            for (int d = 0; d < DM; ++d) fetch(rl[ta + d], Mq[d]);
+#pragma unroll
+            for (int e = 0; e < DS; ++e) So[e] = rl[ta + DM + e];
+        }
+        uint64_t co = 0;   // carry-out mask of the wave's previous step
         for (int g = G0; g <= G1; ++g) {
-            const int t0 = 64 * (g - Wg);
-            uint64_t in = 0;
-            if (wave > 0) {
-                const uint64_t p1 = g - 1 >= G0 ? hand[(g - 1) % 3][wave - 1] : 0ull;
-                const uint64_t p2 = g - 2 >= G0 ? hand[(g - 2) % 3][wave - 1] : 0ull;
-                in = (p1 << 1) | (p2 >> 63);
-            } else if (seg > 0) {
-                const uint64_t x1 = seg_word(g - 1);
-                in = (x1 << 1) | (xprev >> 63);
-                xprev = x1;
-            }
-            uint64_t outbits = 0;
-            for (int j0 = 0; j0 < 64; j0 += D) {
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const int j = j0 + d;
-                    const int t = t0 + j;
-                    const int i = t - L;
-                    const int from_left = __builtin_amdgcn_update_dpp(0, cprev, kDppWaveShr1, 0xf, 0xf, false);
-                    int cc = lane == 0 ? (int)((in >> j) & 1ull) : from_left;
-                    if (i >= 0 && i < m) {
-#pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            const uint64_t p = Mq[d][r] & V[r];
-                            const uint64_t t1 = V[r] + p;
-                            const int o1 = t1 < V[r] ? 1 : 0;
-                            const uint64_t t2 = t1 + (uint64_t)cc;
-                            const int o2 = t2 < t1 ? 1 : 0;
-                            V[r] = (V[r] ^ p) | t2;
-                            cc = o1 | o2;
-                        }
-                    } else {
-                        cc = 0;
-                    }
-                    cprev = cc;
-                    outbits |= (uint64_t)(__builtin_amdgcn_readlane(cc, 63) & 1) << j;
-                    const int in_ = i + D;
-                    const bool ok = in_ >= 0 && in_ < m;
-#pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        Mq[d][r] = *(ok && w0 + r < K ? T + (int64_t)Sq[d] * K + w0 + r : zero);
-                    const int i2 = i + 2 * D;
-                    Sq[d] = A[i2 >= 0 && i2 < m ? i2 : 0];
+            uint64_t ob = 0;   // lane 63's carry-outs, bit j at step j
+            if (g >= ga && g <= gb) {
+                // bit j: the carry into lane 0 at step j of this group
+                uint64_t in = 0;
+                if (wave > 0) {
+                    const uint64_t p1 = g - 1 >= G0 ? hand[(g - 1) % 3][wave - 1] : 0ull;
+                    const uint64_t p2 = g - 2 >= G0 ? hand[(g - 2) % 3][wave - 1] : 0ull;
+                    in = (p1 << 1) | (p2 >> 63);
+                } else if (ext) {
+                    const uint64_t x1 = uniform64(seg_settle(pf_e, g - 1, pf_lo, pf_hi));
+                    in = (x1 << 1) | (xprev >> 63);
+                    xprev = x1;
                 }
+                in = uniform64(in);
+                const uint32_t *rg = rl + 64 * (g - Wg) + DM + DS;
+                uint64_t cm = uniform64((co << 1) | (in & 1ull));
+#pragma unroll
+                for (int j = 0; j < 64; ++j) {
+                    const int d = j % DM, e = j % DS;
+                    uint64_t p[R], sum[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) p[r] = Mq[d][r] & V[r];
+                    co = uniform64(add_words<R>(V, p, cm, sum));
+#pragma unroll
+                    for (int r = 0; r < R; ++r) V[r] = hy_update(V[r], sum[r], p[r]);
+                    ob = (ob >> 1) | (co & 0x8000000000000000ull);
+                    if (j < 63) cm = (co << 1) | ((in >> (j + 1)) & 1ull);
+                    // refill: masks for step j + DM, the code for step j + DM + DS
+                    fetch(So[e], Mq[d]);
+                    So[e] = rg[j];
+                    if (j == 32) {   // the previous segment's word of this group
+                        pf_e = seg_entry(g);
+                        pf_lo = seg_ld(pf_e ? pf_e : dummy);
+                        pf_hi = seg_ld(pf_e ? pf_e + 1 : dummy);
+                    }
+                    // keeps each step's loads in its step, ahead of their use
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            } else {
+                co = 0;
             }
             if (lane == 0) {
-                hand[g % 3][wave] = outbits;
+                hand[g % 3][wave] = ob;
                 if (wave == NW - 1 && seg + 1 < nseg) {
                     const uint64_t tag = (uint64_t)((epoch << 16) | ((uint32_t)g & 0xFFFFu)) << 32;
                     uint64_t *e = xmine + (int64_t)(g - G0) * 2;
-                    seg_st(e, tag | (outbits & 0xFFFFFFFFull));
-                    seg_st(e + 1, tag | (outbits >> 32));
+                    seg_st(e, tag | (ob & 0xFFFFFFFFull));
+                    seg_st(e + 1, tag | (ob >> 32));
                 }
             }
-            __syncthreads();
+            // the hand-off words are in LDS: a barrier that waits for LDS
+            // only, so the ring's global loads stay in flight across it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         }
         int z = 0;
 #pragma unroll
@@ -454,24 +384,23 @@ struct Hyyro {
     std::vector<hy::HyPair> pairs;
     std::vector<std::vector<int32_t>> cls_list;   // launch class -> pair indices
     std::vector<int> wpl;                         // launch class -> words per lane of its kernel
+    std::vector<int> cls_sigma;                   // segmented classes: the largest alphabet
     hy::HyPair *d_pairs = nullptr;
-    int32_t *d_list = nullptr;          // the one-workgroup classes' lists, concatenated
-    std::vector<int64_t> list_off;
     // segmented classes: (pair, segment) tickets for each segment width
     // (1 or 2 words per lane), exchange buffer (sized for the narrower
     // segments, which are more), control
-    std::vector<std::vector<hy::SegItem>> seg_items;   // [class - first][R - 1]
+    std::vector<std::vector<hy::SegItem>> seg_items;   // [2 class + R - 1]
     hy::SegItem *d_items = nullptr;
     std::vector<int64_t> item_off;
     int64_t *d_xoff = nullptr;          // per pair: its segments' exchange entries in d_xbuf
     uint64_t *d_xbuf = nullptr;
     int64_t xbuf_entries = 0;
     uint64_t cleared_at = 0;            // run serial when d_xbuf was last cleared
-    LaunchCtl *d_ctl = nullptr;         // one per segmented class
+    LaunchCtl *d_ctl = nullptr;         // one per class
     // a segment waits a few groups (well under a millisecond) for its
     // neighbour; 2^20 polls is over a second of margin and fails fast
     uint32_t spin_limit = 1u << 20;
-    int32_t *d_rows = nullptr;
+    uint32_t *d_roff = nullptr;         // row-code pool of the segmented pairs
     uint8_t *d_cols = nullptr;
     uint64_t *d_tab = nullptr;
     int32_t *d_llcs = nullptr;
@@ -487,18 +416,13 @@ struct Hyyro {
 namespace {
 
 // Launch classes (length buckets) by packed width K (words): up to 64, 128,
-// 256, 2,048 and 16,384 words (2^20 columns).  Each class offers kernels that
-// differ in words per lane (R), and the batch runs one of them per class:
-// the default below, or the one sl_hyyro_autotune measured fastest on the
-// batch's own pairs.
-//  * up to 64 words: one wave of the lookahead kernel (hy_rows), R = 1 or 2;
-//  * up to 128: the systolic kernel (hy_sys) on 2 waves (R = 1), or the
-//    lookahead kernel on one wave (R = 2);
-//  * up to 256: hy_sys on 4 waves (R = 1) or 2 waves (R = 2), or the
-//    lookahead kernel on one wave (R = 4);
-//  * wider pairs are cut into segments of 256 R words on workgroups of their
-//    own (hy_seg, R = 1 or 2), so a long pair spreads over as many compute
-//    units as it has segments.
+// 256, 2,048 and 16,384 words (2^20 columns).  Every class runs the systolic
+// kernel with 1 or 2 words per lane (R), one of them per class: the default
+// (R = 1) or the one sl_hyyro_autotune measured faster on the batch's own
+// pairs.  Waves per segment: the class's width over 64 R, so the classes up
+// to 2,048 words are one segment per pair (1, 2 and 4 waves at R = 1 up to
+// 64, 128 and 256 words); wider pairs are cut into segments of 4 waves, so a
+// long pair spreads over as many compute units as it has segments.
 int hy_class(int K) {
     static const int limits[] = {64, 128, 256, 2048, 16384};
     int c = 0;
@@ -506,37 +430,35 @@ int hy_class(int K) {
     return c;
 }
 constexpr int kClasses = 5;
+constexpr size_t kSegLdsBytes = 64 * 1024;   // largest table slice the kernel holds in LDS
 constexpr int kClassMaxWords[kClasses] = {64, 128, 256, 2048, 16384};
 constexpr int kFirstSegClass = 3;
-constexpr int kSegClasses = 2;   // classes 3 and 4
+constexpr int kSegClasses = 2;   // classes 3 and 4: several segments per pair
 bool is_seg_class(int cls) { return cls >= kFirstSegClass && cls < kFirstSegClass + kSegClasses; }
 // words per lane each class offers, default first
-const std::vector<int> &class_wpl_options(int cls) {
-    static const std::vector<int> opts[kClasses] = {{1, 2}, {1, 2}, {1, 2, 4}, {1, 2}, {2, 1}};
-    return opts[cls];
+const std::vector<int> &class_wpl_options(int) {
+    static const std::vector<int> opts = {1, 2};
+    return opts;
 }
 bool class_offers(int cls, int R) {
     for (int r : class_wpl_options(cls))
         if (r == R) return true;
     return false;
 }
-// waves per pair (one-workgroup classes) or per segment (segmented ones)
+// waves per segment
 int class_waves(int cls, int R) {
-    if (is_seg_class(cls)) return 4;   // hy::kSegWaves
-    if (cls == 0) return 1;
-    if (cls == 1) return R == 1 ? 2 : 1;
-    return R == 1 ? 4 : (R == 2 ? 2 : 1);
+    if (is_seg_class(cls)) return hy::kSegWaves;
+    return std::max(1, kClassMaxWords[cls] / (64 * R));
 }
 
 void hy_free(Hyyro *h) {
     if (!h) return;
     (void)hipFree(h->d_pairs);
-    (void)hipFree(h->d_list);
     (void)hipFree(h->d_items);
     (void)hipFree(h->d_xoff);
     (void)hipFree(h->d_xbuf);
     (void)hipFree(h->d_ctl);
-    (void)hipFree(h->d_rows);
+    (void)hipFree(h->d_roff);
     (void)hipFree(h->d_cols);
     (void)hipFree(h->d_tab);
     (void)hipFree(h->d_llcs);
@@ -608,10 +530,10 @@ int hy_encode_u32(const uint32_t *a, int m, const uint32_t *b, int n, std::vecto
 // Launches the classes in class_mask (bit c: class c) on their streams,
 // forked from and joined back into h->stream.
 int hy_launch(Hyyro *h, unsigned class_mask = ~0u) {
-    // segmented pairs add their segments' zero counts: clear the scores, the
-    // tickets and the error flags first
+    // every wave adds its words' zero count: clear the scores, the tickets
+    // and the error flags first
     SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_llcs, 0u, (uint64_t)std::max(h->npairs, 1), h->stream));
-    SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_ctl, 0u, 2u * kSegClasses, h->stream));
+    SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_ctl, 0u, 2u * kClasses, h->stream));
     uint32_t epoch = 0;
     if (next_run_tag(h->cleared_at, &epoch) && h->xbuf_entries > 0)
         SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_xbuf, 0u, 2u * (uint64_t)h->xbuf_entries, h->stream));
@@ -622,44 +544,39 @@ int hy_launch(Hyyro *h, unsigned class_mask = ~0u) {
         if (!((class_mask >> c) & 1u)) continue;
         hipStream_t cs = h->cls_stream[(size_t)c];
         const int R = h->wpl[(size_t)c];
-        if (is_seg_class(c)) {
-            const size_t k = (size_t)(c - kFirstSegClass);
-            const size_t v = (size_t)(R - 1);
-            const uint32_t nit = (uint32_t)h->seg_items[2 * k + v].size();
-            if (!nit) continue;
-            SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
-            const hy::SegItem *items = h->d_items + h->item_off[2 * k + v];
-            const unsigned blocks = (unsigned)std::min<int64_t>(nit, 2 * cus);
-#define SL_HY_SEG(R, D)                                                                                         \
-    hipLaunchKernelGGL((hy::hy_seg<R, D>), dim3(blocks), dim3(64 * hy::kSegWaves), 0, cs, h->d_pairs, items, nit, \
-                       h->d_xoff, h->d_rows, h->d_tab, h->d_tab + h->tab_words, h->d_xbuf, epoch, h->d_ctl + k,     \
-                       h->spin_limit, h->d_llcs)
-            if (R == 1) SL_HY_SEG(1, 32);
-            else SL_HY_SEG(2, 16);
-#undef SL_HY_SEG
+        const size_t v = (size_t)(2 * c + R - 1);
+        const uint32_t nit = (uint32_t)h->seg_items[v].size();
+        if (!nit) continue;
+        SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
+        const hy::SegItem *items = h->d_items + h->item_off[v];
+        // one workgroup of NW waves per segment of 64 NW R words; enough
+        // persistent workgroups for about 8 waves per compute unit
+        const int NW = class_waves(c, R);
+        const unsigned blocks = (unsigned)std::min<int64_t>(nit, (int64_t)cus * 8 / NW);
+        // the table slice in LDS when it fits in 64 KiB (segmented classes:
+        // alphabets up to 31 symbols at 1 word per lane, 15 at 2)
+        const size_t slice = (size_t)(h->cls_sigma[(size_t)c] + 1) * 64 * NW * R * 8;
+        const bool lds = slice <= kSegLdsBytes;
+#define SL_HY_SEG(R, DM, LT, NW)                                                                                  \
+    hipLaunchKernelGGL((hy::hy_seg<R, DM, 16, LT, NW>), dim3(blocks), dim3(64 * NW), LT ? slice : 0, cs, h->d_pairs, \
+                       items, nit, h->d_xoff, h->d_roff, h->d_tab, h->d_xbuf, epoch, h->d_ctl + c, h->spin_limit,   \
+                       h->d_llcs)
+#define SL_HY_SEG_NW(R, NW)                  \
+    do {                                     \
+        if (lds) SL_HY_SEG(R, 4, true, NW);  \
+        else SL_HY_SEG(R, 16, false, NW);    \
+    } while (0)
+        if (R == 1) {
+            if (NW == 1) SL_HY_SEG_NW(1, 1);
+            else if (NW == 2) SL_HY_SEG_NW(1, 2);
+            else SL_HY_SEG_NW(1, 4);
         } else {
-            const unsigned cnt = (unsigned)h->cls_list[(size_t)c].size();
-            if (!cnt) continue;
-            SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
-            const int32_t *lst = h->d_list + h->list_off[(size_t)c];
-#define SL_HY_LAUNCH(KERNEL, R, D, NW)                                                                       \
-    hipLaunchKernelGGL((hy::KERNEL<R, D, NW>), dim3(cnt), dim3(64 * NW), 0, cs, h->d_pairs, lst, h->d_rows, \
-                       h->d_tab, h->d_tab + h->tab_words, h->d_llcs)
-            // every form covers the class's widest pair: hy_rows 64 R words,
-            // hy_sys 64 NW R words
-            if (c == 0) {
-                if (R == 1) SL_HY_LAUNCH(hy_rows, 1, 16, 1);
-                else SL_HY_LAUNCH(hy_rows, 2, 16, 1);
-            } else if (c == 1) {
-                if (R == 1) SL_HY_LAUNCH(hy_sys, 1, 32, 2);
-                else SL_HY_LAUNCH(hy_rows, 2, 16, 1);
-            } else {
-                if (R == 1) SL_HY_LAUNCH(hy_sys, 1, 32, 4);
-                else if (R == 2) SL_HY_LAUNCH(hy_sys, 2, 16, 2);
-                else SL_HY_LAUNCH(hy_rows, 4, 8, 1);
-            }
-#undef SL_HY_LAUNCH
+            if (NW == 1) SL_HY_SEG_NW(2, 1);
+            else if (NW == 2) SL_HY_SEG_NW(2, 2);
+            else SL_HY_SEG_NW(2, 4);
         }
+#undef SL_HY_SEG_NW
+#undef SL_HY_SEG
         SL_HIP_TRY(hipGetLastError());
         SL_HIP_TRY(hipEventRecord(h->cls_done[(size_t)c], cs));
         SL_HIP_TRY(hipStreamWaitEvent(h->stream, h->cls_done[(size_t)c], 0));
@@ -667,9 +584,9 @@ int hy_launch(Hyyro *h, unsigned class_mask = ~0u) {
     return kOk;
 }
 
-// After a run (stream synchronised): a bounded wait of a segmented class ran out?
+// After a run (stream synchronised): a segment's bounded wait ran out?
 int hy_check(Hyyro *h) {
-    LaunchCtl ctl[kSegClasses];
+    LaunchCtl ctl[kClasses];
     SL_HIP_TRY(hipMemcpy(ctl, h->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
     for (auto &c : ctl)
         if (c.error) {
@@ -692,7 +609,9 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     Hyyro *h = new Hyyro();
     h->npairs = npairs;
     h->cls_list.assign(kClasses, {});
+    h->cls_sigma.assign(kClasses, 0);
     std::vector<int32_t> rows;
+    std::vector<uint32_t> roff;
     std::vector<uint8_t> cols;
     std::vector<int> table(sym_bytes == 1 ? 256 : (sym_bytes == 2 ? 65536 : 0), -1);
     for (int i = 0; i < npairs; ++i) {
@@ -733,48 +652,54 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
         }
         P.m = (int)((int64_t)rows.size() - P.rows_off);
         P.sigma = sigma;
+        // table rows padded to whole segments of the class's widest form, so
+        // a segment's lanes past K read zero words, and the rows' codes
+        // padded on both sides with sigma (the zero row)
+        const int cls = hy_class(P.K);
+        const int segw = std::max(64 * class_waves(cls, 1), 128 * class_waves(cls, 2));
+        P.ks = (P.K + segw - 1) / segw * segw;
+        h->tab_words = (h->tab_words + 1) & ~(int64_t)1;   // 16-byte aligned tables
         P.tab_off = h->tab_words;
-        h->tab_words += (int64_t)sigma * P.K;
+        h->tab_words += (int64_t)(sigma + 1) * P.ks;
+        roff.insert(roff.end(), hy::kRowPadLo, (uint32_t)sigma);
+        P.roff_off = (int64_t)roff.size();
+        roff.insert(roff.end(), rows.begin() + P.rows_off, rows.begin() + P.rows_off + P.m);
+        roff.insert(roff.end(), hy::kRowPadHi, (uint32_t)sigma);
+        h->cls_sigma[(size_t)cls] = std::max(h->cls_sigma[(size_t)cls], sigma);
         h->word_rows += (int64_t)P.m * P.K;
         h->pairs.push_back(P);
         h->cls_list[(size_t)hy_class(P.K)].push_back(i);
     }
-    std::vector<int32_t> all_lists;
-    h->list_off.assign(kClasses, 0);
-    for (int c = 0; c < kClasses; ++c) {
-        if (is_seg_class(c)) continue;
-        h->list_off[(size_t)c] = (int64_t)all_lists.size();
-        all_lists.insert(all_lists.end(), h->cls_list[(size_t)c].begin(), h->cls_list[(size_t)c].end());
-    }
-    // segmented classes: pairs by work (rows x words) descending, each pair's
-    // segments in order; exchange entries: per segment, 2 per group it runs
+    // ticket-kernel items of every class and width: pairs by work (rows x
+    // words) descending, each pair's segments in order; exchange entries
+    // (segmented classes): per segment, 2 per group it runs
     std::vector<hy::SegItem> all_items;
     std::vector<int64_t> xoff((size_t)std::max(npairs, 1), 0);
     int64_t xtotal = 0;
-    h->seg_items.assign(2 * kSegClasses, {});
-    h->item_off.assign(2 * kSegClasses, 0);
-    for (int c = kFirstSegClass; c < kFirstSegClass + kSegClasses; ++c) {
+    h->seg_items.assign(2 * kClasses, {});
+    h->item_off.assign(2 * kClasses, 0);
+    for (int c = 0; c < kClasses; ++c) {
         auto lst = h->cls_list[(size_t)c];
         std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) {
             return (int64_t)h->pairs[(size_t)x].m * h->pairs[(size_t)x].K >
                    (int64_t)h->pairs[(size_t)y].m * h->pairs[(size_t)y].K;
         });
-        const size_t k = (size_t)(c - kFirstSegClass);
-        for (int pi : lst) {   // exchange entries for the narrower (R = 1) segments, which are more
-            const hy::HyPair &P = h->pairs[(size_t)pi];
-            const int nseg = (P.K + 64 * hy::kSegWaves - 1) / (64 * hy::kSegWaves);
-            const int64_t ng = (P.m + 63) / 64 + 8;   // groups a segment runs (hy_seg: G1 - G0 + 1)
-            xoff[(size_t)pi] = xtotal;
-            xtotal += (int64_t)nseg * ng * 2;
-        }
+        if (is_seg_class(c))
+            for (int pi : lst) {   // exchange entries for the narrower (R = 1) segments, which are more
+                const hy::HyPair &P = h->pairs[(size_t)pi];
+                const int nseg = (P.K + 64 * hy::kSegWaves - 1) / (64 * hy::kSegWaves);
+                const int64_t ng = (P.m + 63) / 64 + 8;   // groups a segment runs (hy_seg: G1 - G0 + 1)
+                xoff[(size_t)pi] = xtotal;
+                xtotal += (int64_t)nseg * ng * 2;
+            }
         for (int R = 1; R <= 2; ++R) {
-            const int segw = 64 * hy::kSegWaves * R;
-            auto &items = h->seg_items[2 * k + (size_t)(R - 1)];
+            const int segw = 64 * class_waves(c, R) * R;
+            auto &items = h->seg_items[2 * (size_t)c + (size_t)(R - 1)];
             for (int pi : lst) {
                 const int nseg = (h->pairs[(size_t)pi].K + segw - 1) / segw;
                 for (int sg = 0; sg < nseg; ++sg) items.push_back(hy::SegItem{pi, sg});
             }
-            h->item_off[2 * k + (size_t)(R - 1)] = (int64_t)all_items.size();
+            h->item_off[2 * (size_t)c + (size_t)(R - 1)] = (int64_t)all_items.size();
             all_items.insert(all_items.end(), items.begin(), items.end());
         }
     }
@@ -795,19 +720,16 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
         if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 8));
     };
     alloc((void **)&h->d_pairs, sizeof(hy::HyPair) * h->pairs.size());
-    alloc((void **)&h->d_list, sizeof(int32_t) * all_lists.size());
     alloc((void **)&h->d_items, sizeof(hy::SegItem) * all_items.size());
     alloc((void **)&h->d_xoff, sizeof(int64_t) * xoff.size());
     alloc((void **)&h->d_xbuf, sizeof(uint64_t) * (size_t)xtotal);
-    alloc((void **)&h->d_ctl, sizeof(LaunchCtl) * kSegClasses);
-    alloc((void **)&h->d_rows, sizeof(int32_t) * (rows.size() + 1));   // + a spare: clamped loads read index 0
+    alloc((void **)&h->d_ctl, sizeof(LaunchCtl) * kClasses);
+    alloc((void **)&h->d_roff, sizeof(uint32_t) * roff.size());
     alloc((void **)&h->d_cols, cols.size());
-    alloc((void **)&h->d_tab, sizeof(uint64_t) * (size_t)(h->tab_words + 1));   // + the zero word
+    alloc((void **)&h->d_tab, sizeof(uint64_t) * (size_t)h->tab_words);
     alloc((void **)&h->d_llcs, sizeof(int32_t) * (size_t)std::max(npairs, 1));
     if (e == hipSuccess && !h->pairs.empty())
         e = hipMemcpy(h->d_pairs, h->pairs.data(), sizeof(hy::HyPair) * h->pairs.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !all_lists.empty())
-        e = hipMemcpy(h->d_list, all_lists.data(), sizeof(int32_t) * all_lists.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !all_items.empty())
         e = hipMemcpy(h->d_items, all_items.data(), sizeof(hy::SegItem) * all_items.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_xoff, xoff.data(), sizeof(int64_t) * xoff.size(), hipMemcpyHostToDevice);
@@ -815,11 +737,9 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     if (e == hipSuccess && xtotal > 0) e = launch_fill_u32((uint32_t *)h->d_xbuf, 0u, 2u * (uint64_t)xtotal, h->stream);
     h->xbuf_entries = xtotal;
     h->cleared_at = run_serial_now();
-    if (e == hipSuccess && !rows.empty())
-        e = hipMemcpy(h->d_rows, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !roff.empty())
+        e = hipMemcpy(h->d_roff, roff.data(), sizeof(uint32_t) * roff.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !cols.empty()) e = hipMemcpy(h->d_cols, cols.data(), cols.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess)   // the zero word after the table pool (masks of rows and words outside a pair)
-        e = hipMemsetAsync(h->d_tab + h->tab_words, 0, sizeof(uint64_t), h->stream);
     if (e == hipSuccess && npairs > 0) {
         hipLaunchKernelGGL(hy::hy_table, dim3((unsigned)npairs), dim3(256), 0, h->stream, h->d_pairs, h->d_cols,
                            h->d_tab);

@@ -1,9 +1,9 @@
 """GPU parity of the Hyyro bit-vector LLCS (sl_llcs_hyyro, sl_hyyro_*):
 scores equal plain LCS dynamic programming (oracle) across the launch
-classes -- the one-wave lookahead kernel up to 64 words, the one-workgroup
-systolic kernel up to 256, the segmented systolic kernel (1 and 2 words per
-lane, one workgroup per segment) up to 8,192 and the 16-wave systolic kernel
-beyond -- at word, lane, wave and segment boundaries, for 1-, 2- and 4-byte
+classes -- the systolic kernel with 1 and 2 words per lane, one workgroup of
+1, 2 or 4 waves per pair up to 2,048 words and segments of 4 waves beyond,
+its mask table in LDS for small alphabets and in global memory for large
+ones -- at word, lane, wave and segment boundaries, for 1-, 2- and 4-byte
 symbols; at full size they equal the semi-local kernel's LLCS.  Bar: exact
 integers."""
 import numpy as np
@@ -76,7 +76,7 @@ def test_unsupported_inputs_are_rejected(gpu):
 
 # offered words per lane per bucket (packed width up to 64, 128, 256, 2,048
 # and 16,384 words), default first
-WPL_OPTIONS = [[1, 2], [1, 2], [1, 2, 4], [1, 2], [2, 1]]
+WPL_OPTIONS = [[1, 2], [1, 2], [1, 2], [1, 2], [1, 2]]
 
 
 def test_every_words_per_lane_form_equals_dp(gpu, oracle):
@@ -101,7 +101,7 @@ def test_every_words_per_lane_form_equals_dp(gpu, oracle):
         print(f"hyyro words per lane {wpl}", flush=True)
         assert hb.run().tolist() == want
     with pytest.raises(gpu.SemiLocalError):
-        hb.set_words_per_lane([4, 0, 0, 0, 0])                # bucket 0 has no 4-word form
+        hb.set_words_per_lane([4, 0, 0, 0, 0])                # no bucket has a 4-word form
     chosen = hb.autotune(rounds=1)
     assert all(c in o for c, o in zip(chosen, WPL_OPTIONS))
     assert [b["words_per_lane"] for b in hb.buckets()] == chosen
