@@ -194,8 +194,11 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
                                                          const uint32_t *__restrict__ roff,
                                                          const uint64_t *__restrict__ tab, uint64_t *__restrict__ xbuf,
                                                          uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit,
-                                                         int32_t *__restrict__ llcs) {
+                                                         int32_t *__restrict__ llcs, int raise_prio) {
     static_assert(R == 1 || R == 2, "one or two words per lane");
+    // the widest classes hold the longest row chains: their waves issue first
+    // on a SIMD they share with the other classes' waves
+    if (raise_prio) __builtin_amdgcn_s_setprio(2);
     static_assert(64 % DM == 0 && 64 % DS == 0, "the rings must divide a group of 64 steps");
     constexpr int SW = 64 * NW * R;   // words per segment
     __shared__ uint64_t hand[3][NW];
@@ -406,10 +409,11 @@ struct Hyyro {
     int32_t *d_llcs = nullptr;
     int64_t tab_words = 0, word_rows = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    // launch classes are independent: each runs on a stream of its own
-    // (default priority), forked from and joined back into `stream`
-    std::vector<hipStream_t> cls_stream;
-    std::vector<hipEvent_t> cls_done;
+    // launch classes are independent: the segmented (long-chain) classes run
+    // on one stream, the others one after another on a second, both forked
+    // from and joined back into `stream` (default priorities)
+    hipStream_t lane_stream[2] = {nullptr, nullptr};   // [0]: segmented classes, [1]: the others
+    hipEvent_t lane_done[2] = {nullptr, nullptr};
     hipEvent_t fork = nullptr;
 };
 
@@ -464,10 +468,10 @@ void hy_free(Hyyro *h) {
     (void)hipFree(h->d_llcs);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
-    for (auto &e : h->cls_done)
+    for (auto &e : h->lane_done)
         if (e) (void)hipEventDestroy(e);
     if (h->fork) (void)hipEventDestroy(h->fork);
-    for (auto &cs : h->cls_stream)
+    for (auto &cs : h->lane_stream)
         if (cs) (void)hipStreamDestroy(cs);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -538,16 +542,19 @@ int hy_launch(Hyyro *h, unsigned class_mask = ~0u) {
     if (next_run_tag(h->cleared_at, &epoch) && h->xbuf_entries > 0)
         SL_HIP_TRY(launch_fill_u32((uint32_t *)h->d_xbuf, 0u, 2u * (uint64_t)h->xbuf_entries, h->stream));
     SL_HIP_TRY(hipEventRecord(h->fork, h->stream));
+    for (hipStream_t ls : h->lane_stream) SL_HIP_TRY(hipStreamWaitEvent(ls, h->fork, 0));
     const int cus = device_cus();
-    // widest class first: it holds the longest pairs
+    // widest class first: it holds the longest pairs.  Every launch is
+    // enqueued before `stream` waits for any of them: streams can share a
+    // hardware queue, and a launch behind such a wait would start only
+    // when the waited-for lane ends.
     for (int c = kClasses - 1; c >= 0; --c) {
         if (!((class_mask >> c) & 1u)) continue;
-        hipStream_t cs = h->cls_stream[(size_t)c];
+        hipStream_t cs = h->lane_stream[is_seg_class(c) ? 0 : 1];
         const int R = h->wpl[(size_t)c];
         const size_t v = (size_t)(2 * c + R - 1);
         const uint32_t nit = (uint32_t)h->seg_items[v].size();
         if (!nit) continue;
-        SL_HIP_TRY(hipStreamWaitEvent(cs, h->fork, 0));
         const hy::SegItem *items = h->d_items + h->item_off[v];
         // one workgroup of NW waves per segment of 64 NW R words; enough
         // persistent workgroups for about 8 waves per compute unit
@@ -560,7 +567,7 @@ int hy_launch(Hyyro *h, unsigned class_mask = ~0u) {
 #define SL_HY_SEG(R, DM, LT, NW)                                                                                  \
     hipLaunchKernelGGL((hy::hy_seg<R, DM, 16, LT, NW>), dim3(blocks), dim3(64 * NW), LT ? slice : 0, cs, h->d_pairs, \
                        items, nit, h->d_xoff, h->d_roff, h->d_tab, h->d_xbuf, epoch, h->d_ctl + c, h->spin_limit,   \
-                       h->d_llcs)
+                       h->d_llcs, is_seg_class(c) ? 1 : 0)
 #define SL_HY_SEG_NW(R, NW)                  \
     do {                                     \
         if (lds) SL_HY_SEG(R, 4, true, NW);  \
@@ -578,8 +585,10 @@ int hy_launch(Hyyro *h, unsigned class_mask = ~0u) {
 #undef SL_HY_SEG_NW
 #undef SL_HY_SEG
         SL_HIP_TRY(hipGetLastError());
-        SL_HIP_TRY(hipEventRecord(h->cls_done[(size_t)c], cs));
-        SL_HIP_TRY(hipStreamWaitEvent(h->stream, h->cls_done[(size_t)c], 0));
+    }
+    for (int k = 0; k < 2; ++k) {
+        SL_HIP_TRY(hipEventRecord(h->lane_done[k], h->lane_stream[k]));
+        SL_HIP_TRY(hipStreamWaitEvent(h->stream, h->lane_done[k], 0));
     }
     return kOk;
 }
@@ -709,12 +718,9 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     for (auto &ev : h->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
-    h->cls_stream.assign(kClasses, nullptr);
-    h->cls_done.assign(kClasses, nullptr);
-    for (int c = 0; c < kClasses && e == hipSuccess; ++c) {
-        if (h->cls_list[(size_t)c].empty()) continue;
-        e = hipStreamCreateWithFlags(&h->cls_stream[(size_t)c], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->cls_done[(size_t)c], hipEventDisableTiming);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&h->lane_stream[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->lane_done[k], hipEventDisableTiming);
     }
     auto alloc = [&](void **p, size_t bytes) {
         if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 8));
