@@ -416,9 +416,9 @@ int sl_hyyro_stats(const sl_hyyro *h, int64_t *word_rows, int64_t *table_words);
 /* Length buckets by packed width (64-bit words of the longer string): up to */
 /* 64, 128, 256, 2048 and 16384 words.  Each bucket runs one kernel form,    */
 /* chosen by its words per lane (the reference's analogue is the word type   */
-/* and threads of hyyro_magic_mpi_with_precalc, one per call):              */
-/*   bucket 0: 1 or 2; 1: 1 (2 waves) or 2 (1 wave); 2: 1, 2 or 4 (4, 2, 1   */
-/*   waves); 3 and 4: 1 or 2 (segments of 256 x words-per-lane words).       */
+/* and threads of hyyro_magic_mpi_with_precalc, one per call): 1 or 2 in     */
+/* every bucket; buckets 0-2 run one workgroup per pair (1, 2, 4 waves at 1  */
+/* word per lane, half at 2), buckets 3-4 segments of 4 waves.               */
 /* Arrays of SL_HYYRO_BUCKETS entries; any pointer may be null.  waves[b] is */
 /* per pair (buckets 0-2) or per segment (3-4).                              */
 #define SL_HYYRO_BUCKETS 5
