@@ -187,6 +187,10 @@ __device__ __forceinline__ void load_masks(Ptr q, uint64_t (&mq)[R]) {
 // from the global table that is 64 cache lines per wave and step, which the
 // vector memory pipeline serves at about one line per cycle; from LDS it is
 // one conflict-free read.
+// s_waitcnt vmcnt(15) lgkmcnt(3) (expcnt not waited): the encoding of the
+// immediate on gfx9 -- vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8
+constexpr int kWaitVm15Lgkm3 = 0xF | (0x7 << 4) | (0x3 << 8);
+
 template <int R, int DM, int DS, bool LT, int NW>
 __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pairs,
                                                          const SegItem *__restrict__ items, uint32_t nitems,
@@ -200,6 +204,7 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
     // on a SIMD they share with the other classes' waves
     if (raise_prio) __builtin_amdgcn_s_setprio(2);
     static_assert(64 % DM == 0 && 64 % DS == 0, "the rings must divide a group of 64 steps");
+    static_assert(!LT || (DM == 4 && DS == 16), "the LDS form's one wait per step assumes rings of 4 and 16");
     constexpr int SW = 64 * NW * R;   // words per segment
     __shared__ uint64_t hand[3][NW];
     extern __shared__ uint64_t stab[];   // LT: the segment's table slice, row s at s * SW
@@ -313,6 +318,9 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
         for (int g = G0; g <= G1; ++g) {
             uint64_t ob = 0;   // lane 63's carry-outs, bit j at step j
             if (g >= ga && g <= gb) {
+                // lane j of vob: the high half of step j's carry-out mask
+                // (bit 31: lane 63's carry), one v_writelane_b32 per step
+                uint32_t vob = 0;
                 // bit j: the carry into lane 0 at step j of this group
                 uint64_t in = 0;
                 if (wave > 0) {
@@ -330,13 +338,17 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
 #pragma unroll
                 for (int j = 0; j < 64; ++j) {
                     const int d = j % DM, e = j % DS;
+                    // one wait for both rings: the masks read DM steps ago
+                    // (LDS) and the code read DS steps ago (global); each step
+                    // issues one read of each
+                    if (LT) __builtin_amdgcn_s_waitcnt(kWaitVm15Lgkm3);
                     uint64_t p[R], sum[R];
 #pragma unroll
                     for (int r = 0; r < R; ++r) p[r] = Mq[d][r] & V[r];
                     co = uniform64(add_words<R>(V, p, cm, sum));
 #pragma unroll
                     for (int r = 0; r < R; ++r) V[r] = hy_update(V[r], sum[r], p[r]);
-                    ob = (ob >> 1) | (co & 0x8000000000000000ull);
+                    asm("v_writelane_b32 %0, %1, %2" : "+v"(vob) : "s"((uint32_t)(co >> 32)), "i"(j));
                     if (j < 63) cm = (co << 1) | ((in >> (j + 1)) & 1ull);
                     // refill: masks for step j + DM, the code for step j + DM + DS
                     fetch(So[e], Mq[d]);
@@ -349,6 +361,7 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
                     // keeps each step's loads in its step, ahead of their use
                     __builtin_amdgcn_sched_barrier(0);
                 }
+                ob = __builtin_amdgcn_ballot_w64((int32_t)vob < 0);
             } else {
                 co = 0;
             }
