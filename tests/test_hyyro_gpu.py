@@ -53,6 +53,27 @@ def test_wide_symbols_equal_dp(gpu, oracle, dtype, sigma):
     assert gpu.llcs_hyyro(a, b) == oracle.llcs_dp(a.astype(np.int64).astype(np.int32), b.astype(np.int32))
 
 
+# the segmented form's mask table sits in LDS while (sigma + 1) rows of a
+# segment fit 64 KiB: sigma <= 31 at 1 word per lane, <= 15 at 2; larger
+# alphabets gather from the global table.  Pairs of 4 to 8 segments, both
+# sides of both limits, with the segment hand-off in play.
+@pytest.mark.parametrize("sigma", [15, 16, 31, 32, 200])
+def test_segmented_pairs_both_table_forms_equal_dp(gpu, oracle, sigma):
+    g = np.random.default_rng(100 + sigma)
+    a = g.integers(0, sigma, 400).astype(np.uint8)
+    b = g.integers(0, sigma, 64 * 1024 + 37).astype(np.uint8)   # 1,025 words
+    b[:sigma] = np.arange(sigma, dtype=np.uint8)               # every symbol occurs in the packed string
+    want = oracle.llcs_dp(a, b)
+    hb = gpu.HyyroBatch([(a, b), (b[:5000], a)])
+    for r in (1, 2):
+        hb.set_words_per_lane([0, 0, 0, r, 0])
+        print(f"hyyro segmented sigma={sigma} words per lane {r}", flush=True)
+        got = hb.run()
+        assert got[0] == want
+        assert got[1] == oracle.llcs_dp(b[:5000], a)
+    hb.close()
+
+
 def test_large_pairs_equal_the_semi_local_kernel(gpu):
     g = np.random.default_rng(9)
     for m, n in [(3000, 1 << 20), (20_000, 300_000)]:
