@@ -415,17 +415,19 @@ int sl_hyyro_time(sl_hyyro *h, int warmup, int iters, double *ms_per_run);
 int sl_hyyro_stats(const sl_hyyro *h, int64_t *word_rows, int64_t *table_words);
 /* Length buckets by packed width (64-bit words of the longer string): up to */
 /* 64, 128, 256, 2048 and 16384 words.  Each bucket runs one kernel form,    */
-/* chosen by its words per lane (the reference's analogue is the word type   */
-/* and threads of hyyro_magic_mpi_with_precalc, one per call): 1 or 2 in     */
-/* every bucket; buckets 0-2 run one workgroup per pair (1, 2, 4 waves at 1  */
-/* word per lane, half at 2), buckets 3-4 segments of 4 waves.               */
-/* Arrays of SL_HYYRO_BUCKETS entries; any pointer may be null.  waves[b] is */
-/* per pair (buckets 0-2) or per segment (3-4).                              */
+/* chosen by its 32-bit words per lane (the reference's analogue is the word */
+/* type and threads of hyyro_magic_mpi_with_precalc, one per call): 2 (64    */
+/* bits, the default), 1 or 4 in buckets 0, 1, 3, 4; 2 or 4 in bucket 2.     */
+/* Buckets 0-2 run one workgroup per pair (1, 2, 4 waves at 2 words per      */
+/* lane), buckets 3-4 segments of 4 waves.  Arrays of SL_HYYRO_BUCKETS       */
+/* entries; any pointer may be null.  max_words[b] is in 64-bit words,       */
+/* words_per_lane[b] in 32-bit words, waves[b] per pair (buckets 0-2) or per */
+/* segment (3-4).                                                            */
 #define SL_HYYRO_BUCKETS 5
 int sl_hyyro_buckets(const sl_hyyro *h, int32_t *max_words, int32_t *pairs, int32_t *words_per_lane,
                      int32_t *waves);
-/* Forces the words per lane per bucket (0 keeps a bucket's choice); a value */
-/* the bucket does not offer is SL_ERR_ARG.                                  */
+/* Forces the 32-bit words per lane per bucket (0 keeps a bucket's choice); */
+/* a value the bucket does not offer is SL_ERR_ARG.                          */
 int sl_hyyro_set_words_per_lane(sl_hyyro *h, const int32_t *words_per_lane);
 /* Autotuner: times every offered form on each bucket's own pairs (rounds    */
 /* rounds of two timed launches after a warm-up) and keeps the fastest;      */

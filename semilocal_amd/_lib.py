@@ -336,8 +336,9 @@ class HyyroBatch:
     HYYRO_BUCKETS = 5
 
     def buckets(self) -> list:
-        """Per length bucket (packed width up to max_words): pairs, words per
-        lane of the kernel it runs, and waves per pair or segment."""
+        """Per length bucket (packed width up to max_words 64-bit words):
+        pairs, 32-bit words per lane of the kernel it runs (1, 2 or 4: 32, 64
+        or 128 bits), and waves per pair or segment."""
         n = self.HYYRO_BUCKETS
         mw, pr, wpl, wv = (np.zeros(n, np.int32) for _ in range(4))
         check(lib().sl_hyyro_buckets(self.h, mw, pr, wpl, wv))
@@ -345,6 +346,7 @@ class HyyroBatch:
                  "waves": int(wv[i])} for i in range(n)]
 
     def set_words_per_lane(self, wpl) -> None:
+        """Forces the 32-bit words per lane per bucket (0 keeps a bucket's)."""
         arr = np.ascontiguousarray(wpl, np.int32)
         if arr.shape != (self.HYYRO_BUCKETS,):
             raise ValueError("one words-per-lane value per bucket")

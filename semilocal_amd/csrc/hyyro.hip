@@ -114,40 +114,41 @@ __device__ __forceinline__ void seg_st(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// s = v + p + carry-in over a lane's R words (word 0 lowest), the carry-in
-// of every lane taken from the lane mask `cin`; returns the carry-out mask.
-// One v_addc_co_u32 per 32-bit half, the carry passed in scalar registers.
-template <int R>
-__device__ __forceinline__ uint64_t add_words(const uint64_t (&v)[R], const uint64_t (&p)[R], uint64_t cin,
-                                              uint64_t (&s)[R]);
+// A lane holds H 32-bit words of the packed string (H = 1, 2 or 4: 32, 64
+// or 128 bits, word 0 lowest).  s = v + p + carry-in over the lane's words,
+// the carry-in of every lane taken from the lane mask `cin`; returns the
+// carry-out mask.  One v_addc_co_u32 per word, the carry passed in scalar
+// registers.
+template <int H>
+__device__ __forceinline__ uint64_t add_words(const uint32_t (&v)[H], const uint32_t (&p)[H], uint64_t cin,
+                                              uint32_t (&s)[H]);
 template <>
-__device__ __forceinline__ uint64_t add_words<1>(const uint64_t (&v)[1], const uint64_t (&p)[1], uint64_t cin,
-                                                 uint64_t (&s)[1]) {
-    uint32_t lo, hi;
-    uint64_t c1, co;
-    asm("v_addc_co_u32_e64 %0, %2, %4, %5, %8\n\t"
-        "v_addc_co_u32_e64 %1, %3, %6, %7, %2"
-        : "=&v"(lo), "=v"(hi), "=&s"(c1), "=s"(co)
-        : "v"((uint32_t)v[0]), "v"((uint32_t)p[0]), "v"((uint32_t)(v[0] >> 32)), "v"((uint32_t)(p[0] >> 32)),
-          "s"(cin));
-    s[0] = ((uint64_t)hi << 32) | lo;
+__device__ __forceinline__ uint64_t add_words<1>(const uint32_t (&v)[1], const uint32_t (&p)[1], uint64_t cin,
+                                                 uint32_t (&s)[1]) {
+    uint64_t co;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(s[0]), "=s"(co) : "v"(v[0]), "v"(p[0]), "s"(cin));
     return co;
 }
 template <>
-__device__ __forceinline__ uint64_t add_words<2>(const uint64_t (&v)[2], const uint64_t (&p)[2], uint64_t cin,
-                                                 uint64_t (&s)[2]) {
-    uint32_t l0, h0, l1, h1;
+__device__ __forceinline__ uint64_t add_words<2>(const uint32_t (&v)[2], const uint32_t (&p)[2], uint64_t cin,
+                                                 uint32_t (&s)[2]) {
+    uint64_t c1, co;
+    asm("v_addc_co_u32_e64 %0, %2, %4, %5, %8\n\t"
+        "v_addc_co_u32_e64 %1, %3, %6, %7, %2"
+        : "=&v"(s[0]), "=v"(s[1]), "=&s"(c1), "=s"(co)
+        : "v"(v[0]), "v"(p[0]), "v"(v[1]), "v"(p[1]), "s"(cin));
+    return co;
+}
+template <>
+__device__ __forceinline__ uint64_t add_words<4>(const uint32_t (&v)[4], const uint32_t (&p)[4], uint64_t cin,
+                                                 uint32_t (&s)[4]) {
     uint64_t c1, c2, c3, co;
     asm("v_addc_co_u32_e64 %0, %4, %8, %9, %16\n\t"
         "v_addc_co_u32_e64 %1, %5, %10, %11, %4\n\t"
         "v_addc_co_u32_e64 %2, %6, %12, %13, %5\n\t"
         "v_addc_co_u32_e64 %3, %7, %14, %15, %6"
-        : "=&v"(l0), "=&v"(h0), "=&v"(l1), "=v"(h1), "=&s"(c1), "=&s"(c2), "=&s"(c3), "=s"(co)
-        : "v"((uint32_t)v[0]), "v"((uint32_t)p[0]), "v"((uint32_t)(v[0] >> 32)), "v"((uint32_t)(p[0] >> 32)),
-          "v"((uint32_t)v[1]), "v"((uint32_t)p[1]), "v"((uint32_t)(v[1] >> 32)), "v"((uint32_t)(p[1] >> 32)),
-          "s"(cin));
-    s[0] = ((uint64_t)h0 << 32) | l0;
-    s[1] = ((uint64_t)h1 << 32) | l1;
+        : "=&v"(s[0]), "=&v"(s[1]), "=&v"(s[2]), "=v"(s[3]), "=&s"(c1), "=&s"(c2), "=&s"(c3), "=s"(co)
+        : "v"(v[0]), "v"(p[0]), "v"(v[1]), "v"(p[1]), "v"(v[2]), "v"(p[2]), "v"(v[3]), "v"(p[3]), "s"(cin));
     return co;
 }
 
@@ -159,30 +160,34 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t x) {
            (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
 }
 
-// V = (V ^ p) | s, one v_bitop3_b32 per half (truth table 0xDE: (a ^ c) | b)
-__device__ __forceinline__ uint64_t hy_update(uint64_t v, uint64_t s, uint64_t p) {
-    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)v, (uint32_t)s, (uint32_t)p, 0xDE);
-    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(v >> 32), (uint32_t)(s >> 32), (uint32_t)(p >> 32), 0xDE);
-    return ((uint64_t)hi << 32) | lo;
+// V = (V ^ p) | s: one v_bitop3_b32 (truth table 0xDE: (a ^ c) | b)
+__device__ __forceinline__ uint32_t hy_update(uint32_t v, uint32_t s, uint32_t p) {
+    return __builtin_amdgcn_bitop3_b32(v, s, p, 0xDE);
 }
 
-// The R mask words at `q` (global or LDS): one load (R = 2: 16 aligned
-// bytes).
-template <int R, typename Ptr>
-__device__ __forceinline__ void load_masks(Ptr q, uint64_t (&mq)[R]) {
-    if (R == 1) {
-        mq[0] = *(const uint64_t *)q;
-    } else {
-        const ulonglong2 x = *(const ulonglong2 *)q;
+// The H mask words at `q` (global or LDS): one load of 4 H aligned bytes.
+template <int H, typename Ptr>
+__device__ __forceinline__ void load_masks(Ptr q, uint32_t (&mq)[H]) {
+    if (H == 1) {
+        mq[0] = *(const uint32_t *)q;
+    } else if (H == 2) {
+        const uint2 x = *(const uint2 *)q;
         mq[0] = x.x;
         mq[1] = x.y;
+    } else {
+        const uint4 x = *(const uint4 *)q;
+        mq[0] = x.x;
+        mq[1] = x.y;
+        mq[2] = x.z;
+        mq[3] = x.w;
     }
 }
 
-// DM: mask ring depth (steps a lane's masks are read ahead); DS: code ring
-// depth (steps the row codes are read ahead of their masks).  LT: the
-// segment's table slice ((sigma + 1) rows of 256 R words) is copied into LDS
-// first and the masks are gathered from there.  A step's gather reads one
+// H: 32-bit words per lane (1, 2 or 4).  DM: mask ring depth (steps a lane's
+// masks are read ahead); DS: code ring depth (steps the row codes are read
+// ahead of their masks).  LT: the segment's table slice ((sigma + 1) rows of
+// 64 NW H words) is copied into LDS first and the masks are gathered from
+// there.  A step's gather reads one
 // word of a different row in every lane (the lanes are on different rows):
 // from the global table that is 64 cache lines per wave and step, which the
 // vector memory pipeline serves at about one line per cycle; from LDS it is
@@ -191,7 +196,7 @@ __device__ __forceinline__ void load_masks(Ptr q, uint64_t (&mq)[R]) {
 // immediate on gfx9 -- vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8
 constexpr int kWaitVm15Lgkm3 = 0xF | (0x7 << 4) | (0x3 << 8);
 
-template <int R, int DM, int DS, bool LT, int NW>
+template <int H, int DM, int DS, bool LT, int NW>
 __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pairs,
                                                          const SegItem *__restrict__ items, uint32_t nitems,
                                                          const int64_t *__restrict__ xoff,
@@ -199,15 +204,15 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
                                                          const uint64_t *__restrict__ tab, uint64_t *__restrict__ xbuf,
                                                          uint32_t epoch, LaunchCtl *ctl, uint32_t spin_limit,
                                                          int32_t *__restrict__ llcs, int raise_prio) {
-    static_assert(R == 1 || R == 2, "one or two words per lane");
+    static_assert(H == 1 || H == 2 || H == 4, "32, 64 or 128 bits per lane");
     // the widest classes hold the longest row chains: their waves issue first
     // on a SIMD they share with the other classes' waves
     if (raise_prio) __builtin_amdgcn_s_setprio(2);
     static_assert(64 % DM == 0 && 64 % DS == 0, "the rings must divide a group of 64 steps");
     static_assert(!LT || (DM == 4 && DS == 16), "the LDS form's one wait per step assumes rings of 4 and 16");
-    constexpr int SW = 64 * NW * R;   // words per segment
+    constexpr int SW = 64 * NW * H;   // 32-bit words per segment
     __shared__ uint64_t hand[3][NW];
-    extern __shared__ uint64_t stab[];   // LT: the segment's table slice, row s at s * SW
+    extern __shared__ uint32_t stab[];   // LT: the segment's table slice, row s at s * SW
     __shared__ uint32_t ticket;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // The ticket is read into a scalar register, so the compiler sees a
@@ -226,26 +231,29 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
         const SegItem it = items[tk];
         const HyPair P = pairs[it.pair];
         const int seg = it.seg;
-        const int K = P.K, m = P.m;
-        const int nseg = (K + 64 * NW * R - 1) / (64 * NW * R);
+        const int m = P.m;
+        const int K32 = 2 * P.K;   // 32-bit words of the packed string
+        const int nseg = (K32 + SW - 1) / SW;
         const int Wg = NW * seg + wave;
         const int L = 64 * Wg + lane;   // global lane of the pair
-        const int w0 = L * R;
+        const int w0 = L * H;           // the lane's first 32-bit word
+        // the table as 32-bit words: row stride 2 ks
+        const uint32_t *tab32 = (const uint32_t *)(tab + P.tab_off);
+        const uint32_t ks2 = 2u * (uint32_t)P.ks;
         if (LT) {
             // the slice: rows 0 .. sigma (sigma: zeros), the segment's words
-            const uint64_t *src = tab + P.tab_off + (int64_t)seg * SW;
+            const uint32_t *src = tab32 + (int64_t)seg * SW;
             for (int idx = threadIdx.x; idx < (P.sigma + 1) * SW; idx += 64 * NW)
-                stab[idx] = src[(int64_t)(idx / SW) * P.ks + idx % SW];
+                stab[idx] = src[(int64_t)(idx / SW) * ks2 + idx % SW];
             __syncthreads();
         }
-        // the lane's masks of row code c: global table word c * ks + w0, or
-        // LDS slice word c * SW + (64 wave + lane) R
-        const char *tb = (const char *)(tab + P.tab_off) + (size_t)w0 * 8u;
-        const uint32_t ks8 = (uint32_t)P.ks * 8u;
-        const uint64_t *sl = stab + (64 * wave + lane) * R;
-        auto fetch = [&](uint32_t code, uint64_t (&mq)[R]) {
-            if (LT) load_masks<R>(sl + code * SW, mq);
-            else load_masks<R>(tb + code * ks8, mq);
+        // the lane's masks of row code c: global table word c * 2 ks + w0, or
+        // LDS slice word c * SW + (64 wave + lane) H
+        const uint32_t *tb = tab32 + w0;
+        const uint32_t *sl = stab + (64 * wave + lane) * H;
+        auto fetch = [&](uint32_t code, uint32_t (&mq)[H]) {
+            if (LT) load_masks<H>(sl + code * SW, mq);
+            else load_masks<H>(tb + code * ks2, mq);
         };
         // rl[t]: code of the row this lane meets at step t (row t - L)
         const uint32_t *rl = roff + P.roff_off - L;
@@ -258,9 +266,9 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
         const int ga = 2 * Wg, gb = 2 * Wg + (m + 62) / 64;
         uint64_t *xmine = xbuf + xoff[it.pair] + (int64_t)seg * ng * 2;
         const uint64_t *xprevseg = xbuf + xoff[it.pair] + (int64_t)(seg - 1) * ng * 2;
-        uint64_t V[R];
+        uint32_t V[H];
 #pragma unroll
-        for (int r = 0; r < R; ++r) V[r] = ~0ull;
+        for (int r = 0; r < H; ++r) V[r] = ~0u;
         // The previous segment's carry word of group gp sits in two tagged
         // entries (none outside the groups that segment ran: rows there lie
         // outside [0, m) and the word is 0).  Wave 0 issues the loads of
@@ -305,7 +313,7 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
         // the code of step j + DM.  Primed after the hand-off word's loads
         // above, so that a wait for those (at the first group) leaves the
         // rings' loads in flight.
-        uint64_t Mq[DM][R];
+        uint32_t Mq[DM][H];
         uint32_t So[DS];
         {
             const int ta = 64 * (ga - Wg);   // the wave's first step with a row of [0, m)
@@ -342,12 +350,12 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
                     // (LDS) and the code read DS steps ago (global); each step
                     // issues one read of each
                     if (LT) __builtin_amdgcn_s_waitcnt(kWaitVm15Lgkm3);
-                    uint64_t p[R], sum[R];
+                    uint32_t p[H], sum[H];
 #pragma unroll
-                    for (int r = 0; r < R; ++r) p[r] = Mq[d][r] & V[r];
-                    co = uniform64(add_words<R>(V, p, cm, sum));
+                    for (int r = 0; r < H; ++r) p[r] = Mq[d][r] & V[r];
+                    co = uniform64(add_words<H>(V, p, cm, sum));
 #pragma unroll
-                    for (int r = 0; r < R; ++r) V[r] = hy_update(V[r], sum[r], p[r]);
+                    for (int r = 0; r < H; ++r) V[r] = hy_update(V[r], sum[r], p[r]);
                     asm("v_writelane_b32 %0, %1, %2" : "+v"(vob) : "s"((uint32_t)(co >> 32)), "i"(j));
                     if (j < 63) cm = (co << 1) | ((in >> (j + 1)) & 1ull);
                     // refill: masks for step j + DM, the code for step j + DM + DS
@@ -382,7 +390,7 @@ __global__ __launch_bounds__(64 * NW) void hy_seg(const HyPair *__restrict__ pai
         }
         int z = 0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) z += w0 + r < K ? 64 - __popcll(V[r]) : 0;
+        for (int r = 0; r < H; ++r) z += w0 + r < K32 ? 32 - __popc(V[r]) : 0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
         if (lane == 0) atomicAdd(&llcs[P.out], z);   // each wave adds its own zero count
@@ -405,7 +413,7 @@ struct Hyyro {
     // segmented classes: (pair, segment) tickets for each segment width
     // (1 or 2 words per lane), exchange buffer (sized for the narrower
     // segments, which are more), control
-    std::vector<std::vector<hy::SegItem>> seg_items;   // [2 class + R - 1]
+    std::vector<std::vector<hy::SegItem>> seg_items;   // [kWidths class + width_index(H)]
     hy::SegItem *d_items = nullptr;
     std::vector<int64_t> item_off;
     int64_t *d_xoff = nullptr;          // per pair: its segments' exchange entries in d_xbuf
@@ -432,14 +440,17 @@ struct Hyyro {
 
 namespace {
 
-// Launch classes (length buckets) by packed width K (words): up to 64, 128,
-// 256, 2,048 and 16,384 words (2^20 columns).  Every class runs the systolic
-// kernel with 1 or 2 words per lane (R), one of them per class: the default
-// (R = 1) or the one sl_hyyro_autotune measured faster on the batch's own
-// pairs.  Waves per segment: the class's width over 64 R, so the classes up
-// to 2,048 words are one segment per pair (1, 2 and 4 waves at R = 1 up to
-// 64, 128 and 256 words); wider pairs are cut into segments of 4 waves, so a
-// long pair spreads over as many compute units as it has segments.
+// Launch classes (length buckets) by packed width K (64-bit words): up to
+// 64, 128, 256, 2,048 and 16,384 words (2^20 columns).  Every class runs the
+// systolic kernel with H 32-bit words per lane (the reference's analogue is
+// Hyyro's word type), one H per class: the default (H = 2, 64 bits) or the
+// one sl_hyyro_autotune measured fastest on the batch's own pairs.  Waves per
+// segment: the class's width over 64 H, so the classes up to 2,048 words are
+// one segment per pair (1, 2 and 4 waves at H = 2 up to 64, 128 and 256
+// words); wider pairs are cut into segments of 4 waves, so a long pair
+// spreads over as many compute units as it has segments.  H = 1 (half the
+// instructions per step, twice the lanes) is offered where it needs at most
+// 4 waves per pair.
 int hy_class(int K) {
     static const int limits[] = {64, 128, 256, 2048, 16384};
     int c = 0;
@@ -447,26 +458,30 @@ int hy_class(int K) {
     return c;
 }
 constexpr int kClasses = 5;
+constexpr int kWidths = 3;   // H = 1, 2, 4 32-bit words per lane
+int width_index(int H) { return H == 1 ? 0 : (H == 2 ? 1 : 2); }
 constexpr size_t kSegLdsBytes = 64 * 1024;   // largest table slice the kernel holds in LDS
 constexpr int kClassMaxWords[kClasses] = {64, 128, 256, 2048, 16384};
 constexpr int kFirstSegClass = 3;
 constexpr int kSegClasses = 2;   // classes 3 and 4: several segments per pair
 bool is_seg_class(int cls) { return cls >= kFirstSegClass && cls < kFirstSegClass + kSegClasses; }
-// words per lane each class offers, default first
-const std::vector<int> &class_wpl_options(int) {
-    static const std::vector<int> opts = {1, 2};
-    return opts;
+// 32-bit words per lane each class offers, default first
+const std::vector<int> &class_wpl_options(int cls) {
+    static const std::vector<int> wide = {2, 1, 4}, no_half = {2, 4};
+    return cls == 2 ? no_half : wide;
 }
-bool class_offers(int cls, int R) {
+bool class_offers(int cls, int H) {
     for (int r : class_wpl_options(cls))
-        if (r == R) return true;
+        if (r == H) return true;
     return false;
 }
 // waves per segment
-int class_waves(int cls, int R) {
+int class_waves(int cls, int H) {
     if (is_seg_class(cls)) return hy::kSegWaves;
-    return std::max(1, kClassMaxWords[cls] / (64 * R));
+    return std::max(1, 2 * kClassMaxWords[cls] / (64 * H));
 }
+// 64-bit words per segment
+int class_segment_words(int cls, int H) { return 64 * class_waves(cls, H) * H / 2; }
 
 void hy_free(Hyyro *h) {
     if (!h) return;
@@ -564,37 +579,38 @@ int hy_launch(Hyyro *h, unsigned class_mask = ~0u) {
     for (int c = kClasses - 1; c >= 0; --c) {
         if (!((class_mask >> c) & 1u)) continue;
         hipStream_t cs = h->lane_stream[is_seg_class(c) ? 0 : 1];
-        const int R = h->wpl[(size_t)c];
-        const size_t v = (size_t)(2 * c + R - 1);
+        const int H = h->wpl[(size_t)c];
+        const size_t v = (size_t)(kWidths * c + width_index(H));
         const uint32_t nit = (uint32_t)h->seg_items[v].size();
         if (!nit) continue;
         const hy::SegItem *items = h->d_items + h->item_off[v];
-        // one workgroup of NW waves per segment of 64 NW R words; enough
-        // persistent workgroups for about 8 waves per compute unit
-        const int NW = class_waves(c, R);
+        // one workgroup of NW waves per segment of 64 NW H 32-bit words;
+        // enough persistent workgroups for about 8 waves per compute unit
+        const int NW = class_waves(c, H);
         const unsigned blocks = (unsigned)std::min<int64_t>(nit, (int64_t)cus * 8 / NW);
         // the table slice in LDS when it fits in 64 KiB (segmented classes:
-        // alphabets up to 31 symbols at 1 word per lane, 15 at 2)
-        const size_t slice = (size_t)(h->cls_sigma[(size_t)c] + 1) * 64 * NW * R * 8;
+        // alphabets up to 63, 31 and 15 symbols at 32, 64 and 128 bits per lane)
+        const size_t slice = (size_t)(h->cls_sigma[(size_t)c] + 1) * 64 * NW * H * 4;
         const bool lds = slice <= kSegLdsBytes;
-#define SL_HY_SEG(R, DM, LT, NW)                                                                                  \
-    hipLaunchKernelGGL((hy::hy_seg<R, DM, 16, LT, NW>), dim3(blocks), dim3(64 * NW), LT ? slice : 0, cs, h->d_pairs, \
+#define SL_HY_SEG(H, DM, LT, NW)                                                                                  \
+    hipLaunchKernelGGL((hy::hy_seg<H, DM, 16, LT, NW>), dim3(blocks), dim3(64 * NW), LT ? slice : 0, cs, h->d_pairs, \
                        items, nit, h->d_xoff, h->d_roff, h->d_tab, h->d_xbuf, epoch, h->d_ctl + c, h->spin_limit,   \
                        h->d_llcs, is_seg_class(c) ? 1 : 0)
-#define SL_HY_SEG_NW(R, NW)                  \
+#define SL_HY_SEG_NW(H, NW)                  \
     do {                                     \
-        if (lds) SL_HY_SEG(R, 4, true, NW);  \
-        else SL_HY_SEG(R, 16, false, NW);    \
+        if (lds) SL_HY_SEG(H, 4, true, NW);  \
+        else SL_HY_SEG(H, 16, false, NW);    \
     } while (0)
-        if (R == 1) {
-            if (NW == 1) SL_HY_SEG_NW(1, 1);
-            else if (NW == 2) SL_HY_SEG_NW(1, 2);
-            else SL_HY_SEG_NW(1, 4);
-        } else {
-            if (NW == 1) SL_HY_SEG_NW(2, 1);
-            else if (NW == 2) SL_HY_SEG_NW(2, 2);
-            else SL_HY_SEG_NW(2, 4);
-        }
+#define SL_HY_SEG_H(H)                      \
+    do {                                    \
+        if (NW == 1) SL_HY_SEG_NW(H, 1);    \
+        else if (NW == 2) SL_HY_SEG_NW(H, 2); \
+        else SL_HY_SEG_NW(H, 4);            \
+    } while (0)
+        if (H == 1) SL_HY_SEG_H(1);
+        else if (H == 2) SL_HY_SEG_H(2);
+        else SL_HY_SEG_H(4);
+#undef SL_HY_SEG_H
 #undef SL_HY_SEG_NW
 #undef SL_HY_SEG
         SL_HIP_TRY(hipGetLastError());
@@ -678,7 +694,8 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
         // a segment's lanes past K read zero words, and the rows' codes
         // padded on both sides with sigma (the zero row)
         const int cls = hy_class(P.K);
-        const int segw = std::max(64 * class_waves(cls, 1), 128 * class_waves(cls, 2));
+        int segw = 1;
+        for (int H : class_wpl_options(cls)) segw = std::max(segw, class_segment_words(cls, H));
         P.ks = (P.K + segw - 1) / segw * segw;
         h->tab_words = (h->tab_words + 1) & ~(int64_t)1;   // 16-byte aligned tables
         P.tab_off = h->tab_words;
@@ -698,8 +715,8 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
     std::vector<hy::SegItem> all_items;
     std::vector<int64_t> xoff((size_t)std::max(npairs, 1), 0);
     int64_t xtotal = 0;
-    h->seg_items.assign(2 * kClasses, {});
-    h->item_off.assign(2 * kClasses, 0);
+    h->seg_items.assign(kWidths * kClasses, {});
+    h->item_off.assign(kWidths * kClasses, 0);
     for (int c = 0; c < kClasses; ++c) {
         auto lst = h->cls_list[(size_t)c];
         std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) {
@@ -707,21 +724,23 @@ int hy_create(Hyyro **out, int npairs, const void *a_cat, const int64_t *a_off, 
                    (int64_t)h->pairs[(size_t)y].m * h->pairs[(size_t)y].K;
         });
         if (is_seg_class(c))
-            for (int pi : lst) {   // exchange entries for the narrower (R = 1) segments, which are more
+            for (int pi : lst) {   // exchange entries for the narrowest (H = 1) segments, which are most
                 const hy::HyPair &P = h->pairs[(size_t)pi];
-                const int nseg = (P.K + 64 * hy::kSegWaves - 1) / (64 * hy::kSegWaves);
+                const int sw = class_segment_words(c, 1);
+                const int nseg = (P.K + sw - 1) / sw;
                 const int64_t ng = (P.m + 63) / 64 + 8;   // groups a segment runs (hy_seg: G1 - G0 + 1)
                 xoff[(size_t)pi] = xtotal;
                 xtotal += (int64_t)nseg * ng * 2;
             }
-        for (int R = 1; R <= 2; ++R) {
-            const int segw = 64 * class_waves(c, R) * R;
-            auto &items = h->seg_items[2 * (size_t)c + (size_t)(R - 1)];
+        for (int H : class_wpl_options(c)) {
+            const int segw = class_segment_words(c, H);
+            const size_t v = (size_t)(kWidths * c + width_index(H));
+            auto &items = h->seg_items[v];
             for (int pi : lst) {
                 const int nseg = (h->pairs[(size_t)pi].K + segw - 1) / segw;
                 for (int sg = 0; sg < nseg; ++sg) items.push_back(hy::SegItem{pi, sg});
             }
-            h->item_off[2 * (size_t)c + (size_t)(R - 1)] = (int64_t)all_items.size();
+            h->item_off[v] = (int64_t)all_items.size();
             all_items.insert(all_items.end(), items.begin(), items.end());
         }
     }

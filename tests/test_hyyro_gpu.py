@@ -1,7 +1,8 @@
 """GPU parity of the Hyyro bit-vector LLCS (sl_llcs_hyyro, sl_hyyro_*):
 scores equal plain LCS dynamic programming (oracle) across the launch
-classes -- the systolic kernel with 1 and 2 words per lane, one workgroup of
-1, 2 or 4 waves per pair up to 2,048 words and segments of 4 waves beyond,
+classes -- the systolic kernel with 32, 64 and 128 bits per lane, one
+workgroup of 1, 2 or 4 waves per pair up to 2,048 words and segments of 4
+waves beyond,
 its mask table in LDS for small alphabets and in global memory for large
 ones -- at word, lane, wave and segment boundaries, for 1-, 2- and 4-byte
 symbols; at full size they equal the semi-local kernel's LLCS.  Bar: exact
@@ -54,10 +55,10 @@ def test_wide_symbols_equal_dp(gpu, oracle, dtype, sigma):
 
 
 # the segmented form's mask table sits in LDS while (sigma + 1) rows of a
-# segment fit 64 KiB: sigma <= 31 at 1 word per lane, <= 15 at 2; larger
-# alphabets gather from the global table.  Pairs of 4 to 8 segments, both
-# sides of both limits, with the segment hand-off in play.
-@pytest.mark.parametrize("sigma", [15, 16, 31, 32, 200])
+# segment fit 64 KiB: sigma <= 63, 31 and 15 at 32, 64 and 128 bits per
+# lane; larger alphabets gather from the global table.  Pairs of 3 to 9
+# segments, both sides of every limit, with the segment hand-off in play.
+@pytest.mark.parametrize("sigma", [15, 16, 31, 32, 63, 64, 200])
 def test_segmented_pairs_both_table_forms_equal_dp(gpu, oracle, sigma):
     g = np.random.default_rng(100 + sigma)
     a = g.integers(0, sigma, 400).astype(np.uint8)
@@ -65,9 +66,9 @@ def test_segmented_pairs_both_table_forms_equal_dp(gpu, oracle, sigma):
     b[:sigma] = np.arange(sigma, dtype=np.uint8)               # every symbol occurs in the packed string
     want = oracle.llcs_dp(a, b)
     hb = gpu.HyyroBatch([(a, b), (b[:5000], a)])
-    for r in (1, 2):
+    for r in (1, 2, 4):
         hb.set_words_per_lane([0, 0, 0, r, 0])
-        print(f"hyyro segmented sigma={sigma} words per lane {r}", flush=True)
+        print(f"hyyro segmented sigma={sigma} 32-bit words per lane {r}", flush=True)
         got = hb.run()
         assert got[0] == want
         assert got[1] == oracle.llcs_dp(b[:5000], a)
@@ -95,9 +96,9 @@ def test_unsupported_inputs_are_rejected(gpu):
     assert gpu.llcs_hyyro(np.array([1, 2, 3], np.uint8), np.array([4, 5], np.uint8)) == 0
 
 
-# offered words per lane per bucket (packed width up to 64, 128, 256, 2,048
-# and 16,384 words), default first
-WPL_OPTIONS = [[1, 2], [1, 2], [1, 2], [1, 2], [1, 2]]
+# offered 32-bit words per lane per bucket (packed width up to 64, 128, 256,
+# 2,048 and 16,384 64-bit words), default first
+WPL_OPTIONS = [[2, 1, 4], [2, 1, 4], [2, 4], [2, 1, 4], [2, 1, 4]]
 
 
 def test_every_words_per_lane_form_equals_dp(gpu, oracle):
@@ -116,13 +117,13 @@ def test_every_words_per_lane_form_equals_dp(gpu, oracle):
     assert [b["pairs"] for b in b0] == [6, 3, 3, 4, 4]
     assert [b["words_per_lane"] for b in b0] == [o[0] for o in WPL_OPTIONS]
     for k in range(3):
-        wpl = [o[min(k, len(o) - 1)] for o in WPL_OPTIONS]
+        wpl = [o[min(k, len(o) - 1)] for o in WPL_OPTIONS]   # every form of every bucket
         hb.set_words_per_lane(wpl)
         assert [b["words_per_lane"] for b in hb.buckets()] == wpl
         print(f"hyyro words per lane {wpl}", flush=True)
         assert hb.run().tolist() == want
     with pytest.raises(gpu.SemiLocalError):
-        hb.set_words_per_lane([4, 0, 0, 0, 0])                # no bucket has a 4-word form
+        hb.set_words_per_lane([0, 0, 1, 0, 0])                # bucket 2 has no 32-bit form
     chosen = hb.autotune(rounds=1)
     assert all(c in o for c, o in zip(chosen, WPL_OPTIONS))
     assert [b["words_per_lane"] for b in hb.buckets()] == chosen
