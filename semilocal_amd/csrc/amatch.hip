@@ -248,6 +248,88 @@ static int bits_for(int64_t v) {   // bits to hold 0..v
     return b;
 }
 
+// Selection of the windows whose score reaches a threshold, in window
+// order: the duplicate searches only ever keep those (a few percent of the
+// windows), so only they cross to the host.  Blocks of kSelThreads threads
+// take kSelPer consecutive windows per thread; a count pass, one scan of the
+// block counts, and a write pass that places each selected window at its
+// block's offset plus its rank in the block.
+constexpr int kSelThreads = 256, kSelPer = 4, kSelBlock = kSelThreads * kSelPer;
+
+// exclusive prefix of v over the block's threads; *total receives the sum
+__device__ __forceinline__ int sel_block_scan(int v, int *tmp /* kSelThreads / 64 */, int *total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[wave] = x;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < kSelThreads / 64; ++k) {
+        before += k < wave ? tmp[k] : 0;
+        all += tmp[k];
+    }
+    *total = all;
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(kSelThreads) void am_sel_count(const int32_t *__restrict__ ws, int nw, double kdi,
+                                                             int32_t *__restrict__ cnt) {
+    __shared__ int tmp[kSelThreads / 64];
+    const int64_t i0 = (int64_t)blockIdx.x * kSelBlock + (int64_t)threadIdx.x * kSelPer;
+    int c = 0;
+#pragma unroll
+    for (int e = 0; e < kSelPer; ++e) c += (i0 + e < nw && (double)ws[i0 + e] >= kdi) ? 1 : 0;
+    int total = 0;
+    (void)sel_block_scan(c, tmp, &total);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = total;
+}
+
+// one block: cnt[0 .. nb) -> exclusive offsets, cnt[nb] = the total
+__global__ __launch_bounds__(kSelThreads) void am_sel_scan(int32_t *__restrict__ cnt, int nb) {
+    __shared__ int tmp[kSelThreads / 64];
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kSelThreads) {   // uniform
+        const int b = b0 + (int)threadIdx.x;
+        const int v = b < nb ? cnt[b] : 0;
+        int total = 0;
+        const int ex = sel_block_scan(v, tmp, &total);
+        if (b < nb) cnt[b] = carry + ex;
+        carry += total;
+        __syncthreads();   // tmp is reused by the next chunk
+    }
+    if (threadIdx.x == 0) cnt[nb] = carry;
+}
+
+// selected window i -> (i, best start, best end, best score) at its rank
+__global__ __launch_bounds__(kSelThreads) void am_sel_write(const int32_t *__restrict__ ws,
+                                                             const int32_t *__restrict__ bs,
+                                                             const int32_t *__restrict__ be,
+                                                             const int32_t *__restrict__ bsc, int nw, double kdi,
+                                                             const int32_t *__restrict__ off, int4 *__restrict__ out) {
+    __shared__ int tmp[kSelThreads / 64];
+    const int64_t i0 = (int64_t)blockIdx.x * kSelBlock + (int64_t)threadIdx.x * kSelPer;
+    bool sel[kSelPer];
+    int c = 0;
+#pragma unroll
+    for (int e = 0; e < kSelPer; ++e) {
+        sel[e] = i0 + e < nw && (double)ws[i0 + e] >= kdi;
+        c += sel[e] ? 1 : 0;
+    }
+    int total = 0;
+    int pos = off[blockIdx.x] + sel_block_scan(c, tmp, &total);
+#pragma unroll
+    for (int e = 0; e < kSelPer; ++e)
+        if (sel[e]) {
+            const int64_t i = i0 + e;
+            out[pos++] = make_int4((int)i, bs[i], be[i], bsc[i]);
+        }
+}
+
 }  // namespace am
 
 // ---------------------------------------------------------------------------
@@ -271,6 +353,11 @@ struct AMatch {
     int32_t *d_out = nullptr;   // window score, best start, best end, best score (4 x cap)
     int32_t *h_out = nullptr;   // pinned host copy of d_out
     int64_t out_cap = 0;
+    // the windows a duplicate search keeps: block counts / offsets, the
+    // selected (window, best start, best end, best score), pinned copies
+    int32_t *d_selcnt = nullptr, *h_seltotal = nullptr;
+    int4 *d_sel = nullptr, *h_sel = nullptr;
+    int64_t sel_cap = 0;
     std::vector<am::AmItem> h_items;   // work list of the last (w, L) launched
     int items_w = -1, items_L = -1;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -284,6 +371,10 @@ static void amatch_free(AMatch *a) {
     (void)hipFree(a->d_items);
     (void)hipFree(a->d_out);
     if (a->h_out) (void)hipHostFree(a->h_out);
+    (void)hipFree(a->d_selcnt);
+    (void)hipFree(a->d_sel);
+    if (a->h_seltotal) (void)hipHostFree(a->h_seltotal);
+    if (a->h_sel) (void)hipHostFree(a->h_sel);
     for (auto &e : a->ev)
         if (e) (void)hipEventDestroy(e);
     if (a->batch) batch_destroy_internal(a->batch);
@@ -435,6 +526,59 @@ private:
     }
 };
 
+// Runs the window kernel, selects on the device the windows whose score
+// reaches kdi (in window order), and copies them -- and, with `with_scores`,
+// every window's score -- to pinned host memory: *sel, *nsel, *scores.
+static int amatch_select_host(AMatch *a, int w, int L, double kdi, bool with_scores, const int4 **sel, int *nsel,
+                              const int32_t **scores) {
+    int rc = amatch_launch(a, w, L);
+    if (rc) return rc;
+    const int nw = a->n - w + 1;
+    const int nb = (nw + am::kSelBlock - 1) / am::kSelBlock;
+    if (a->sel_cap < a->out_cap) {
+        (void)hipFree(a->d_selcnt);
+        (void)hipFree(a->d_sel);
+        if (a->h_seltotal) (void)hipHostFree(a->h_seltotal);
+        if (a->h_sel) (void)hipHostFree(a->h_sel);
+        a->d_selcnt = a->h_seltotal = nullptr;
+        a->d_sel = a->h_sel = nullptr;
+        a->sel_cap = 0;
+        const int64_t nbcap = (a->out_cap + am::kSelBlock - 1) / am::kSelBlock + 1;
+        SL_HIP_TRY(hipMalloc((void **)&a->d_selcnt, sizeof(int32_t) * (size_t)nbcap));
+        SL_HIP_TRY(hipMalloc((void **)&a->d_sel, sizeof(int4) * (size_t)a->out_cap));
+        // cached pinned memory: the host reads it only after the stream has synced
+        SL_HIP_TRY(hipHostMalloc((void **)&a->h_seltotal, sizeof(int32_t), hipHostMallocNonCoherent));
+        SL_HIP_TRY(hipHostMalloc((void **)&a->h_sel, sizeof(int4) * (size_t)a->out_cap, hipHostMallocNonCoherent));
+        a->sel_cap = a->out_cap;
+    }
+    const int64_t cap = a->out_cap;
+    const int32_t *o = a->d_out;
+    hipLaunchKernelGGL(am::am_sel_count, dim3((unsigned)nb), dim3(am::kSelThreads), 0, a->stream, o, nw, kdi,
+                       a->d_selcnt);
+    hipLaunchKernelGGL(am::am_sel_scan, dim3(1), dim3(am::kSelThreads), 0, a->stream, a->d_selcnt, nb);
+    hipLaunchKernelGGL(am::am_sel_write, dim3((unsigned)nb), dim3(am::kSelThreads), 0, a->stream, o, o + cap,
+                       o + 2 * cap, o + 3 * cap, nw, kdi, a->d_selcnt, a->d_sel);
+    SL_HIP_TRY(hipGetLastError());
+    SL_HIP_TRY(hipMemcpyAsync(a->h_seltotal, a->d_selcnt + nb, sizeof(int32_t), hipMemcpyDeviceToHost, a->stream));
+    if (with_scores)
+        SL_HIP_TRY(hipMemcpyAsync(a->h_out, o, sizeof(int32_t) * (size_t)nw, hipMemcpyDeviceToHost, a->stream));
+    SL_HIP_TRY(hipStreamSynchronize(a->stream));
+    const int total = *a->h_seltotal;
+    if (total < 0 || total > nw) {
+        set_error("sl_duplicate_search: window selection count out of range");
+        return kErrHip;
+    }
+    if (total > 0) {
+        SL_HIP_TRY(hipMemcpyAsync(a->h_sel, a->d_sel, sizeof(int4) * (size_t)total, hipMemcpyDeviceToHost,
+                                  a->stream));
+        SL_HIP_TRY(hipStreamSynchronize(a->stream));
+    }
+    *sel = a->h_sel;
+    *nsel = total;
+    *scores = with_scores ? a->h_out : nullptr;
+    return kOk;
+}
+
 // DuplicateSearch::phase3 (ApproximateMatching.h:212-228): sort by start, then
 // score descending (std::sort, so the input order decides among equal keys),
 // then keep clones that start strictly right of the previous kept start and
@@ -479,15 +623,17 @@ static int duplicate_search(AMatch *a, double k, int variant) {
     }
     int rc = amatch_check_window(a, w, L, "sl_duplicate_search");
     if (rc) return rc;
-    const int32_t *h = nullptr;
+    // Only the windows whose score reaches kdi are kept by either search;
+    // they come from the device already selected, in window order.  The
+    // semi-sparse search also walks the scores of the windows it skips over.
+    const int4 *sel = nullptr;
+    const int32_t *ws = nullptr;
+    int nsel = 0;
     const auto t0 = std::chrono::steady_clock::now();
-    rc = amatch_windows_host(a, w, L, &h);
+    rc = amatch_select_host(a, w, L, kdi, variant == 1, &sel, &nsel, &ws);
     if (rc) return rc;
     const auto t1 = std::chrono::steady_clock::now();
-    const int64_t cap = a->out_cap;
-    const int32_t *ws = h, *bs = h + cap, *be = h + 2 * cap, *bsc = h + 3 * cap;
     const int n = a->n;
-    const int nw = n - w + 1;
     KeySet seen;
     std::vector<AmInterval> set_w2;
     if (variant == 0) {
@@ -495,10 +641,13 @@ static int duplicate_search(AMatch *a, double k, int variant) {
         // windows from the right end of the text to the left; a window whose
         // own score reaches kdi contributes its best substring once; then the
         // list is reversed and passed to phase 3.
-        for (int i = nw - 1; i >= 0; --i) {
-            const double window_dist = ws[i];
-            if (window_dist >= kdi && seen.insert(interval_key(bs[i], be[i])))
-                set_w2.push_back(AmInterval{bs[i], be[i], (double)bsc[i]});
+        long long last = -1;   // the previous window's key: neighbours mostly share one
+        for (int k = nsel - 1; k >= 0; --k) {
+            const int4 e = sel[k];   // (window, best start, best end, best score)
+            const long long key = interval_key(e.y, e.z);
+            if (key == last) continue;
+            last = key;
+            if (seen.insert(key)) set_w2.push_back(AmInterval{e.y, e.z, (double)e.w});
         }
         std::reverse(set_w2.begin(), set_w2.end());
     } else {
@@ -517,9 +666,23 @@ static int duplicate_search(AMatch *a, double k, int variant) {
                 ++end;
             }
         }
-        // phase 2: each kept window's best substring, once
-        for (int i : set_w1)
-            if (seen.insert(interval_key(bs[i], be[i]))) set_w2.push_back(AmInterval{bs[i], be[i], (double)bsc[i]});
+        // phase 2: each kept window's best substring, once; the kept windows
+        // are selected ones, in increasing order, so one pass over the
+        // selection finds them
+        int k = 0;
+        long long last = -1;
+        for (int i : set_w1) {
+            while (k < nsel && sel[k].x < i) ++k;
+            if (k >= nsel || sel[k].x != i) {
+                set_error("sl_duplicate_search: a kept window is missing from the device selection");
+                return kErrHip;
+            }
+            const int4 e = sel[k];
+            const long long key = interval_key(e.y, e.z);
+            if (key == last) continue;
+            last = key;
+            if (seen.insert(key)) set_w2.push_back(AmInterval{e.y, e.z, (double)e.w});
+        }
     }
     phase3(set_w2, a->results);
     const auto t2 = std::chrono::steady_clock::now();
