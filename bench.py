@@ -473,7 +473,7 @@ def bench_hyyro(S, W, pairs=None):
     create_s = time.perf_counter() - t0
     scores = hb.run()
     default_ms = hb.time(warmup=1, iters=5)
-    # words per lane per length bucket: the autotuner times every offered
+    # lane width (32-bit words per lane) per length bucket: the autotuner times every offered
     # form on each bucket's own pairs and keeps the fastest
     t0 = time.perf_counter()
     hb.autotune(rounds=3)
