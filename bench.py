@@ -625,6 +625,13 @@ def bench_binary(S, W, golden):
     ms = bp.time(warmup=1, iters=3)
     bp.close()
     rec = golden["binary_pair_2p20"]
+    # the same score by Hyyro's recurrence (any alphabet), lane width autotuned
+    hb = S.HyyroBatch([(a, b)])
+    hb.autotune(rounds=1)
+    hy_wpl = [bk["words_per_lane"] for bk in hb.buckets() if bk["pairs"]]
+    hy_llcs = int(hb.run()[0])
+    hy_ms = hb.time(warmup=1, iters=3)
+    hb.close()
     t = time.perf_counter()
     perm = S.semi_local_comb(a, b)
     pms = (time.perf_counter() - t) * 1e3
@@ -634,6 +641,8 @@ def bench_binary(S, W, golden):
     bt.close()
     return {"workload": "binary pair, |a| = |b| = 2^20 (seed 2)",
             "bitparallel_ms": round(ms, 3), "bitparallel_gcups": round(cells / (ms * 1e-3) / 1e9, 1),
+            "hyyro_score_ms": round(hy_ms, 3), "hyyro_words_per_lane": hy_wpl,
+            "hyyro_score_equal": hy_llcs == rec["llcs_bitparallel"],
             "permutation_ms": round(kms, 2), "permutation_gcups": round(cells / (kms * 1e-3) / 1e9, 1),
             "permutation_ms_incl_transfers": round(pms, 1),
             "bitparallel_roofline": valu_roofline(cells * BITPAR_OPS_PER_BIT_CELL, ms, "0.93 per bit-cell"),
