@@ -220,6 +220,18 @@ int sl_bitpar_run(sl_bitpar *plan);                     /* asynchronous */
 int sl_bitpar_result(sl_bitpar *plan, int *llcs);      /* waits, then reads the score */
 int sl_bitpar_time(sl_bitpar *plan, int warmup, int iters, double *ms_per_run);
 void sl_bitpar_destroy(sl_bitpar *plan);
+/* The kernel a plan runs (same score either way).  No counterpart in the     */
+/* reference, whose one routine is the combing:                              */
+/*   SL_BITPAR_CARRY   -- the Hyyro kernel (sl_hyyro_*) on the two full       */
+/*                        strings, padding included: one step per symbol of   */
+/*                        the shorter, the longer bit-packed.  The default    */
+/*                        where the longer string has at most 2^20 symbols;   */
+/*   SL_BITPAR_COMBING -- sliding-diagonal combing by formula 2 (any size).   */
+/* result() reads the form of the last run.                                  */
+#define SL_BITPAR_CARRY 0
+#define SL_BITPAR_COMBING 1
+int sl_bitpar_set_form(sl_bitpar *plan, int form);   /* SL_ERR_ARG if the form is not available */
+int sl_bitpar_get_form(sl_bitpar *plan, int *form);
 
 /* ------------------------------------------------------------------------ */
 /* Unit-Monge (sticky braid) multiplication R = P (.) Q of n x n permutations  */

@@ -5,12 +5,12 @@ the C ABI in include/semilocal_hip.h, plus the C++ drop-in strategies in
 include/semilocal_hip.hpp).  This package builds it and exposes a thin
 Python binding used by the tests and the benchmark.
 """
-from ._lib import (AMatch, Batch, BitParallel, BraidIndex, Hybrid, HyyroBatch, SemiLocalError, braid_multiply,
+from ._lib import (AMatch, BITPAR_CARRY, BITPAR_COMBING, Batch, BitParallel, BraidIndex, Hybrid, HyyroBatch, SemiLocalError, braid_multiply,
                    braid_multiply_batch, braid_multiply_time, comb_batch, device_info, duplicate_search,
                    hybrid_autotune_blocks, hybrid_bucket_blocks, hybrid_iterative, lib, llcs_bitparallel, llcs_hyyro, plan_queue, semi_local_comb,
                    staggered_multiply, Server)
 
-__all__ = ["AMatch", "Batch", "BitParallel", "BraidIndex", "Hybrid", "HyyroBatch", "SemiLocalError",
+__all__ = ["AMatch", "BITPAR_CARRY", "BITPAR_COMBING", "Batch", "BitParallel", "BraidIndex", "Hybrid", "HyyroBatch", "SemiLocalError",
            "braid_multiply", "braid_multiply_batch", "braid_multiply_time", "comb_batch", "device_info",
            "duplicate_search", "hybrid_autotune_blocks", "hybrid_bucket_blocks", "hybrid_iterative", "lib", "llcs_bitparallel", "llcs_hyyro", "plan_queue",
            "semi_local_comb", "staggered_multiply", "Server", "llcs_from_kernel"]

@@ -101,6 +101,8 @@ SIGNATURES = {
     "sl_bitpar_result": ([_vp, C.POINTER(C.c_int)], C.c_int),
     "sl_bitpar_time": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "sl_bitpar_destroy": ([_vp], None),
+    "sl_bitpar_set_form": ([_vp, C.c_int], C.c_int),
+    "sl_bitpar_get_form": ([_vp, C.POINTER(C.c_int)], C.c_int),
     "sl_comb_batch": ([C.c_int, _vp, _i64p, _i32p, _vp, _i64p, _i32p, C.c_int, _vp, _vp], C.c_int),
     "sl_braid_multiply": ([_vp, _vp, C.c_int, _vp], C.c_int),
     "sl_braid_multiply_batch": ([C.c_int, _vp, _vp, _vp, _vp], C.c_int),
@@ -572,16 +574,35 @@ def pack_bits(bits) -> np.ndarray:
     return np.ascontiguousarray(np.packbits(padded.reshape(words, 64), axis=1, bitorder="little").view("<u8").astype(np.uint64).ravel())
 
 
-class BitParallel:
-    """Resident bit-parallel LLCS problem (sl_bitpar_* entry points)."""
+BITPAR_CARRY, BITPAR_COMBING = 0, 1   # SL_BITPAR_CARRY, SL_BITPAR_COMBING
 
-    def __init__(self, a_bits, b_bits):
+
+class BitParallel:
+    """Resident bit-parallel LLCS problem (sl_bitpar_* entry points).
+
+    `form`: BITPAR_CARRY (Hyyro's carry recurrence with the packed bits as
+    masks; the default where it fits) or BITPAR_COMBING (sliding-diagonal
+    combing).  Both return the same score."""
+
+    def __init__(self, a_bits, b_bits, form=None):
         self.ar = pack_bits_reverse(a_bits)
         self.bw = pack_bits(b_bits)
         h = _vp()
         check(lib().sl_bitpar_create(C.byref(h), _ptr(self.ar), len(self.ar), _ptr(self.bw), len(self.bw),
                                      len(a_bits)))
         self.h = h
+        if form is not None:
+            self.form = form
+
+    @property
+    def form(self) -> int:
+        out = C.c_int()
+        check(lib().sl_bitpar_get_form(self.h, C.byref(out)))
+        return out.value
+
+    @form.setter
+    def form(self, value: int):
+        check(lib().sl_bitpar_set_form(self.h, int(value)))
 
     def llcs(self) -> int:
         out = C.c_int()

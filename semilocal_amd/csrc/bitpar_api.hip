@@ -1,6 +1,7 @@
 // bitpar_api.hip -- host side of kernel (b): resident bit-parallel problems.
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "../../include/semilocal_hip.h"
 #include "common.h"
@@ -10,12 +11,20 @@ namespace slhip {
 hipError_t launch_bitpar(const BitDesc &d, unsigned waves, LaunchCtl *ctl, uint32_t spin_limit, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t stream);
 
+// The carry form runs the Hyyro kernel on the two full strings (every symbol
+// of the packed words, padding included): the longer one bit-packed, which
+// the kernel takes up to 2^20 symbols.
+constexpr int64_t kCarryMaxColumns = 1 << 20;
+
 struct BitPlan {
     BitDesc d{};
     int64_t a_total = 0;
     uint32_t *d_words = nullptr;   // a_rev (M words) then b (N words), 32-bit view
     uint64_t *d_buf = nullptr;
     uint32_t *d_scratch = nullptr; // [0,2): LaunchCtl, [2]: left strands leaving at the bottom
+    int form = SL_BITPAR_COMBING;  // the form the next run uses
+    int run_form = SL_BITPAR_COMBING;   // the form of the last run: the one result() reads
+    sl_hyyro *carry = nullptr;     // the carry form's handle (null: the form is not offered)
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     uint32_t spin_limit = 1u << 23;
@@ -28,6 +37,7 @@ static void bitplan_free(BitPlan *p) {
     (void)hipFree(p->d_words);
     (void)hipFree(p->d_buf);
     (void)hipFree(p->d_scratch);
+    if (p->carry) sl_hyyro_destroy(p->carry);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -39,6 +49,8 @@ static int bitplan_run(BitPlan *p) {
         set_error("bit-parallel plan is missing a device buffer (internal defect); nothing launched");
         return kErrHip;
     }
+    p->run_form = p->form;
+    if (p->form == SL_BITPAR_CARRY) return sl_hyyro_run(p->carry);
     // unique tags for this run's hand-offs; tag 0 is a cleared buffer
     if (next_run_tag(p->cleared_at, &p->d.epoch))
         SL_HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(p->d_buf), 0u, 2u * (uint64_t)p->d.N, p->stream));
@@ -48,6 +60,15 @@ static int bitplan_run(BitPlan *p) {
 }
 
 static int bitplan_result(BitPlan *p, int *llcs) {
+    if (p->run_form == SL_BITPAR_CARRY) {
+        // the LLCS of the two full packed strings is the number of left
+        // strands leaving at the bottom
+        int32_t full = 0;
+        const int rc = sl_hyyro_result(p->carry, &full);
+        if (rc) return rc;
+        if (llcs) *llcs = (int)(p->a_total - 32 * (int64_t)p->d.M + full);
+        return kOk;
+    }
     SL_HIP_TRY(hipStreamSynchronize(p->stream));
     uint32_t h[4];
     SL_HIP_TRY(hipMemcpy(h, p->d_scratch, sizeof h, hipMemcpyDeviceToHost));
@@ -106,7 +127,62 @@ int sl_bitpar_create(sl_bitpar **out, const uint64_t *a_rev, int a_words, const 
     p->d.npasses = (int32_t)((M + kBitRowLanes - 1) / kBitRowLanes);
     p->d.epoch = 0;
     p->waves = (unsigned)std::min<int64_t>(p->d.npasses, (int64_t)device_cus() * 16);
+    // Carry form: the full strings as symbols (a_rev word M-1-g holds a's
+    // symbols 32g .. 32g+31, symbol 32g+i at bit 31-i; b word c holds b's
+    // symbols 32c .. 32c+31, symbol 32c+j at bit j).  The rows are the
+    // shorter string, the bit vector the longer; 32 bits per lane where the
+    // rows outnumber the vector's 32-bit words (a lone long chain of steps
+    // runs faster on the narrower lanes), the kernel's default otherwise.
+    if (std::max(M, N) * 32 <= kCarryMaxColumns) {
+        const uint32_t *ar32 = reinterpret_cast<const uint32_t *>(a_rev);
+        const uint32_t *b32 = reinterpret_cast<const uint32_t *>(b);
+        std::vector<uint8_t> sa((size_t)(32 * M)), sb((size_t)(32 * N));
+        for (int64_t g = 0; g < M; ++g)
+            for (int i = 0; i < 32; ++i) sa[(size_t)(32 * g + i)] = (uint8_t)(ar32[M - 1 - g] >> (31 - i) & 1u);
+        for (int64_t c = 0; c < N; ++c)
+            for (int j = 0; j < 32; ++j) sb[(size_t)(32 * c + j)] = (uint8_t)(b32[c] >> j & 1u);
+        const bool rows_a = M <= N;
+        const std::vector<uint8_t> &rows = rows_a ? sa : sb, &cols = rows_a ? sb : sa;
+        const int64_t off = 0;
+        const int32_t lr = (int32_t)rows.size(), lc = (int32_t)cols.size();
+        int rc2 = sl_hyyro_create(&p->carry, 1, rows.data(), &off, &lr, cols.data(), &off, &lc, 1);
+        if (!rc2 && (int64_t)lr >= (int64_t)lc / 32) {
+            const int32_t wpl[SL_HYYRO_BUCKETS] = {1, 1, 0, 1, 1};   // bucket 2 offers 2 or 4 only
+            rc2 = sl_hyyro_set_words_per_lane(p->carry, wpl);
+        }
+        if (rc2) {
+            bitplan_free(p);
+            return rc2;
+        }
+        p->form = SL_BITPAR_CARRY;
+    }
     *out = reinterpret_cast<sl_bitpar *>(p);
+    return kOk;
+}
+
+int sl_bitpar_set_form(sl_bitpar *h, int form) {
+    BitPlan *p = reinterpret_cast<BitPlan *>(h);
+    if (!p || (form != SL_BITPAR_CARRY && form != SL_BITPAR_COMBING)) {
+        set_error("sl_bitpar_set_form: bad argument");
+        return kErrArg;
+    }
+    if (form == SL_BITPAR_CARRY && !p->carry) {
+        set_error("sl_bitpar_set_form: the carry form needs the longer string within 2^20 symbols");
+        return kErrArg;
+    }
+    // each form has its own buffers and stream; result() reads the form of
+    // the last run, so a switch between a run and its result is harmless
+    p->form = form;
+    return kOk;
+}
+
+int sl_bitpar_get_form(sl_bitpar *h, int *form) {
+    BitPlan *p = reinterpret_cast<BitPlan *>(h);
+    if (!p || !form) {
+        set_error("sl_bitpar_get_form: null argument");
+        return kErrArg;
+    }
+    *form = p->form;
     return kOk;
 }
 
@@ -131,6 +207,10 @@ int sl_bitpar_time(sl_bitpar *h, int warmup, int iters, double *ms_per_run) {
     if (!p || iters <= 0 || warmup < 0 || !ms_per_run) {
         set_error("sl_bitpar_time: bad argument");
         return kErrArg;
+    }
+    if (p->form == SL_BITPAR_CARRY) {
+        p->run_form = SL_BITPAR_CARRY;
+        return sl_hyyro_time(p->carry, warmup, iters, ms_per_run);
     }
     int rc = kOk;
     for (int w = 0; w < warmup && !rc; ++w) rc = bitplan_run(p);
