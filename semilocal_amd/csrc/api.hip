@@ -342,6 +342,18 @@ static int cell_ops(const PairDesc &d, int R) {
 }
 static int64_t passes_at(const PairDesc &d, int R) { return ((int64_t)d.m + 64LL * R - 1) / (64LL * R); }
 static double step_ops(const PairDesc &d, int R) { return cell_ops(d, R) * (double)R + kStepOverheadOps; }
+// Tile height of the last pass at main height R: the smallest of 4, 8, 16
+// (no larger than R) whose 64 x height rows cover the rows left for it.
+static int last_height(const PairDesc &d, int R) {
+    const int64_t rest = (int64_t)d.m - (passes_at(d, R) - 1) * 64LL * R;
+    for (int h : kTileHeights)
+        if (h <= R && 64LL * h >= rest) return h;
+    return R;
+}
+// Instructions of the pair's passes at main height R, the last at its own.
+static double pair_ops(const PairDesc &d, int R) {
+    return ((double)(passes_at(d, R) - 1) * step_ops(d, R) + step_ops(d, last_height(d, R))) * ((double)d.n + 63.0);
+}
 // Cycles from the start of the pair's first pass to the end of its last: the
 // last pass starts (P - 1) lags after the first and runs n + 64 steps.
 static double chain_cycles(const PairDesc &d, int R, double lag, double cycles_per_op) {
@@ -354,9 +366,10 @@ static double chain_cycles(const PairDesc &d, int R, double lag, double cycles_p
 //    - a batch whose passes at R = 16 cannot give every SIMD two waves is
 //      latency-bound: one height for all from model_rows_per_lane;
 //    - otherwise each pair takes the height with the fewest instructions for
-//      its padded rows, ceil(m / 64R) passes x (n + 63) steps x step_ops(R)
-//      (a pair of 600 rows wastes 40 % of a 1,024-row pass, 15 % of two
-//      512-row passes), and the batch's modelled throughput time is those
+//      its padded rows: ceil(m / 64R) passes x (n + 63) steps x step_ops,
+//      the last pass at the smallest height that covers its rows (a pair of
+//      1,100 rows at R = 16: a pass of 1,024 rows and one of 256, not two of
+//      1,024), and the batch's modelled throughput time is those
 //      instructions x 4 cycles / (SIMDs x kValuEfficiency);
 //    - a pair whose chain of passes at that height and the bulk lag would
 //      take longer than critical_fraction x that time is a critical chain:
@@ -395,7 +408,7 @@ static void assign_tiles(Batch *bt) {
         if (d.m <= 0 || d.n <= 0) continue;
         double best = 1e300;
         for (int R : kTileHeights) {
-            const double c = (double)passes_at(d, R) * ((double)d.n + 63.0) * step_ops(d, R);
+            const double c = pair_ops(d, R);
             if (c < best) {
                 best = c;
                 d.rows_per_lane = R;
@@ -516,6 +529,8 @@ static void plan_items(Batch *bt) {
         PairDesc &d = bt->pairs[i];
         const int64_t rpp = 64LL * d.rows_per_lane;
         d.npasses = (d.m > 0 && d.n > 0) ? (int32_t)((d.m + rpp - 1) / rpp) : 0;
+        // a launch instantiated for one height runs every pass at it
+        d.last_rows = bt->launch_rows != 0 || d.npasses == 0 ? d.rows_per_lane : last_height(d, d.rows_per_lane);
         d.item0 = (int32_t)bt->items.size();
         for (int p = 0; p < d.npasses; ++p) bt->items.push_back(WorkItem{i, p});
     }

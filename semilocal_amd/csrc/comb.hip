@@ -153,7 +153,9 @@ __device__ __attribute__((always_inline)) void comb_pass(const PairDesc &d, int 
     const bool last = pass == d.npasses - 1;
     const uint32_t want_tag = strand_tag(epoch, (uint32_t)pass - 1u);   // entries written by pass-1
     const uint64_t my_tag = (uint64_t)strand_tag(epoch, (uint32_t)pass) << 32;
-    const int64_t row0 = (int64_t)pass * 64 * R + (int64_t)lane * R;
+    // passes before the last hold 64 d.rows_per_lane rows each; the last may
+    // run at a smaller height R (the fewest rows that cover the rest)
+    const int64_t row0 = (int64_t)pass * 64 * d.rows_per_lane + (int64_t)lane * R;
     // values of padding rows / absent columns: never equal to anything
     const uint32_t pad_row = MODE == kCellCode ? 0x10000u : 0u;
     const uint32_t pad_col = MODE == kCellOneHot ? 0u : MODE == kCellTable ? (uint32_t)d.sigma * 256u : 0x20000u;
@@ -344,10 +346,10 @@ __device__ __attribute__((always_inline)) void comb_pass_cells(const PairDesc &d
 }  // namespace
 
 // Persistent queue consumer: every wave takes (pair, pass) tickets until the
-// queue is empty.  RSET = 4, 8 or 16: every pair of the launch has that tile
+// queue is empty.  RSET = 4, 8 or 16: every pass of the launch has that tile
 // height (one instantiation per height); RSET = 0: each pair has its own
-// (d.rows_per_lane, any of 4, 8, 16), dispatched per ticket like the cell
-// mode.  A pair marked `prio` (a chain of passes on the batch's critical path)
+// (d.rows_per_lane, any of 4, 8, 16; its last pass d.last_rows, no larger),
+// dispatched per ticket like the cell mode.  A pair marked `prio` (a chain of passes on the batch's critical path)
 // runs its passes at raised wave priority, so on a SIMD shared with bulk
 // passes its instructions issue first.  wait_flags (one word per queue item,
 // or null) carry the wait feedback between consecutive passes of a pair.
@@ -369,8 +371,10 @@ __global__ __launch_bounds__(256) void comb_batch_kernel(const PairDesc *__restr
         if (tk >= nitems) break;     // every wave leaves once the queue is empty
         const WorkItem it = items[tk];
         const PairDesc d = pairs[it.pair];
-        const int R = d.rows_per_lane;
-        if (RSET != 0 ? R != RSET : (R != 4 && R != 8 && R != 16)) {
+        const int Rm = d.rows_per_lane;
+        const int R = it.pass == d.npasses - 1 ? d.last_rows : Rm;
+        const bool height_ok = (Rm == 4 || Rm == 8 || Rm == 16) && (R == 4 || R == 8 || R == 16) && R <= Rm;
+        if (RSET != 0 ? (R != RSET || Rm != RSET) : !height_ok) {
             // a descriptor this instantiation cannot comb (a planning defect):
             // reported, never run with the wrong row layout
             if (lane == 0) __hip_atomic_fetch_or(&ctl->error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -53,6 +53,7 @@ struct PairDesc {
     int32_t item0;      // queue index of pass 0 of this pair
     int32_t npasses;    // ceil(m / rows_per_pass)
     int32_t rows_per_lane;  // R: rows held by one lane (rows_per_pass = 64 R)
+    int32_t last_rows;  // tile height of the last pass (<= R: the fewest rows that cover the rest)
     int32_t cell;       // how the cell tests symbols: kCellCode, kCellOneHot or kCellTable
     int32_t tag_shift;  // bits of the largest strand id (one-hot tags start here)
     int32_t sigma;      // distinct symbols of the pair (dense codes 0..sigma-1)

@@ -156,6 +156,7 @@ int sl_semi_local_comb_noalloc(const void *d_a, int m, const void *d_b, int n, i
     d.tag_shift = 0;
     const int R = model_rows_one_pair(d.m, d.n, device_cus());
     d.rows_per_lane = R;
+    d.last_rows = R;   // one tile height for every pass (the launch's instantiation)
     d.npasses = (int32_t)((d.m + 64 * R - 1) / (64 * R));
     d.item0 = 0;
     const unsigned waves = (unsigned)std::min<int64_t>(d.npasses, (int64_t)device_cus() * 8);
