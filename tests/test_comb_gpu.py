@@ -112,6 +112,36 @@ def test_ragged_batch_4096_matches_reference_fixture(gpu):
     b.close()
 
 
+def test_last_pass_heights_match_full_height_passes(gpu, oracle):
+    """Per-pair heights run a pair's last pass at the smallest height that
+    covers its rows.  1,024 pairs whose row counts sit just past a pass
+    boundary at each height (64 R k + r) -- enough passes for the per-pair
+    plan -- give the same permutations as every pass at R = 16, and a sample
+    matches the oracle."""
+    g = np.random.default_rng(31)
+    rests = [1, 2, 63, 64, 65, 127, 255, 256, 257, 300, 511, 512, 513, 700, 1023]
+    pairs = []
+    for i in range(1024):
+        m = 1024 * int(g.integers(1, 3)) + rests[i % len(rests)]
+        n = m + int(g.integers(0, 200))
+        a = g.integers(0, 20, m).astype(np.uint8)
+        b = g.integers(0, 20, n).astype(np.uint8)
+        pairs.append((a, b) if i % 2 else (b, a))   # either string as the rows' side
+    per_pair = gpu.Batch(pairs)
+    assert per_pair.tiles()[0] == 0
+    rows, _ = per_pair.pair_tiles()
+    assert (rows == 16).sum() >= 100, np.bincount(rows)   # pairs whose last pass can be smaller
+    got = per_pair.run().results()
+    full = gpu.Batch(pairs, rows_per_lane=16)
+    want = full.run().results()
+    bad = [i for i, (x, y) in enumerate(zip(got, want)) if not np.array_equal(x, y)]
+    assert not bad, bad[:10]
+    for i in g.choice(len(pairs), 8, replace=False):
+        assert np.array_equal(got[i], oracle.comb(*pairs[i])), i
+    per_pair.close()
+    full.close()
+
+
 def test_autotune_and_tiles_do_not_change_results(gpu, oracle):
     g = np.random.default_rng(15)
     pairs = [(g.integers(0, 20, int(g.integers(1, 4000))).astype(np.uint8),
