@@ -26,6 +26,11 @@ def test_binary_pair_2p20_permutation_and_score(gpu):
     llcs = gpu.llcs_from_kernel(perm, len(a), len(b))
     assert llcs == rec["llcs"] == rec["llcs_bitparallel"]
     assert gpu.llcs_bitparallel(a, b) == rec["llcs_bitparallel"]
+    # the hybrid's block grids the bench chooses among: the same permutation
+    for sm, sn in [(1, 2), (2, 1)]:
+        h = gpu.Hybrid(a, b, sm, sn)
+        _check(h.run(), rec)
+        h.close()
 
 
 def test_similar_pair_1e6_direct_combing_equals_reference_hybrid(gpu):
