@@ -582,7 +582,7 @@ def bench_hybrid(S, W, golden, grid_autotune: bool = True):
         return out
     # block split per length bucket: the grid autotuner times every candidate
     # grid on a pair of the bucket's length and keeps the fastest for it
-    grids = [(4, 4), (8, 8), (16, 16), (8, 16), (16, 8), (32, 32)]
+    grids = [(1, 1), (1, 2), (2, 1), (2, 2), (4, 4), (8, 8), (16, 16), (8, 16), (16, 8), (32, 32)]
     by_bucket = []
     t0 = time.perf_counter()
     for length in (100_000, 300_000, 1_000_000):
