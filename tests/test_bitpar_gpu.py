@@ -99,3 +99,37 @@ def test_carry_form_refused_past_its_length_limit(gpu):
     with pytest.raises(gpu.SemiLocalError):
         plan.form = 7
     plan.close()
+
+
+def test_padding_bits_are_symbols_in_both_forms(gpu, oracle):
+    """The reference combs every bit of the packed words and returns
+    a_total_symbols - (left strands leaving right), so padding bits past
+    a_total are symbols too: the score is a_total - 64 words(a) + LLCS of the
+    two full packed strings.  Random (non-zero) padding, a_total below 64
+    words(a), both forms, through the C ABI."""
+    import ctypes as C
+
+    from semilocal_amd._lib import pack_bits, pack_bits_reverse
+    lib = gpu.lib()
+    g = np.random.default_rng(41)
+    for aw, bw, cut in [(1, 1, 1), (1, 3, 40), (3, 1, 100), (5, 7, 257), (7, 5, 300)]:
+        A = g.integers(0, 2, 64 * aw).astype(np.uint8)   # full strings, padding included
+        B = g.integers(0, 2, 64 * bw).astype(np.uint8)
+        a_total = 64 * aw - cut % 64 - 1
+        ar, bp = pack_bits_reverse(A), pack_bits(B)
+        want = a_total - 64 * aw + oracle.llcs_dp(A, B)
+        if aw <= bw:   # the shapes the reference's diagonal loops accept: its restated combing agrees
+            assert oracle.L.orc_bitpar_llcs(ar, aw, bp, bw, a_total, 1) == want
+        for form in FORMS:
+            h = C.c_void_p()
+            assert lib.sl_bitpar_create(C.byref(h), ar.ctypes.data_as(C.c_void_p), aw,
+                                        bp.ctypes.data_as(C.c_void_p), bw, a_total) == 0
+            assert lib.sl_bitpar_set_form(h, form) == 0
+            out = C.c_int()
+            assert lib.sl_bitpar_run(h) == 0 and lib.sl_bitpar_result(h, C.byref(out)) == 0
+            lib.sl_bitpar_destroy(h)
+            assert out.value == want, (aw, bw, a_total, form)
+        one = C.c_int()
+        assert lib.sl_llcs_bitparallel(ar.ctypes.data_as(C.c_void_p), aw, bp.ctypes.data_as(C.c_void_p), bw,
+                                       a_total, 1, 0, C.byref(one)) == 0
+        assert one.value == want
