@@ -67,11 +67,6 @@ int sl_probe_copy(int64_t bytes, double *ms);
 /*   semi_local::OpenMPIterativeCombing::compute -- SemiLocalLCS.h:238-244;   */
 /*   semi_local::SimpleIterativeCombing::compute -- SemiLocalLCS.h:68.         */
 /* Host buffers; col_to_row may be NULL.  m or n may be 0.                    */
-/* A large near-square pair (shorter side >= 16,384, sides within 4x) is      */
-/* combed as blocks merged by steady-ant levels (sl_hybrid_iterative): the    */
-/* grid this process tuned for the pair's length bucket, else 2 x 2, 4 x 4    */
-/* (longer side >= 24,576) or 16 x 16 (>= 150,000).  The permutation is the   */
-/* same (it is unique); only the time changes.                                */
 /* ------------------------------------------------------------------------ */
 int sl_semi_local_comb(const void *a, int m, const void *b, int n, int sym_bytes,
                        int32_t *row_to_col, int32_t *col_to_row);

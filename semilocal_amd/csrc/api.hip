@@ -1270,42 +1270,11 @@ int sl_comb_batch(int npairs, const void *a_cat, const int64_t *a_off, const int
     return rc;
 }
 
-// Block grid of the one-pair entry point: the grid this process measured for
-// the pair's length bucket (sl_hybrid_autotune_blocks), else a rule from
-// measurements on MI355X (square-ish random pairs, one launch vs the hybrid):
-// 10k one launch 0.79 ms (2 x 2: 0.82); 20k 2 x 2 1.29 ms (one launch 1.57);
-// 30k-100k 4 x 4 1.5-3.8 ms (2.3-9.0); 200k-2^20 16 x 16 9.7-122 ms
-// (15.8-131).  A pair whose shorter side is under 16,384, or whose sides
-// differ by more than 4x, is combed in one launch.
-static void one_pair_grid(int m, int n, int32_t *sm, int32_t *sn) {
-    *sm = *sn = 1;
-    const int lo = std::min(m, n), hi = std::max(m, n);
-    if (lo < 16384 || hi > 4LL * lo) return;
-    int32_t tm = 0, tn = 0;
-    if (sl_hybrid_bucket_blocks(m, n, &tm, &tn) == kOk && tm > 0 && tn > 0) {
-        *sm = tm;
-        *sn = tn;
-        return;
-    }
-    const int g = hi < 24576 ? 2 : (hi < 150000 ? 4 : 16);
-    *sm = *sn = g;
-}
-
 int sl_semi_local_comb(const void *a, int m, const void *b, int n, int sym_bytes, int32_t *row_to_col,
                        int32_t *col_to_row) {
     if (m < 0 || n < 0 || (m > 0 && !a) || (n > 0 && !b) || (!row_to_col && m + n > 0)) {
         set_error("sl_semi_local_comb: bad argument");
         return kErrArg;
-    }
-    // A large pair is combed as blocks in one launch and merged by steady-ant
-    // levels (the hybrid): the same permutation, the block chains shorter.
-    int32_t sm = 1, sn = 1;
-    one_pair_grid(m, n, &sm, &sn);
-    if (sm * sn > 1) {
-        const int rc = sl_hybrid_iterative(a, m, b, n, sym_bytes, sm, sn, row_to_col);
-        if (rc || !col_to_row) return rc;
-        for (int i = 0; i < m + n; ++i) col_to_row[row_to_col[i]] = i;
-        return kOk;
     }
     static const uint8_t dummy = 0;
     const int64_t zero = 0;

@@ -78,24 +78,6 @@ def test_reference_sweep_naive_vs_openmp_shapes(gpu, oracle):
         assert np.array_equal(got, oracle.comb(x, y))
 
 
-def test_one_pair_entry_blocks_large_pairs(gpu, oracle):
-    """The one-pair entry point combs a large near-square pair as blocks merged
-    by steady-ant levels (2 x 2 from a shorter side of 16,384, 4 x 4 past a
-    longer side of 24,576); the permutation and its inverse equal one launch,
-    and the oracle where it is quick.  A pair with sides 5x apart stays one
-    launch."""
-    g = np.random.default_rng(47)
-    for m, n, check_oracle in [(20000, 18000, True), (30000, 40000, False), (17000, 90000, False)]:
-        a = g.integers(0, 4, m).astype(np.uint8)
-        b = g.integers(0, 4, n).astype(np.uint8)
-        rc, cr = gpu.semi_local_comb(a, b, with_col_to_row=True)
-        direct = gpu.comb_batch([(a, b)])[0]
-        assert np.array_equal(rc, direct), (m, n)
-        assert np.array_equal(cr[rc], np.arange(m + n)), (m, n)
-        if check_oracle:
-            assert np.array_equal(rc, oracle.comb(a, b)), (m, n)
-
-
 def test_plain_pair_10k_matches_reference_fixture(gpu):
     c = G.configs()["plain_pair_10k"]
     a, b = W.plain_pair()

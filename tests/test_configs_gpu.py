@@ -20,10 +20,9 @@ def test_binary_pair_2p20_permutation_and_score(gpu):
     rec = G.configs()["binary_pair_2p20"]
     a, b = W.binary_pair()
     assert W.sha256(a, b) == rec["input_sha256"]
-    perm = gpu.semi_local_comb(a, b)                                 # the one-pair entry: blocks
+    perm = gpu.semi_local_comb(a, b)
     assert np.array_equal(np.sort(perm), np.arange(len(perm)))      # a permutation
     _check(perm, rec)
-    _check(gpu.comb_batch([(a, b)])[0], rec)                         # one launch, no blocks
     llcs = gpu.llcs_from_kernel(perm, len(a), len(b))
     assert llcs == rec["llcs"] == rec["llcs_bitparallel"]
     assert gpu.llcs_bitparallel(a, b) == rec["llcs_bitparallel"]
@@ -40,9 +39,8 @@ def test_similar_pair_1e6_direct_combing_equals_reference_hybrid(gpu):
     rec = G.configs()["similar_pair_1e6"]
     a, b = W.similar_pair()
     assert W.sha256(a, b) == rec["input_sha256"]
-    perm = gpu.comb_batch([(a, b)])[0]                               # one launch, no blocks
+    perm = gpu.semi_local_comb(a, b)
     _check(perm, rec)
-    _check(gpu.semi_local_comb(a, b), rec)                           # the one-pair entry: blocks
     assert gpu.llcs_from_kernel(perm, len(a), len(b)) == rec["llcs"]
 
 

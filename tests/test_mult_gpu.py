@@ -156,7 +156,7 @@ def test_hybrid_graph_replays_equal_the_eager_run(gpu):
     a = g.integers(0, 4, size=30000).astype(np.uint8)
     b = g.integers(0, 4, size=26000).astype(np.uint8)
     eager = gpu.Hybrid(a, b, 4, 4).run()
-    assert np.array_equal(eager, gpu.comb_batch([(a, b)])[0])          # one launch, no blocks
+    assert np.array_equal(eager, gpu.semi_local_comb(a, b))
     h = gpu.Hybrid(a, b, 4, 4)
     h.set_graph(True)
     for k in range(4):
@@ -178,7 +178,7 @@ def test_hybrid_autotune_keeps_results_and_recaptures_the_graph(gpu):
     g = _rng(41)
     a = g.integers(0, 4, size=20000).astype(np.uint8)
     b = g.integers(0, 4, size=17000).astype(np.uint8)
-    want = gpu.comb_batch([(a, b)])[0]                          # one launch, no blocks
+    want = gpu.semi_local_comb(a, b)
     h = gpu.Hybrid(a, b, 4, 2)
     h.set_graph(True)
     assert np.array_equal(h.run(), want)                      # graph captured at the default shape
@@ -206,9 +206,6 @@ def test_block_grid_autotuner_picks_a_candidate_per_length_bucket(gpu):
     assert ms[grids.index(best)] == min(ms)
     assert gpu.hybrid_bucket_blocks(len(a), len(b)) == best
     assert gpu.hybrid_bucket_blocks(33000, 20000) == best       # the same bucket (2^15 .. 2^16 - 1)
-    direct = gpu.comb_batch([(a, b)])[0]                        # one launch, no blocks
-    assert np.array_equal(gpu.hybrid_iterative(a, b, *best), direct)
-    # the one-pair entry point now combs this pair at the tuned grid: the same
-    assert np.array_equal(gpu.semi_local_comb(a, b), direct)
+    assert np.array_equal(gpu.hybrid_iterative(a, b, *best), gpu.semi_local_comb(a, b))
     with pytest.raises(gpu.SemiLocalError):
         gpu.hybrid_autotune_blocks(a, b, [(0, 2)])
